@@ -1,0 +1,174 @@
+"""Benchmark: Newton steps/s of the batched Monte Carlo ring-oscillator engine.
+
+Workload: 4096 CMOS ring oscillators, stage counts odd and uniform on
+[11, 201], per-device Gaussian VTO (sigma 30 mV) and KP (sigma 5 %) from a
+per-instance seed, .TRAN 0-20 ns with a 10 ps maximum step, FP64 throughout.
+A "step" is one batched Newton iteration: one launch in which every active
+instance evaluates its MOSFETs, stamps, refactors and solves (or, when its
+previous update converged, runs its step control).  ``value`` counts the
+Newton iterations (eval + stamp + LU + solve) actually completed per second
+over the timed launches, all instances together, inputs resident in HBM.
+
+Also reported: the same engine on one instance of the 101-stage ring
+(single-instance FP64 GPU path) and the ratio, the roofline of the batched
+kernel, and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(circuits_list):
+    """Bytes one batched Newton iteration must move (all instances active):
+    per MOSFET 3 FP64 parameters + class id + stamp mask (36 B), the 6 store
+    values at t_n and at the last iterate and the 8 state values at t_n read
+    (160 B), 14 store/state values written (112 B); per unknown x read and
+    written, F/Q/B written, two charge histories read (56 B); per Jacobian
+    entry the two linear-part values read (16 B)."""
+    import xyce_amd as X
+    total = 0
+    cache = {}
+    for c in circuits_list:
+        key = (len(c.mos_nodes), c.n_nodes)
+        if key not in cache:
+            hp = X.host_pattern(c)
+            cache[key] = (len(c.mos_nodes), hp["n"], hp["nnz"])
+        nm, n, nnz = cache[key]
+        total += nm * (36 + 160 + 112) + n * 56 + nnz * 16
+    return total
+
+
+def cpu_baseline(circuits_list, sample, tstop, tstep, hmax, threads):
+    import ctypes as C
+    from oracle.pyoracle import Oracle, lib
+    orcs = [Oracle(c) for c in circuits_list[:sample]]
+    L = lib()
+    L.orc_tran_many.argtypes = [C.c_int, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p]
+    arr = (C.c_void_p * len(orcs))(*[o.h.value for o in orcs])
+    out = np.zeros(4)
+    L.orc_tran_many(len(orcs), arr, tstop, tstep, hmax, threads, out.ctypes.data_as(C.c_void_p))
+    return dict(newton_iters=out[0], seconds=out[1], steps=out[2], failures=out[3])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-tstop", type=float, default=2e-9)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the batched engine shards "
+              "by instance with no exchange, so N GPUs give N independent runs", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import xyce_amd as X
+    from xyce_amd import circuits as C
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    tstop, tstep, hmax = 20e-9, 1e-12, 10e-12
+
+    t0 = time.time()
+    ckts = C.mc_ring_batch(args.instances, seed=3)
+    log(f"generated {len(ckts)} circuits in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    b = X.Batch(ckts, tstop, tstep, hmax)
+    log(f"batch setup {time.time() - t0:.1f}s")
+    tune = b.autotune()
+    log(f"autotune: {tune}")
+    b.advance(args.warmup, graph_chunk=50)
+    c0 = b.counters()
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    chunk = 50 if args.steps % 50 == 0 else 0
+    b.advance(args.steps, graph_chunk=chunk)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    c1 = b.counters()
+    dev_s = c1["seconds"]
+    iters = c1["newton_iters"] - c0["newton_iters"]
+    loads = c1["loads"] - c0["loads"]
+    value = iters / wall
+    log(f"timed {args.steps} launches: wall {wall*1e3:.2f} ms, events {dev_s*1e3:.2f} ms, "
+        f"{iters:.0f} Newton iterations, {loads:.0f} loads, done {c1['done']}, failed {c1['failed']}, "
+        f"dense {c1['dense_solves'] - c0['dense_solves']:.0f}")
+
+    # single-instance FP64 GPU path: the same engine on the 101-stage ring
+    r1 = C.ring(101, seed=1)
+    b1 = X.Batch([r1], 50e-9, tstep, hmax)
+    b1.advance(args.warmup, graph_chunk=50)
+    s0 = b1.counters()
+    torch.cuda.synchronize()
+    w1 = time.perf_counter()
+    b1.advance(args.steps, graph_chunk=chunk)
+    torch.cuda.synchronize()
+    wall1 = time.perf_counter() - w1
+    s1 = b1.counters()
+    single = (s1["newton_iters"] - s0["newton_iters"]) / wall1
+    log(f"single instance: {single:.1f} Newton steps/s ({wall1 / args.steps * 1e6:.1f} us/launch)")
+
+    byts = algorithmic_bytes(ckts)
+    us_launch = dev_s / args.steps * 1e6
+    achieved = byts / (dev_s / args.steps) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_batch_step", "bytes_per_launch": int(byts), "us_per_launch": round(us_launch, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        r = cpu_baseline(ckts, args.cpu_sample, args.cpu_tstop, tstep, hmax, threads)
+        cpu = {"value": round(r["newton_iters"] / r["seconds"], 1), "unit": "Newton steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"first {args.cpu_sample} instances of the batch, .TRAN 0-{args.cpu_tstop * 1e9:g} ns each, "
+                         f"one instance per thread at a time ({r['newton_iters']:.0f} Newton iterations in "
+                         f"{r['seconds']:.1f} s)"}
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "Newton steps/sec (eval+stamp+LU) per batch; x vs plain FP64 GPU at RELTOL",
+        "value": round(value, 1),
+        "unit": "Newton steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "x_vs_single_instance_gpu": round(value / single, 2) if single > 0 else None,
+        "single_instance_gpu": round(single, 1),
+        "dtype": "f64",
+        "data": "synthetic (seeded Monte Carlo ring oscillators)",
+        "config": {"workload": f"{args.instances} Monte Carlo CMOS ring oscillators, stages odd U[11,201], "
+                               "level-1 MOSFETs, VTO sigma 30 mV, KP sigma 5%, .TRAN 0-20ns hmax 10ps, FP64",
+                   "instances": args.instances, "newton_iterations_timed": int(iters),
+                   "autotune": tune, "parallelism": "one workgroup per instance"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,194 @@
+/*
+ * xyce_amd — MI355X-native per-Newton hot path of a Xyce transient run.
+ *
+ * C ABI of libxyce_amd.so.  Plain pointers and sizes only; no exceptions cross
+ * it; every entry point returns an int status:
+ *    0            success
+ *   >0            singular pivot: (pivot index + 1)      (linear entry points)
+ *   <0            error (see XG_E_* below)
+ * One handle owns one HIP stream; calls on a handle must be serialised by the
+ * caller (as Xyce's single-threaded DeviceMgr / Linear::Solver calls are).
+ *
+ * Reference interfaces replaced (paths relative to the Xyce source root):
+ *   device side   Device::updateState / loadDAEVectors / loadDAEMatrices /
+ *                 isConverged   src/DeviceModelPKG/Core/N_DEV_Device.h:312-531,
+ *                 MOSFET level 1 Master  src/DeviceModelPKG/OpenModels/N_DEV_MOSFET1.C:4241,4343,4587
+ *   linear side   Linear::Solver::doSolve(reuse_factors, transpose)
+ *                 src/LinearAlgebraServicesPKG/N_LAS_Solver.h:95-107 (KLU via
+ *                 N_LAS_AmesosSolver.C:216-478)
+ */
+#ifndef XYCE_AMD_H
+#define XYCE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XG_OK 0
+#define XG_E_ARG (-1)        /* bad argument / shape */
+#define XG_E_HIP (-2)        /* HIP runtime error */
+#define XG_E_SETUP (-3)      /* circuit setup failed (e.g. effective length <= 0) */
+#define XG_E_NAN (-99)       /* NaN in the solution (N_LAS_AmesosSolver.C:415-424) */
+#define XG_E_NOGPU (-10)     /* no HIP device */
+
+/* Device types in netlist order (dev_type[]). */
+#define XG_DEV_MOS1 0
+#define XG_DEV_RES 1
+#define XG_DEV_CAP 2
+#define XG_DEV_VSRC 3
+#define XG_DEV_ISRC 4
+
+/* Waveform types. DC: [v]; PULSE: [V1 V2 TD TR TF PW PER] (NaN = default);
+ * PWL: [t0 v0 t1 v1 ...] — src/DeviceModelPKG/Core/N_DEV_SourceData.C:1168,1770 */
+#define XG_WAVE_DC 0
+#define XG_WAVE_PULSE 1
+#define XG_WAVE_PWL 2
+
+/* Number of MOSFET level-1 .model parameters per card and instance parameters
+ * per device (order: see xyce_amd/circuits.py MODEL_PARAMS / INST_PARAMS). */
+#define XG_NMP 30
+#define XG_NIP 11
+
+/*
+ * Circuit description: a netlist as flat arrays.  Node 0 is ground.
+ * Replaces the device/topology construction of Topology::addDevice
+ * (src/TopoManagerPKG/N_TOP_Topology.C:191) for the supported device types.
+ */
+typedef struct xg_circuit_desc {
+  int n_nodes;
+  int n_dev;            const int* dev_type;   const int* dev_idx;   /* netlist order */
+  int n_models;         const double* model_par;                    /* n_models * XG_NMP */
+  int n_mos;            const int* mos_nodes;  const int* mos_model; const double* mos_inst; /* 4, 1, XG_NIP per device */
+  int n_res;            const int* res_nodes;  const double* res_val;
+  int n_cap;            const int* cap_nodes;  const double* cap_val;
+  int n_vsrc;           const int* vsrc_nodes; const int* vsrc_wave;
+  int n_isrc;           const int* isrc_nodes; const int* isrc_wave;
+  int n_wave;           const int* wave_type;  const int* wave_off;  const double* wave_data;
+  double gmin;          /* DeviceOptions gmin, default 1e-12 (N_DEV_DeviceOptions.C:90) */
+  double temp;          /* Kelvin, default 300.15 */
+} xg_circuit_desc;
+
+/* Analysis flags of SolverState::setupSolverInfo (N_DEV_SolverState.C:292-430). */
+typedef struct xg_solver_state {
+  int dcop, tran, init_jct, init_fix;
+  int newton_iter;      /* 0 on the first load of a Newton solve */
+  int order;            /* integration order 1 or 2 (OneStep) */
+  double h;             /* current time step */
+  double time;          /* t_{n+1}: time at which sources are evaluated */
+  double tstep, tstop;  /* .TRAN step / stop (PULSE defaults) */
+} xg_solver_state;
+
+typedef struct xg_circuit xg_circuit;
+typedef struct xg_lu xg_lu;
+typedef struct xg_batch xg_batch;
+
+/* ---------------- circuit (device side) ---------------- */
+
+/* Builds the circuit on the current HIP device: model/instance setup
+ * (MOSFET1 Model::processParams :3939, Instance::processParams :383,
+ * updateTemperature :3487), node ordering and CSR pattern
+ * (SerialLSUtil::setupNodeGIDs / setupSolnAndStateGIDs / generateRowColData,
+ * N_TOP_SerialLSUtil.C:468,504,620) and the stamp gather maps. */
+int xg_circuit_create(const xg_circuit_desc* desc, xg_circuit** out);
+int xg_circuit_destroy(xg_circuit* c);
+
+/* sizes[0..5] = n (unknowns), nnz, n_state, n_store, n_mos, n_nodes */
+int xg_circuit_sizes(const xg_circuit* c, int* sizes);
+
+/* The Jacobian CSR pattern (rowptr n+1, colind nnz), node -> solution index
+ * (-1 = ground), per-MOSFET solution indices (6: d g s b d' s'; n = ground),
+ * F/Q stamp slots (17 and 14 per MOSFET; -1 = ground dummy, as
+ * EpetraMatrix::operator() N_LAS_EpetraMatrix.C:658), state/store offsets. */
+int xg_circuit_pattern(const xg_circuit* c, int* rowptr, int* colind, int* node_gid, int* mos_lid,
+                       int* mos_fslot, int* mos_qslot, int* mos_stasto);
+
+int xg_set_solver_state(xg_circuit* c, const xg_solver_state* ss);
+
+/* a2: DeviceMgr::updateState -> MOSFET1 Master::updateState (N_DEV_MOSFET1.C:4241).
+ * Host arrays in Xyce layout: x[n], state/store vectors as laid out by
+ * xg_circuit_pattern's mos_stasto.  *limited = !allDevicesConverged. */
+int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
+                    double* sta_next, double* sto_next, int* limited);
+
+/* a5+a7: DeviceMgr::loadDAEVectors (N_DEV_MOSFET1.C:4343) plus the linear
+ * devices' separateLoad contribution (N_LOA_CktLoader.C:700-787).  Uses the
+ * device state of the last xg_update_state.  Each output has n entries. */
+int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp);
+
+/* a6+a7: DeviceMgr::loadDAEMatrices (N_DEV_MOSFET1.C:4587) plus the linear
+ * part (FilteredMatrix::addToMatrix N_LAS_FilteredMatrix.C:632). nnz each. */
+int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals);
+
+/* a8: OneStep::obtainResidual / obtainJacobian (N_TIA_OneStep.C:219,469)
+ * on the vectors/matrices of the last loads.  q0/q2 = qHistory[0]/[2]. */
+int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals);
+
+/* ---------------- sparse LU (linear side, a9) ---------------- */
+
+/* Symbolic analysis once (AmesosSolver first call: KLU analyse, BTF + fill
+ * ordering, N_LAS_AmesosSolver.C:335).  values may be NULL; if given they
+ * steer the transversal toward large entries. */
+int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* values, xg_lu** out);
+int xg_lu_destroy(xg_lu* lu);
+/* Numeric refactor with the symbolic pivot order (KLU NumericFactorization,
+ * :363). Returns >0 (pivot+1) if a pivot is zero/non-finite or fails the
+ * threshold test; the caller then re-runs xg_lu_symbolic with values. */
+int xg_lu_numeric(xg_lu* lu, const double* vals);
+/* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
+int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
+/* info[0..3] = nnz(L+U), number of factor levels, number of solve levels, flops per factorization */
+int xg_lu_info(const xg_lu* lu, double* info);
+
+/* ---------------- device-resident Newton / transient ---------------- */
+
+/* Runs a full transient (DC operating point, then OneStep trapezoid with
+ * DampedNewton) for one circuit with every Newton step on the GPU
+ * (eval, stamp, combine, LU refactor, solve, update, norms), driven from the
+ * host one Newton iteration at a time.  stats[0..7] = accepted steps,
+ * rejected steps, Newton iterations, residual loads, seconds, status,
+ * recorded points, mean Newton-iteration time (s). */
+int xg_tran_run(xg_circuit* c, double tstop, double tstep, double hmax, int record, double* stats);
+int xg_tran_points(xg_circuit* c, double* t, double* x);
+
+/* ---------------- batched Monte Carlo engine ---------------- */
+
+/* Builds a batch of independent circuits (each evaluated, stamped, factored
+ * and solved by one workgroup), one launch per Newton iteration of every
+ * instance, with per-instance DCOP + OneStep step control on the GPU. */
+int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, double tstep, double hmax,
+                    int record_stride, xg_batch** out);
+int xg_batch_destroy(xg_batch* b);
+/* Launches `launches` batched Newton iterations (captured in a hipGraph of
+ * `graph_chunk` launches when graph_chunk > 0).  Returns when done. */
+int xg_batch_advance(xg_batch* b, int launches, int graph_chunk);
+/* counters[0..8] = Newton iterations (solves), residual loads, accepted
+ * steps, rejected steps, instances finished, instances failed, launches,
+ * device time of the last xg_batch_advance (s, HIP events), solves that
+ * used the partial-pivoting path (DC iterations and threshold failures) */
+int xg_batch_counters(xg_batch* b, double* counters);
+/* Per-instance state: time, h, order, mode (0 DCOP, 1 tran, 2 done, <0 failed). */
+int xg_batch_instance(xg_batch* b, int inst, double* tstate, double* x);
+/* Recorded accepted points of one instance: t[max], x[max*n]; returns count. */
+int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x);
+/* Autotuner: times candidate block sizes / instance orders on the real batch
+ * and keeps the fastest; result[0..2] = block size, order mode, us/launch. */
+int xg_batch_autotune(xg_batch* b, double* result);
+
+/* ---------------- host-only introspection (no GPU needed) ---------------- */
+
+/* The setup xg_circuit_create runs on the host (pattern, node order, stamp
+ * slots), returned without creating device state.  Arrays as in
+ * xg_circuit_pattern; sizes as in xg_circuit_sizes.  No reference counterpart:
+ * for checking the pattern on machines without a GPU. */
+int xg_host_pattern(const xg_circuit_desc* desc, int* sizes, int* rowptr, int* colind, int* node_gid, int* mos_lid,
+                    int* mos_fslot, int* mos_qslot, int* mos_stasto);
+/* The symbolic analysis of xg_lu_symbolic on the host: info as xg_lu_info;
+ * B = A(perm_row, perm_col); L+U pattern of B (lu_colind filled if it fits in
+ * max_lu entries).  No reference counterpart (KLU's analyse is internal). */
+int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double* values, double* info,
+                       int* perm_row, int* perm_col, int* lu_rowptr, int* lu_colind, int max_lu);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

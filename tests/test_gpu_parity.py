@@ -1,0 +1,140 @@
+"""GPU parity: the HIP path against the CPU oracle through the C ABI.
+
+Bars:
+  * pattern, node order and stamp slots: bit-exact;
+  * F, Q, B, limiter vectors, dFdx, dQdx, J, RHS entries: |gpu - oracle| <=
+    1e-12 * (magnitude of the entry's terms) + 1e-30;
+  * linear solves: relative residual <= 1e-12 and agreement with numpy;
+  * transient waveforms: within RELTOL*|V| + ABSTOL of the oracle's
+    (RELTOL 1e-3, ABSTOL 1e-6, plus the interpolation error between the two
+    step sequences, bounded by the LTE tolerance).
+"""
+import numpy as np
+import pytest
+
+import xyce_amd as X
+from xyce_amd import circuits as C
+from oracle.pyoracle import Oracle
+from tests.helpers import entry_tol, waveform_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _loaded_state(o, seed, tran=True):
+    """Oracle DC point + a perturbed transient state (store/state from a
+    previous load) so limiting, both MOSFET modes and all regions occur."""
+    st, x0, sta0, sto0, Q, F, B = o.dcop()
+    assert st > 0
+    rng = np.random.default_rng(seed)
+    x = x0 + rng.normal(0.0, 0.4, size=o.n)
+    return x, sta0, sto0
+
+
+@pytest.mark.parametrize("stages,seed", [(3, 0), (11, 1), (101, 2)])
+def test_pattern_bit_exact(gpu, stages, seed):
+    c = C.ring(stages, seed=seed, mc_seed=seed)
+    d = X.DeviceMgr(c)
+    o = Oracle(c)
+    pg, po = d.pattern(), o.pattern()
+    for k in ("rowptr", "colind", "node_gid", "mos_lid", "mos_fslot", "mos_qslot", "mos_stasto"):
+        assert np.array_equal(pg[k], po[k]), k
+
+
+@pytest.mark.parametrize("stages,seed,newton_iter,order", [(5, 3, 0, 1), (11, 4, 1, 2), (101, 5, 2, 2)])
+def test_load_parity(gpu, stages, seed, newton_iter, order):
+    c = C.ring(stages, seed=seed, mc_seed=seed)
+    d = X.DeviceMgr(c)
+    o = Oracle(c)
+    x, sta0, sto0 = _loaded_state(o, seed)
+    kw = dict(dcop=0, tran=1, init_jct=0, newton_iter=newton_iter, order=order, h=7e-12, time=0.3e-9,
+              tstep=1e-12, tstop=5e-9)
+    ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    d.set_solver_state(**kw)
+    sta_n, sto_n, lim = d.updateState(x, sta0, sto0, sto_next=sto0)
+    assert lim == ref["limited"]
+    assert entry_tol(sta_n, ref["sta_next"], np.abs(ref["sta_next"]) + 1e-300) <= 1
+    assert entry_tol(sto_n, ref["sto_next"], np.abs(ref["sto_next"]) + 1e-300) <= 1
+    vec = d.loadDAEVectors()
+    # magnitude of the terms summed into each entry: bound by the sum of |contributions|,
+    # which the oracle's |value| underestimates only under cancellation; use max(|F|, 1e-6*max|F|)
+    for k in ("F", "Q", "B", "dFdxdVp", "dQdxdVp"):
+        a, b = vec[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1, k
+    dF, dQ = d.loadDAEMatrices()
+    for a, b in ((dF, ref["dFdx"]), (dQ, ref["dQdx"])):
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
+    q0 = ref["Q"] * 0.999
+    q2 = ref["F"] - ref["B"]
+    rhs_g, J_g = d.combine(q0, q2)
+    rhs_o, J_o = o.combine(ref, q0, q2)
+    s_r = np.maximum(np.abs(rhs_o), 1e-9 * np.max(np.abs(rhs_o)))
+    assert entry_tol(rhs_g, rhs_o, s_r, abs_floor=1e-30) <= 1
+    s_j = np.maximum(np.abs(J_o), 1e-9 * np.max(np.abs(J_o)))
+    assert entry_tol(J_g, J_o, s_j, abs_floor=1e-30) <= 1
+
+
+def test_dc_load_parity(gpu):
+    c = C.ring(7, seed=9, mc_seed=9)
+    d = X.DeviceMgr(c)
+    o = Oracle(c)
+    x = np.random.default_rng(1).uniform(0, 1.8, size=o.n)
+    z_sta = np.zeros(o.nsta); z_sto = np.zeros(o.nsto)
+    kw = dict(dcop=1, tran=0, init_jct=1, newton_iter=0, order=1, h=1e-12, time=0.0, tstep=1e-12, tstop=1e-9)
+    ref = o.load(x, z_sta, z_sto, **kw)
+    d.set_solver_state(**kw)
+    sta_n, sto_n, lim = d.updateState(x, z_sta, z_sto, sto_next=z_sto)
+    assert lim == ref["limited"]
+    np.testing.assert_allclose(sto_n, ref["sto_next"], rtol=1e-12, atol=1e-30)
+    dF, dQ = d.loadDAEMatrices()
+    np.testing.assert_allclose(dF, ref["dFdx"], rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("stages", [5, 101, 201])
+def test_lu_solve_parity(gpu, stages):
+    c = C.ring(stages, seed=stages, mc_seed=stages)
+    o = Oracle(c)
+    x, sta0, sto0 = _loaded_state(o, stages)
+    ref = o.load(x, sta0, sto0, h=5e-12, order=2)
+    rhs, J = o.combine(ref, ref["Q"], ref["F"] - ref["B"])
+    p = o.pattern()
+    s = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    st, xg = s.doSolve(J, rhs)
+    assert st == 0
+    A = np.zeros((o.n, o.n))
+    for r in range(o.n):
+        for k in range(p["rowptr"][r], p["rowptr"][r + 1]):
+            A[r, p["colind"][k]] += J[k]
+    xn = np.linalg.solve(A, rhs)
+    res = np.linalg.norm(A @ xg - rhs) / np.linalg.norm(rhs)
+    assert res <= 1e-12, res
+    np.testing.assert_allclose(xg, xn, rtol=1e-9, atol=1e-12 * np.max(np.abs(xn)))
+
+
+def test_lu_singular_returns_status(gpu):
+    # structurally nonsingular, numerically singular 3x3
+    rp = np.array([0, 2, 4, 6], np.int32)
+    ci = np.array([0, 1, 0, 1, 1, 2], np.int32)
+    v = np.array([1.0, 2.0, 2.0, 4.0, 1.0, 1.0])
+    s = X.DirectSolver(3, rp, ci)
+    st, x = s.doSolve(v, np.ones(3))
+    assert st > 0 and np.all(x == 0.0)
+
+
+@pytest.mark.parametrize("stages,tstop", [(5, 2e-9), (11, 2e-9)])
+def test_batch_single_transient_matches_oracle(gpu, stages, tstop):
+    c = C.ring(stages, seed=stages)
+    b = X.Batch([c], tstop, 1e-12, 1e-11, record_max=20000)
+    cnt = b.run_to_end(chunk=512)
+    assert cnt["done"] == 1 and cnt["failed"] == 0, cnt
+    o = Oracle(c)
+    r = o.tran(tstop, 1e-12, 1e-11)
+    assert r["status"] == 0
+    t_g, x_g = b.points(0, o.n)
+    st, _ = b.instance(0, o.n)
+    # same algorithm: same step count up to rounding-driven differences
+    assert abs(st["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
+    volt = [j for j in range(o.n) if j != int(np.argmax(np.abs(r["x"][-1])) * 0 - 1)]
+    err = waveform_error(t_g, x_g[:, :], r["t"], r["x"][:, :])
+    assert err <= 1e-3 * 1.8 + 1e-6 + 5e-3, err

@@ -1,0 +1,303 @@
+"""Pins for the CPU oracle's device models (no GPU).
+
+The reference ships no golden vectors for this path and its C sources need a
+generated header to compile, so the oracle is pinned by what the mathematics
+fixes, chosen so that a plausible slip (dropped term, sign, swapped operand,
+wrong branch) fails:
+
+* limiters: pnjlim's defining property (the limited junction voltage carries
+  the current the linearised diode would carry at the requested voltage),
+  fixed points, and the documented bounds of fetlim / limvds;
+* Meyer capacitances: textbook values (2/3 Cox saturation, Cox/2 each at
+  Vds -> 0, Cox gate-bulk in accumulation) and continuity;
+* Shichman-Hodges currents in cutoff / saturation / linear, with and without
+  lambda and gamma;
+* the finite-difference Jacobian check the reference itself offers
+  (N_DEV_NumericalJacobian.C:154), KCL, the n/p mirror and d/s swap.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as P
+from oracle.pyoracle import Oracle
+from xyce_amd import circuits as C
+
+VT300 = 300.15 * 1.3806226e-23 / 1.6021918e-19
+
+
+# --------------------------------------------------------------------- limiters
+def test_pnjlim_current_matching():
+    rng = np.random.default_rng(0)
+    vt = np.full(200, VT300)
+    vcrit = np.full(200, 0.6)
+    vold = rng.uniform(0.05, 0.7, 200)
+    vnew = vold + rng.uniform(0.1, 2.0, 200)
+    out, chk = P.pnjlim(vnew, vold, vt, vcrit)
+    lim = (vnew > vcrit) & (np.abs(vnew - vold) > 2 * vt)
+    assert np.array_equal(chk.astype(bool), lim)
+    # exp((out-vold)/vt) == 1 + (vnew-vold)/vt  (Newton step taken on the current)
+    lhs = np.exp((out[lim] - vold[lim]) / vt[lim])
+    rhs = 1 + (vnew[lim] - vold[lim]) / vt[lim]
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-12)
+    # not limited -> unchanged
+    assert np.array_equal(out[~lim], vnew[~lim])
+
+
+def test_pnjlim_from_reverse_bias():
+    # vold <= 0: the limited value is where the diode current equals vnew/vt * Is*...:
+    # exp(out/vt) == vnew/vt
+    vnew = np.array([0.8, 1.2, 3.0]); vold = np.array([-0.3, 0.0, -1.0])
+    out, chk = P.pnjlim(vnew, vold, np.full(3, VT300), np.full(3, 0.6))
+    assert np.all(chk == 1)
+    np.testing.assert_allclose(np.exp(out / VT300), vnew / VT300, rtol=1e-12)
+
+
+def test_limiters_fixed_points():
+    v = np.linspace(-5, 5, 101)
+    # limvds: a non-rising Vds below 3.5 V is floored at -0.5 V, so it is a
+    # fixed point only on [-0.5, inf)
+    vv = v[v >= -0.5]
+    assert np.array_equal(P.limvds(vv, vv), vv)
+    assert np.all(P.limvds(v[v < -0.5], v[v < -0.5]) == -0.5)
+    assert np.array_equal(P.fetlim(v, v, np.full_like(v, 0.7)), v)
+    out, chk = P.pnjlim(v, v, np.full_like(v, VT300), np.full_like(v, 0.6))
+    assert np.array_equal(out, v) and np.all(chk == 0)
+
+
+def test_limvds_bounds():
+    rng = np.random.default_rng(1)
+    vold = rng.uniform(-1, 6, 2000); vnew = rng.uniform(-10, 20, 2000)
+    out = P.limvds(vnew, vold)
+    lo = vold < 3.5
+    # below 3.5 V: a rising Vds is capped at 4 V, a falling one floored at -0.5 V
+    up = lo & (vnew > vold)
+    assert np.all(out[up] <= 4.0 + 1e-15) and np.all(out[up] == np.minimum(vnew[up], 4.0))
+    dn = lo & (vnew <= vold)
+    assert np.all(out[dn] == np.maximum(vnew[dn], -0.5))
+    # above 3.5 V: rising limited to 3*vold+2; falling below 3.5 floored at 2
+    hi_up = (~lo) & (vnew > vold)
+    assert np.all(out[hi_up] <= 3 * vold[hi_up] + 2 + 1e-12)
+    hi_dn = (~lo) & (vnew <= vold) & (vnew < 3.5)
+    assert np.all(out[hi_dn] >= 2.0)
+
+
+def test_fetlim_bounds():
+    rng = np.random.default_rng(2)
+    vto = np.full(5000, 0.7)
+    vold = rng.uniform(-3, 8, 5000); vnew = rng.uniform(-15, 20, 5000)
+    out = P.fetlim(vnew, vold, vto)
+    vtsthi = np.abs(2 * (vold - vto)) + 2
+    # the limiter never reverses the requested direction nor overshoots it
+    assert np.all((out - vold) * (vnew - vold) >= -1e-12)
+    assert np.all(np.abs(out - vold) <= np.abs(vnew - vold) + 1e-12)
+    off, on = vold < vto, vold >= vto + 3.5
+    mid = ~off & ~on
+    up = vnew > vold
+    # off and turning off further: at most vtsthi per step
+    assert np.all(out[off & ~up] >= vold[off & ~up] - vtsthi[off & ~up] - 1e-12)
+    # off device turning on stops at vto + 0.5
+    m = off & (vnew > vto + 0.5)
+    assert np.all(out[m] == vto[m] + 0.5)
+    # strongly on and rising: at most vtsthi per step
+    assert np.all(out[on & up] <= vold[on & up] + vtsthi[on & up] + 1e-12)
+    # middle region: capped at vto + 4 rising, floored at vto - 0.5 falling
+    assert np.all(out[mid & up] == np.minimum(vnew[mid & up], vto[mid & up] + 4))
+    assert np.all(out[mid & ~up] == np.maximum(vnew[mid & ~up], vto[mid & ~up] - 0.5))
+
+
+# --------------------------------------------------------------------- Meyer
+def _qm(vgs, vgd, von, vdsat, phi=0.6, cox=1.0):
+    return P.qmeyer(np.array([[vgs, vgd, vgs, von, vdsat, phi, cox]]))[0]
+
+
+def test_meyer_textbook_values():
+    cox = 3.0e-15
+    # saturation (vds >= vdsat): Cgs = 2/3 Cox, Cgd = 0 (qmeyer returns half of the non-constant part)
+    cs, cd, cb = _qm(2.0, 0.5, 0.7, 1.3, cox=cox)
+    assert 2 * cs == pytest.approx(2 * cox / 3, rel=1e-15) and cd == 0 and cb == 0
+    # linear, Vds -> 0: Cgs = Cgd = Cox/2
+    cs, cd, cb = _qm(2.0, 2.0 - 1e-9, 0.7, 1.3, cox=cox)
+    assert 2 * cs == pytest.approx(cox / 2, rel=1e-8) and 2 * cd == pytest.approx(cox / 2, rel=1e-8)
+    # accumulation (vgs - von <= -phi): Cgb = Cox
+    cs, cd, cb = _qm(-1.0, -1.0, 0.7, 0.0, cox=cox)
+    assert 2 * cb == pytest.approx(cox, rel=1e-15) and cs == 0 and cd == 0
+
+
+def test_meyer_continuity():
+    cox, phi, von = 2e-15, 0.6, 0.7
+    eps = 1e-9
+    # across vgst = -phi, -phi/2, 0 and vds = vdsat
+    for vgst in (-phi, -phi / 2, 0.0):
+        a = _qm(von + vgst - eps, von + vgst - eps - 0.1, von, max(vgst - eps, 0), phi, cox)
+        b = _qm(von + vgst + eps, von + vgst + eps - 0.1, von, max(vgst + eps, 0), phi, cox)
+        if vgst != 0.0:
+            np.testing.assert_allclose(a, b, atol=1e-6 * cox)
+    vdsat = 1.0
+    a = _qm(1.7, 1.7 - (vdsat - eps), von, vdsat, phi, cox)
+    b = _qm(1.7, 1.7 - (vdsat + eps), von, vdsat, phi, cox)
+    np.testing.assert_allclose(a, b, atol=1e-6 * cox)
+
+
+# --------------------------------------------------------------------- MOSFET
+def _one_mos(mtype=+1, vto=0.7, kp=110e-6, w=2e-6, l=0.5e-6, vd=1.5, vg=1.8, vs=0.0, vb=0.0, **mp):
+    """One MOSFET whose four terminals are driven by DC sources."""
+    c = C.Circuit()
+    nd, ng, ns, nb = c.node("d"), c.node("g"), c.node("s"), c.node("b")
+    m = c.add_model(C.model_card(mtype, VTO=vto, KP=kp, **mp))
+    c.add_mos(nd, ng, ns, nb, m, C.inst_card(W=w, L=l))
+    for node, v in ((nd, vd), (ng, vg), (ns, vs), (nb, vb)):
+        c.add_vsrc(node, 0, c.add_wave(C.W_DC, [v]))
+    return c, (nd, ng, ns, nb)
+
+
+def _eval(c, nodes, volts, o=None, **kw):
+    o = o or Oracle(c)
+    p = o.pattern()
+    x = np.zeros(o.n)
+    for nd, v in zip(nodes, volts):
+        x[p["node_gid"][nd]] = v
+    sto = np.zeros(o.nsto); sta = np.zeros(o.nsta)
+    # store the junction voltages of x itself, so no limiting applies
+    dtype = kw.pop("dtype", 1)
+    vd, vg, vs, vb = volts
+    s0 = p["mos_stasto"][0, 1]
+    vbs, vgs, vds = dtype * (vb - vs), dtype * (vg - vs), dtype * (vd - vs)
+    sto[s0:s0 + 6] = [vbs - vds, vbs, vgs, vds, dtype * 0.7, 0.0]
+    args = dict(dcop=0, tran=1, newton_iter=1, order=1, h=1e-11)
+    args.update(kw)
+    r = o.load(x, sta, sto, sta_next=sta, sto_next=sto, **args)
+    return o, p, r, x
+
+
+def _drain_current(p, r, nd):
+    return r["F"][p["node_gid"][nd]]
+
+
+@pytest.mark.parametrize("vd,vg,region", [(1.5, 1.8, "sat"), (0.3, 1.8, "lin"), (1.0, 0.5, "off")])
+def test_shichman_hodges_nmos(vd, vg, region):
+    kp, w, l, vto = 110e-6, 2e-6, 0.5e-6, 0.7
+    c, nodes = _one_mos(vd=vd, vg=vg)
+    o, p, r, _ = _eval(c, nodes, (vd, vg, 0.0, 0.0))
+    beta = kp * w / l
+    vgst = vg - vto
+    if region == "sat":
+        ids = beta / 2 * vgst ** 2
+    elif region == "lin":
+        ids = beta * (vgst - vd / 2) * vd
+    else:
+        ids = 0.0
+    # bulk-drain diode (reverse biased): I = (Is/vt + gmin) * vbd, flows out of the drain node
+    isat = 1e-14
+    ibd = (isat / VT300 + 1e-12) * (0.0 - vd)
+    assert _drain_current(p, r, nodes[0]) == pytest.approx(ids - ibd, rel=1e-12, abs=1e-24)
+
+
+def test_lambda_and_body_effect():
+    kp, w, l, vto, lam, gam, phi = 110e-6, 2e-6, 0.5e-6, 0.7, 0.05, 0.4, 0.65
+    vd, vg, vs, vb = 1.6, 1.8, 0.3, 0.0
+    c, nodes = _one_mos(vd=vd, vg=vg, vs=vs, vb=vb, LAMBDA=lam, GAMMA=gam, PHI=phi)
+    o, p, r, _ = _eval(c, nodes, (vd, vg, vs, vb))
+    vbs = vb - vs
+    vth = vto + gam * (math.sqrt(phi - vbs) - math.sqrt(phi))
+    vgst, vds = vg - vs - vth, vd - vs
+    assert vgst < vds
+    ids = kp * w / l / 2 * vgst ** 2 * (1 + lam * vds)
+    ibd = (1e-14 / VT300 + 1e-12) * (vb - vd)
+    assert _drain_current(p, r, nodes[0]) == pytest.approx(ids - ibd, rel=1e-12)
+
+
+def test_pmos_mirror():
+    cn, nodes_n = _one_mos(+1, 0.7, 50e-6, vd=0.6, vg=1.3, vs=0.0, vb=0.0, LAMBDA=0.02)
+    cp, nodes_p = _one_mos(-1, -0.7, 50e-6, vd=-0.6, vg=-1.3, vs=0.0, vb=0.0, LAMBDA=0.02)
+    _, pn, rn, _ = _eval(cn, nodes_n, (0.6, 1.3, 0.0, 0.0))
+    _, pp, rp, _ = _eval(cp, nodes_p, (-0.6, -1.3, 0.0, 0.0), dtype=-1)
+    for k in range(4):
+        assert rp["F"][pp["node_gid"][nodes_p[k]]] == pytest.approx(-rn["F"][pn["node_gid"][nodes_n[k]]], rel=1e-13, abs=1e-30)
+    np.testing.assert_allclose(rp["dFdx"], rn["dFdx"], rtol=1e-13, atol=1e-30)
+
+
+def test_drain_source_swap():
+    c, nodes = _one_mos(vd=0.2, vg=1.8, vs=1.0, vb=0.0)
+    _, p, r, _ = _eval(c, nodes, (0.2, 1.8, 1.0, 0.0))
+    c2, nodes2 = _one_mos(vd=1.0, vg=1.8, vs=0.2, vb=0.0)
+    _, p2, r2, _ = _eval(c2, nodes2, (1.0, 1.8, 0.2, 0.0))
+    # channel current reverses; the bulk diodes swap roles
+    d, s = p["node_gid"][nodes[0]], p["node_gid"][nodes[2]]
+    d2, s2 = p2["node_gid"][nodes2[0]], p2["node_gid"][nodes2[2]]
+    assert r["F"][d] == pytest.approx(r2["F"][s2], rel=1e-12)
+    assert r["F"][s] == pytest.approx(r2["F"][d2], rel=1e-12)
+
+
+@pytest.mark.parametrize("volts", [(1.2, 1.8, 0.1, 0.0), (0.2, 1.1, 0.9, -0.3), (1.7, 0.2, 0.0, 0.0)])
+def test_kcl_per_device(volts):
+    c, nodes = _one_mos(vd=volts[0], vg=volts[1], vs=volts[2], vb=volts[3], GAMMA=0.3, CJ=1e-3, LAMBDA=0.03)
+    _, p, r, _ = _eval(c, nodes, volts)
+    tot = sum(r["F"][p["node_gid"][nd]] for nd in nodes)
+    mag = sum(abs(r["F"][p["node_gid"][nd]]) for nd in nodes)
+    assert abs(tot) <= 1e-14 * mag + 1e-30
+
+
+@pytest.mark.parametrize("volts", [(1.2, 1.8, 0.1, 0.0), (0.2, 1.1, 0.9, -0.3), (0.05, 1.5, 0.0, 0.0),
+                                   (-0.4, 1.2, 0.3, -0.5)])
+def test_finite_difference_jacobian(volts):
+    """analytic dFdx / dQdx against central differences (the reference's TESTJAC
+    check; relTol 0.01, absTol 1e-8 there — here 1e-5 relative)."""
+    c, nodes = _one_mos(vd=volts[0], vg=volts[1], vs=volts[2], vb=volts[3], GAMMA=0.35, LAMBDA=0.04, CJ=2e-4,
+                        CJSW=1e-9, MJ=0.45)
+    inst = c.mos_inst[0]
+    inst[2] = 4e-12; inst[3] = 4e-12; inst[4] = 8e-6; inst[5] = 8e-6   # AD AS PD PS
+    o, p, r, x0 = _eval(c, nodes, volts)
+    rp, ci = p["rowptr"], p["colind"]
+    n = o.n
+    dFa = np.zeros((n, n)); dQa = np.zeros((n, n))
+    for row in range(n):
+        for k in range(rp[row], rp[row + 1]):
+            dFa[row, ci[k]] += r["dFdx"][k]; dQa[row, ci[k]] += r["dQdx"][k]
+    mos_nodes = [p["node_gid"][nd] for nd in nodes]
+    for j in mos_nodes:
+        h = 1e-6
+        vp = list(volts); vm = list(volts)
+        idx = mos_nodes.index(j)
+        vp[idx] += h; vm[idx] -= h
+        _, _, rp_, _ = _eval(c, nodes, tuple(vp), o=o)
+        _, _, rm_, _ = _eval(c, nodes, tuple(vm), o=o)
+        fd_F = (rp_["F"] - rm_["F"]) / (2 * h)
+        fd_Q = (rp_["Q"] - rm_["Q"]) / (2 * h)
+        for i in mos_nodes:
+            assert fd_F[i] == pytest.approx(dFa[i, j], rel=1e-5, abs=1e-12), (i, j)
+            assert fd_Q[i] == pytest.approx(dQa[i, j], rel=1e-5, abs=1e-22), (i, j)
+
+
+def test_depletion_charge_sqrt_path():
+    """MJ=0.5 junction charge below FC*PB: q = 2*PB*Cj0*(1 - sqrt(1 - v/PB))."""
+    cj = 3e-4; ad = 5e-12
+    c, nodes = _one_mos(vd=0.0, vg=0.0, vs=0.0, vb=0.2, CJ=cj, MJ=0.5, PB=0.8)
+    c.mos_inst[0][2] = ad; c.mos_inst[0][3] = ad
+    o, p, r, _ = _eval(c, nodes, (0.0, 0.0, 0.0, 0.2))
+    # at TNOM the temperature-adjusted Cj and PB equal the given values
+    q1 = 2 * 0.8 * cj * ad * (1 - math.sqrt(1 - 0.2 / 0.8))
+    assert r["Q"][p["node_gid"][nodes[3]]] == pytest.approx(2 * q1, rel=1e-10)   # bulk charge = qbs + qbd
+
+
+# --------------------------------------------------------------------- sources
+def test_pulse_waveform_closed_form():
+    c = C.Circuit()
+    a = c.node("a")
+    w = c.add_wave(C.W_PULSE, [0.2, 1.4, 1e-9, 0.1e-9, 0.3e-9, 2e-9, 5e-9])
+    c.add_vsrc(a, 0, w)
+    o = Oracle(c)
+    for t, v in [(0.0, 0.2), (1e-9, 0.2), (1.05e-9, 0.8), (1.1e-9, 1.4), (2e-9, 1.4), (3.1e-9, 1.4),
+                 (3.25e-9, 0.8), (3.4e-9, 0.2), (6.05e-9, 0.8), (7.5e-9, 1.4)]:
+        assert o.wave(w, t, tstop=20e-9) == pytest.approx(v, rel=1e-12, abs=1e-15), t
+
+
+def test_pwl_waveform_closed_form():
+    c = C.Circuit()
+    a = c.node("a")
+    w = c.add_wave(C.W_PWL, [0.0, 0.0, 1e-9, 1.0, 3e-9, -1.0])
+    c.add_vsrc(a, 0, w)
+    o = Oracle(c)
+    for t, v in [(0.0, 0.0), (0.5e-9, 0.5), (1e-9, 1.0), (2e-9, 0.0), (3e-9, -1.0), (5e-9, -1.0)]:
+        assert o.wave(w, t) == pytest.approx(v, rel=1e-12, abs=1e-15), t

@@ -1,0 +1,333 @@
+"""xyce_amd — MI355X-native per-Newton hot path of a Xyce transient run.
+
+Python mirror of the reference interfaces this library replaces, over the
+C ABI of ``libxyce_amd.so`` (declared in ``include/xyce_amd.h``):
+
+* :class:`DeviceMgr` — ``DeviceMgr::updateState / loadDAEVectors /
+  loadDAEMatrices`` and ``allDevicesConverged``
+  (src/DeviceModelPKG/Core/N_DEV_DeviceMgr.C:3857,4156,3980,5603), for the
+  MOSFET level-1 / R / C / V / I devices of one circuit.
+* :class:`DirectSolver` — ``Linear::Solver::doSolve(reuse_factors, transpose)``
+  as the KLU wrapper implements it (src/LinearAlgebraServicesPKG/
+  N_LAS_AmesosSolver.C:216-478): symbolic analysis once, numeric refactor on
+  every call unless ``reuse_factors``, solve; a singular factor gives a zero
+  solution and a nonzero status.
+* :class:`Batch` — thousands of independent circuits, each running its own
+  DC operating point and OneStep transient, one launch per Newton iteration.
+
+There is no CPU fallback: every entry point fails loudly when the HIP library
+or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import circuits  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libxyce_amd.so")
+_LIB = None
+
+XG_OK = 0
+ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "circuit setup failed", -99: "NaN in solution",
+          -10: "no HIP device"}
+
+EXPORTS = [
+    "xg_circuit_create", "xg_circuit_destroy", "xg_circuit_sizes", "xg_circuit_pattern", "xg_set_solver_state",
+    "xg_update_state", "xg_load_vectors", "xg_load_matrices", "xg_combine",
+    "xg_lu_symbolic", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
+    "xg_tran_run", "xg_tran_points",
+    "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
+    "xg_batch_points", "xg_batch_autotune", "xg_host_pattern", "xg_host_lu_analyse",
+]
+
+
+class XgError(RuntimeError):
+    pass
+
+
+class CircuitDesc(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int),
+        ("n_dev", C.c_int), ("dev_type", C.c_void_p), ("dev_idx", C.c_void_p),
+        ("n_models", C.c_int), ("model_par", C.c_void_p),
+        ("n_mos", C.c_int), ("mos_nodes", C.c_void_p), ("mos_model", C.c_void_p), ("mos_inst", C.c_void_p),
+        ("n_res", C.c_int), ("res_nodes", C.c_void_p), ("res_val", C.c_void_p),
+        ("n_cap", C.c_int), ("cap_nodes", C.c_void_p), ("cap_val", C.c_void_p),
+        ("n_vsrc", C.c_int), ("vsrc_nodes", C.c_void_p), ("vsrc_wave", C.c_void_p),
+        ("n_isrc", C.c_int), ("isrc_nodes", C.c_void_p), ("isrc_wave", C.c_void_p),
+        ("n_wave", C.c_int), ("wave_type", C.c_void_p), ("wave_off", C.c_void_p), ("wave_data", C.c_void_p),
+        ("gmin", C.c_double), ("temp", C.c_double),
+    ]
+
+
+class SolverState(C.Structure):
+    _fields_ = [("dcop", C.c_int), ("tran", C.c_int), ("init_jct", C.c_int), ("init_fix", C.c_int),
+                ("newton_iter", C.c_int), ("order", C.c_int), ("h", C.c_double), ("time", C.c_double),
+                ("tstep", C.c_double), ("tstop", C.c_double)]
+
+
+def lib():
+    """Load libxyce_amd.so (built by ``__graft_entry__.build()``)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise XgError(f"HIP library missing: {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name).restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None or a.size == 0 else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(st, what):
+    if st < 0:
+        raise XgError(f"{what}: {ERRORS.get(st, st)}")
+    return st
+
+
+def make_desc(circuit, gmin=1e-12, temp=300.15):
+    """Flat arrays of a :class:`circuits.Circuit` as an ``xg_circuit_desc``.
+    Returns (desc, keepalive)."""
+    a = circuit.arrays()
+    d = CircuitDesc()
+    d.n_nodes = a["n_nodes"]
+    d.n_dev = len(a["dev_type"]); d.dev_type = _p(a["dev_type"]); d.dev_idx = _p(a["dev_idx"])
+    d.n_models = a["n_models"]; d.model_par = _p(a["model_par"])
+    d.n_mos = a["n_mos"]; d.mos_nodes = _p(a["mos_nodes"]); d.mos_model = _p(a["mos_model"]); d.mos_inst = _p(a["mos_inst"])
+    d.n_res = a["n_res"]; d.res_nodes = _p(a["res_nodes"]); d.res_val = _p(a["res_val"])
+    d.n_cap = a["n_cap"]; d.cap_nodes = _p(a["cap_nodes"]); d.cap_val = _p(a["cap_val"])
+    d.n_vsrc = a["n_vsrc"]; d.vsrc_nodes = _p(a["vsrc_nodes"]); d.vsrc_wave = _p(a["vsrc_wave"])
+    d.n_isrc = a["n_isrc"]; d.isrc_nodes = _p(a["isrc_nodes"]); d.isrc_wave = _p(a["isrc_wave"])
+    d.n_wave = a["n_wave"]; d.wave_type = _p(a["wave_type"]); d.wave_off = _p(a["wave_off"]); d.wave_data = _p(a["wave_data"])
+    d.gmin = gmin; d.temp = temp
+    return d, a
+
+
+def host_pattern(circuit, gmin=1e-12, temp=300.15):
+    """Pattern and stamp slots from the host setup (no GPU)."""
+    d, keep = make_desc(circuit, gmin, temp)
+    sizes = np.zeros(6, np.int32)
+    _check(lib().xg_host_pattern(C.byref(d), _p(sizes), None, None, None, None, None, None, None), "xg_host_pattern")
+    n, nnz, nsta, nsto, n_mos, n_nodes = (int(v) for v in sizes)
+    rp = np.zeros(n + 1, np.int32); ci = np.zeros(max(nnz, 1), np.int32); ng = np.zeros(n_nodes, np.int32)
+    ml = np.zeros(max(6 * n_mos, 1), np.int32); fs = np.zeros(max(17 * n_mos, 1), np.int32)
+    qs = np.zeros(max(14 * n_mos, 1), np.int32); ss = np.zeros(max(2 * n_mos, 1), np.int32)
+    _check(lib().xg_host_pattern(C.byref(d), _p(sizes), _p(rp), _p(ci), _p(ng), _p(ml), _p(fs), _p(qs), _p(ss)),
+           "xg_host_pattern")
+    return dict(n=n, nnz=nnz, nsta=nsta, nsto=nsto, rowptr=rp, colind=ci[:nnz], node_gid=ng,
+                mos_lid=ml[: 6 * n_mos].reshape(-1, 6), mos_fslot=fs[: 17 * n_mos].reshape(-1, 17),
+                mos_qslot=qs[: 14 * n_mos].reshape(-1, 14), mos_stasto=ss[: 2 * n_mos].reshape(-1, 2))
+
+
+def host_lu_analyse(n, rowptr, colind, values=None):
+    """Symbolic LU analysis on the host (no GPU): permutations and L+U pattern."""
+    rowptr = np.ascontiguousarray(rowptr, np.int32)
+    colind = np.ascontiguousarray(colind, np.int32)
+    v = None if values is None else np.ascontiguousarray(values, np.float64)
+    info = np.zeros(4)
+    pr = np.zeros(n, np.int32); pc = np.zeros(n, np.int32); lrp = np.zeros(n + 1, np.int32)
+    _check(lib().xg_host_lu_analyse(n, _p(rowptr), _p(colind), _p(v), _p(info), _p(pr), _p(pc), _p(lrp), None, 0),
+           "xg_host_lu_analyse")
+    nlu = int(info[0])
+    lci = np.zeros(nlu, np.int32)
+    _check(lib().xg_host_lu_analyse(n, _p(rowptr), _p(colind), _p(v), _p(info), _p(pr), _p(pc), _p(lrp), _p(lci), nlu),
+           "xg_host_lu_analyse")
+    return dict(nnz_lu=nlu, factor_levels=int(info[1]), solve_levels=int(info[2]), flops=float(info[3]),
+                perm_row=pr, perm_col=pc, lu_rowptr=lrp, lu_colind=lci)
+
+
+class DeviceMgr:
+    """One circuit's devices on the GPU (the drop-in behind DeviceMgr)."""
+
+    def __init__(self, circuit, gmin=1e-12, temp=300.15):
+        self._desc, self._keep = make_desc(circuit, gmin, temp)
+        h = C.c_void_p()
+        _check(lib().xg_circuit_create(C.byref(self._desc), C.byref(h)), "xg_circuit_create")
+        self.h = h
+        s = np.zeros(6, np.int32)
+        lib().xg_circuit_sizes(self.h, _p(s))
+        self.n, self.nnz, self.nsta, self.nsto, self.n_mos, self.n_nodes = (int(v) for v in s)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xg_circuit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        ng = np.zeros(self.n_nodes, np.int32)
+        ml = np.zeros(max(6 * self.n_mos, 1), np.int32)
+        fs = np.zeros(max(17 * self.n_mos, 1), np.int32)
+        qs = np.zeros(max(14 * self.n_mos, 1), np.int32)
+        ss = np.zeros(max(2 * self.n_mos, 1), np.int32)
+        _check(lib().xg_circuit_pattern(self.h, _p(rp), _p(ci), _p(ng), _p(ml), _p(fs), _p(qs), _p(ss)), "pattern")
+        return dict(rowptr=rp, colind=ci[: self.nnz], node_gid=ng, mos_lid=ml[: 6 * self.n_mos].reshape(-1, 6),
+                    mos_fslot=fs[: 17 * self.n_mos].reshape(-1, 17), mos_qslot=qs[: 14 * self.n_mos].reshape(-1, 14),
+                    mos_stasto=ss[: 2 * self.n_mos].reshape(-1, 2))
+
+    def set_solver_state(self, *, dcop=0, tran=1, init_jct=0, newton_iter=1, order=1, h=1e-11, time=0.0,
+                         tstep=1e-12, tstop=1e-9):
+        ss = SolverState(dcop, tran, init_jct, 0, newton_iter, order, h, time, tstep, tstop)
+        _check(lib().xg_set_solver_state(self.h, C.byref(ss)), "set_solver_state")
+        self._ss = ss
+
+    def updateState(self, x, sta_cur, sto_cur, sto_next=None):
+        """Returns (sta_next, sto_next, limited)."""
+        x = np.ascontiguousarray(x, np.float64)
+        sc = np.ascontiguousarray(sta_cur, np.float64)
+        oc = np.ascontiguousarray(sto_cur, np.float64)
+        sn = np.zeros(max(self.nsta, 1))
+        on = np.array(oc if sto_next is None else sto_next, np.float64)
+        if on.size == 0:
+            on = np.zeros(1)
+        lim = C.c_int(0)
+        _check(lib().xg_update_state(self.h, _p(x), _p(sc), _p(oc), _p(sn), _p(on), C.byref(lim)), "updateState")
+        return sn[: self.nsta], on[: self.nsto], int(lim.value)
+
+    def loadDAEVectors(self):
+        out = [np.zeros(self.n) for _ in range(5)]
+        _check(lib().xg_load_vectors(self.h, *[_p(v) for v in out]), "loadDAEVectors")
+        return dict(zip(("F", "Q", "B", "dFdxdVp", "dQdxdVp"), out))
+
+    def loadDAEMatrices(self):
+        dF = np.zeros(self.nnz)
+        dQ = np.zeros(self.nnz)
+        _check(lib().xg_load_matrices(self.h, _p(dF), _p(dQ)), "loadDAEMatrices")
+        return dF, dQ
+
+    def combine(self, q0, q2=None):
+        rhs = np.zeros(self.n)
+        J = np.zeros(self.nnz)
+        q0 = None if q0 is None else np.ascontiguousarray(q0, np.float64)
+        q2 = None if q2 is None else np.ascontiguousarray(q2, np.float64)
+        _check(lib().xg_combine(self.h, _p(q0), _p(q2), _p(rhs), _p(J)), "combine")
+        return rhs, J
+
+
+class DirectSolver:
+    """KLU-like direct solver on the GPU (symbolic once, refactor every call)."""
+
+    def __init__(self, n, rowptr, colind, values=None):
+        self.n = int(n)
+        self.rowptr = np.ascontiguousarray(rowptr, np.int32)
+        self.colind = np.ascontiguousarray(colind, np.int32)
+        v = None if values is None else np.ascontiguousarray(values, np.float64)
+        h = C.c_void_p()
+        _check(lib().xg_lu_symbolic(self.n, _p(self.rowptr), _p(self.colind), _p(v), C.byref(h)), "xg_lu_symbolic")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().xg_lu_destroy(self.h)
+        except Exception:
+            pass
+
+    def info(self):
+        i = np.zeros(4)
+        lib().xg_lu_info(self.h, _p(i))
+        return dict(nnz_lu=int(i[0]), factor_levels=int(i[1]), solve_levels=int(i[2]), flops=float(i[3]))
+
+    def numeric(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        return _check(lib().xg_lu_numeric(self.h, _p(v)), "xg_lu_numeric")
+
+    def solve(self, b):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n)
+        st = lib().xg_lu_solve(self.h, _p(b), _p(x), 0)
+        if st == -99:
+            return st, x
+        _check(st, "xg_lu_solve")
+        return st, x
+
+    def doSolve(self, values, b, reuse_factors=False):
+        """Linear::Solver::doSolve: returns (status, x); status != 0 on failure."""
+        if not reuse_factors:
+            st = self.numeric(values)
+            if st > 0:
+                return st, np.zeros(self.n)
+        return self.solve(b)
+
+
+class Batch:
+    """Batched independent circuits (one workgroup each) on the GPU."""
+
+    def __init__(self, circuits_list, tstop, tstep, hmax, record_max=0, gmin=1e-12, temp=300.15):
+        self._keep = []
+        descs = (CircuitDesc * len(circuits_list))()
+        for i, c in enumerate(circuits_list):
+            d, k = make_desc(c, gmin, temp)
+            descs[i] = d
+            self._keep.append(k)
+        self._descs = descs
+        h = C.c_void_p()
+        _check(lib().xg_batch_create(len(circuits_list), descs, C.c_double(tstop), C.c_double(tstep),
+                                     C.c_double(hmax), C.c_int(record_max), C.byref(h)), "xg_batch_create")
+        self.h = h
+        self.n_inst = len(circuits_list)
+        self.record_max = record_max
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def advance(self, launches, graph_chunk=0):
+        _check(lib().xg_batch_advance(self.h, C.c_int(launches), C.c_int(graph_chunk)), "xg_batch_advance")
+
+    def counters(self):
+        c = np.zeros(9)
+        _check(lib().xg_batch_counters(self.h, _p(c)), "xg_batch_counters")
+        keys = ("newton_iters", "loads", "steps", "rejects", "done", "failed", "launches", "seconds", "dense_solves")
+        return dict(zip(keys, c.tolist()))
+
+    def instance(self, i, n):
+        ts = np.zeros(8)
+        x = np.zeros(n)
+        _check(lib().xg_batch_instance(self.h, C.c_int(i), _p(ts), _p(x)), "xg_batch_instance")
+        keys = ("t", "h", "order", "mode", "newton_iters", "steps", "rejects", "loads")
+        return dict(zip(keys, ts.tolist())), x
+
+    def points(self, i, n):
+        t = np.zeros(self.record_max)
+        x = np.zeros(self.record_max * n)
+        cnt = lib().xg_batch_points(self.h, C.c_int(i), C.c_int(self.record_max), _p(t), _p(x))
+        _check(cnt, "xg_batch_points")
+        return t[:cnt], x[: cnt * n].reshape(cnt, n)
+
+    def run_to_end(self, chunk=256, max_launches=2_000_000):
+        total = 0
+        while total < max_launches:
+            self.advance(chunk, graph_chunk=0)
+            total += chunk
+            c = self.counters()
+            if c["done"] + c["failed"] >= self.n_inst:
+                break
+        return self.counters()
+
+    def autotune(self):
+        r = np.zeros(3)
+        _check(lib().xg_batch_autotune(self.h, _p(r)), "xg_batch_autotune")
+        return dict(block=int(r[0]), order_mode=int(r[1]), us_per_launch=float(r[2]))

@@ -1,0 +1,1098 @@
+// Batched Monte Carlo engine: thousands of independent circuits of uneven
+// size, each evaluated, stamped, factored and solved by one workgroup, one
+// launch per Newton iteration of every instance, the launches captured in a
+// hipGraph.  Each instance runs its own DC operating point and its own
+// OneStep (trapezoid) step control on the device, so no host round trip
+// happens between launches.
+//
+// Per launch and instance (reference call stack, paths relative to the Xyce
+// source root):
+//   1. device load at x       DeviceMgr::updateState / loadDAEVectors
+//                             (src/DeviceModelPKG/Core/N_DEV_DeviceMgr.C:3857,4156)
+//   2. residual               OneStep::obtainResidual (src/TimeIntegrationPKG/N_TIA_OneStep.C:219)
+//   3. convergence of the previous update
+//                             DampedNewton::converged_ (src/NonlinearSolverPKG/N_NLS_DampedNewton.C:1191)
+//   4a. not converged: Jacobian (loadDAEMatrices + obtainJacobian :469), LU
+//       refactor + solve (AmesosSolver::doSolve), x += dx (updateX_ :808)
+//   4b. converged: error estimate and completeStep / rejectStep
+//       (N_TIA_OneStep.C:2124,1946), updateHistory (:1390), next predictor (:135)
+//
+// Instances with identical structure share one copy of their pattern, gather
+// maps and LU schedules ("structure classes": all rings with the same stage
+// count share one).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "mos1_device.h"
+#include "xg_internal.h"
+
+#define HIPCHK(x)                          \
+  do {                                     \
+    hipError_t e_ = (x);                   \
+    if (e_ != hipSuccess) return XG_E_HIP; \
+  } while (0)
+
+namespace xg {
+
+// ---- per structure class (offsets into the concatenated int arrays)
+struct ClassDesc {
+  int n, nnz, nlu, n_mos, n_src;
+  int nflev, nfs, nbs;
+  int o_rowptr, o_colind, o_vgp, o_vge, o_mgp, o_mge, o_bgp, o_bge, o_lid;
+  int o_a2lu, o_lurp, o_luci, o_diag, o_flp, o_ftg, o_fop, o_fops, o_fdiv, o_fsp, o_fsr, o_bsp, o_bsr, o_prow, o_pcol;
+};
+
+// ---- per instance state (one 512-byte record in HBM)
+struct InstState {
+  int mode;          // 0 DC, 1 transient, 2 done, -1 DC failed, -2 step failure
+  int newton_iter;
+  int order;
+  int nef;
+  int beginning;
+  int bpi;
+  int stall_count;
+  int last_status;
+  long long n_iters, n_loads, n_steps, n_rejects, n_rec, n_dense;
+  double t, h, psi0, psi1, psi2, last_h, h_sum, saved_h, est, ck, beta1, beta2, errwt;
+  double norm_init, norm_old, tmp_rate, wt_dx;
+  double steps_taken;
+  double pad[31];
+};
+
+struct InstDesc {
+  int cls, dev_off, vec_off, nnz_off, src_off, rec_off;
+  long long dense_off;   // n*n scratch for the partial-pivoting fallback
+};
+
+struct BatchArgs {
+  int n_inst;
+  const int* order;
+  const InstDesc* inst;
+  InstState* st;
+  const ClassDesc* cls;
+  const int* ci;               // concatenated class int arrays
+  // devices
+  int nt_mos;
+  const double* kcls; const int* mcls; const double* tvbi; const double* beta; const double* tvto;
+  const unsigned long long* mask;
+  double* sto_cur; double* sto_nxt; double* sta_cur; double* sta_nxt; double* rec;
+  // per-instance vectors (offset vec_off, length n)
+  double* x; double* xn0; double* xH0; double* xH1; double* xH2; double* qH0; double* qH1; double* qH2;
+  double* F; double* Q; double* B; double* F0; double* B0; double* wt;
+  // linear part (offset nnz_off) and sources (offset src_off)
+  const double* linF; const double* linQ;
+  const int* src_wave; const int* wave_type; const int* wave_off; const double* wave_data;
+  // breakpoints (common to all instances)
+  int n_bp; const double* bp;
+  // transient parameters
+  double tstop, tstep, hmax;
+  // recording
+  int rec_max; double* rec_t; double* rec_x;
+  int max_n, max_nlu;
+  double* dense;           // per-instance n*n scratch
+  int* ipiv_scratch;       // unused (row swaps are applied in place)
+};
+
+constexpr double kRelTol = 1e-3, kAbsTol = 1e-6;                       // TIA defaults
+constexpr double kNlRel = 1e-2, kNlAbs = 1e-6, kDxTol = 0.33, kRhsTol = 1e-2;  // transient Newton
+constexpr int kMaxNewton = 20;
+constexpr double kDcRel = 1e-3, kDcAbs = 1e-12, kDcDxTol = 1.0, kDcRhsTol = 1e-6;
+constexpr int kDcMaxNewton = 200;
+constexpr double kHmin = 1e-18;
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmax(red[t], red[t + s]);
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+#pragma clang fp contract(off)
+// J = a*dQdx + b*dFdx without contraction (EpetraMatrix::linearCombo rounding)
+__device__ __forceinline__ double jac_combine(double a, double dq, double b, double df) { return a * dq + b * df; }
+// residual combine (OneStep::obtainResidual / NoTimeIntegration::obtainResidual rounding)
+__device__ __forceinline__ double residual_combine(bool dc, int order, double h, double f, double q, double b,
+                                                   double fl, double ql, double q0, double q2) {
+  double res;
+  if (dc) {
+    res = 1.0 * f + -1.0 * b;
+    res *= -1.0;
+    res = res + 1.0 * fl;
+  } else {
+    res = 1.0 * q + -1.0 * q0;
+    if (order == 2) {
+      res = (1.0 / h) * res + 0.5 * f + -0.5 * b;
+      res = res + 0.5 * q2;
+    } else {
+      res = (1.0 / h) * res + 1.0 * f + -1.0 * b;
+    }
+    res *= -1.0;
+    res = res + 1.0 * ((1.0 / h) * ql);
+    res = res + (order == 2 ? 0.5 : 1.0) * fl;
+  }
+  return res;
+}
+#pragma clang fp contract(fast)
+
+__device__ double next_bp(const BatchArgs& A, InstState& S) {
+  while (S.bpi < A.n_bp && A.bp[S.bpi] <= S.t * (1 + 1e-13) + 1e-30) ++S.bpi;
+  return S.bpi < A.n_bp ? A.bp[S.bpi] : A.tstop;
+}
+
+// Top of a time step: clamp to the next breakpoint, OneStep::updateCoeffs
+// (N_TIA_OneStep.C:1584), predictor (:135), error weights (NEWLTE=1).
+__device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C, const InstDesc& I, double* red) {
+  __shared__ double sh[4];
+  if (threadIdx.x == 0) {
+    const double stop = next_bp(A, S);
+    if (S.t + S.h > stop) { S.saved_h = S.h; S.h = stop - S.t; S.h_sum = S.h + S.last_h; }
+    const double t1 = S.h;
+    if (S.order == 2) S.psi2 = S.psi1;
+    S.psi1 = S.psi0; S.psi0 = t1;
+    if (S.order == 2) {
+      const double t2 = S.psi1;
+      S.beta1 = t1 / t2 + (t1 / t2) * (t1 / t2) / 2;
+      S.beta2 = -1.0 * t1 * t1 / t2 / S.psi2 / 2;
+      S.ck = (S.h / S.h_sum) / 3.0;
+    } else {
+      S.beta1 = t1 / S.psi1;
+      S.beta2 = 0.0;
+      S.ck = S.h / S.h_sum;
+    }
+    S.newton_iter = 0;
+    sh[0] = S.beta1; sh[1] = S.beta2; sh[2] = (double)S.order;
+  }
+  __syncthreads();
+  const double b1 = sh[0], b2 = sh[1];
+  const int order = (int)sh[2];
+  double mx = 0.0;
+  const int off = I.vec_off;
+  for (int i = threadIdx.x; i < C.n; i += blockDim.x) {
+    double v = A.xH0[off + i];
+    v += b1 * A.xH1[off + i];
+    if (order == 2) v += b2 * A.xH2[off + i];
+    A.xn0[off + i] = v;
+    A.x[off + i] = v;
+    mx = fmax(mx, fabs(A.xH0[off + i]));
+  }
+  mx = block_max(mx, red);
+  if (threadIdx.x == 0) S.errwt = kRelTol * mx + kAbsTol;
+  __syncthreads();
+}
+
+// Dense LU with partial pivoting of one instance's Jacobian, in the
+// instance's n*n HBM scratch, solving in place: rhs (LDS) becomes dx.  Used
+// for DC iterations (ring oscillators at their DC point have gmin-sized
+// diagonals: static pivots would grow by 1e8) and when the sparse static
+// pivots fail KLU's threshold test, i.e. where KLU's re-pivoting refactor
+// (N_LAS_AmesosSolver.C:318) would pick different pivots.  Returns nonzero if
+// a zero pivot is met (singular).
+__device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const InstDesc& I, const int* ci,
+                                 double ja, double jb, double* rhs, double* red, int* sh_i) {
+  const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
+  double* D = A.dense + I.dense_off;
+  const int* rp = ci + C.o_rowptr; const int* cc = ci + C.o_colind;
+  const int* mgp = ci + C.o_mgp; const int* mge = ci + C.o_mge;
+  const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+  const int NT = A.nt_mos;
+  for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
+  __syncthreads();
+  for (int r = tid; r < n; r += nth)
+    for (int s = rp[r]; s < rp[r + 1]; ++s) {
+      double df = 0.0, dq = 0.0;
+      for (int e = mgp[s]; e < mgp[s + 1]; ++e) {
+        const int ent = mge[e];
+        const double v = A.rec[(size_t)(ent & 63) * NT + I.dev_off + (ent >> 6)];
+        if ((ent & 63) < R_JQ0) df += v; else dq += v;
+      }
+      if (linF[s] != 0.0) df += linF[s];
+      if (linQ[s] != 0.0) dq += linQ[s];
+      D[(size_t)r * n + cc[s]] = jac_combine(ja, dq, jb, df);
+    }
+  __syncthreads();
+  int singular = 0;
+  for (int k = 0; k < n; ++k) {
+    // pivot: largest |D(i,k)|, i >= k (lowest index on ties)
+    double best = -1.0; int bi = k;
+    for (int i = k + tid; i < n; i += nth) {
+      const double v = fabs(D[(size_t)i * n + k]);
+      if (v > best) { best = v; bi = i; }
+    }
+    red[tid] = best; sh_i[tid] = bi;
+    __syncthreads();
+    for (int st = nth / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        const double o = red[tid + st]; const int oi = sh_i[tid + st];
+        if (o > red[tid] || (o == red[tid] && oi < sh_i[tid])) { red[tid] = o; sh_i[tid] = oi; }
+      }
+      __syncthreads();
+    }
+    const int p = sh_i[0];
+    const double pmax = red[0];
+    __syncthreads();
+    if (!(pmax > 0.0) || !isfinite(pmax)) { singular = 1; break; }
+    if (p != k) {
+      for (int j = tid; j < n; j += nth) {
+        const double t = D[(size_t)k * n + j]; D[(size_t)k * n + j] = D[(size_t)p * n + j]; D[(size_t)p * n + j] = t;
+      }
+      if (tid == 0) { const double t = rhs[k]; rhs[k] = rhs[p]; rhs[p] = t; }
+      __syncthreads();
+    }
+    const double piv = D[(size_t)k * n + k];
+    const int m = n - k - 1;
+    for (int i = k + 1 + tid; i < n; i += nth) D[(size_t)i * n + k] /= piv;
+    __syncthreads();
+    for (int e = tid; e < m * m; e += nth) {
+      const int i = k + 1 + e / m, j = k + 1 + e % m;
+      D[(size_t)i * n + j] -= D[(size_t)i * n + k] * D[(size_t)k * n + j];
+    }
+    if (tid == 0)
+      for (int i = k + 1; i < n; ++i) rhs[i] -= D[(size_t)i * n + k] * rhs[k];
+    __syncthreads();
+  }
+  if (singular) return 1;
+  for (int k = n - 1; k >= 0; --k) {
+    if (tid == 0) rhs[k] = rhs[k] / D[(size_t)k * n + k];
+    __syncthreads();
+    const double xk = rhs[k];
+    for (int i = tid; i < k; i += nth) rhs[i] -= D[(size_t)i * n + k] * xk;
+    __syncthreads();
+  }
+  return 0;
+}
+
+__global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
+  if ((int)blockIdx.x >= A.n_inst) return;
+  const int inst = A.order[blockIdx.x];
+  const InstDesc I = A.inst[inst];
+  const ClassDesc C = A.cls[I.cls];
+  __shared__ InstState S;
+  __shared__ double red[512];
+  __shared__ int red_i[512];
+  __shared__ int sh_int[8];
+  __shared__ double sh_d[8];
+  extern __shared__ double dyn[];
+  double* xs = dyn;                         // n + 1
+  double* rhs = xs + (A.max_n + 1);         // n
+  double* y = rhs + A.max_n;                // n
+  double* lu = y + A.max_n;                 // nlu
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (tid == 0) S = A.st[inst];
+  if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
+  __syncthreads();
+  if (S.mode >= 2 || S.mode < 0) return;
+  const int n = C.n, off = I.vec_off;
+  const int* ci = A.ci;
+  const int NT = A.nt_mos;
+  const bool dc = S.mode == 0;
+  // ---- 1. x into LDS
+  for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
+  if (tid == 0) xs[n] = 0.0;
+  __syncthreads();
+  // ---- 2. device evaluation (one thread per MOSFET)
+  {
+    MosFlags fl;
+    fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0; fl.newton_iter = S.newton_iter;
+    const int* lid = ci + C.o_lid;
+    int lim_any = 0;
+    for (int d = tid; d < C.n_mos; d += nth) {
+      const int g = I.dev_off + d;
+      const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
+      double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+      for (int j = 0; j < 6; ++j) volt[j] = xs[lid[6 * d + j]];
+      for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
+      for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
+      const unsigned long long m = A.mask[g];
+      double* rec = A.rec;
+      auto emit = [&](int plane, double v) { if (m >> plane & 1ull) rec[(size_t)plane * NT + g] = v; };
+      lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
+      for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
+      for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
+    }
+    if (lim_any) atomicOr(&sh_int[0], 1);
+  }
+  __syncthreads();
+  const int limited = sh_int[0];
+  // ---- 3. vectors per row and the residual
+  const double h = S.h;
+  const double tsrc = dc ? 0.0 : S.t + S.h;
+  double n2 = 0.0, ninf = 0.0;
+  {
+    const int* rp = ci + C.o_rowptr; const int* cc = ci + C.o_colind;
+    const int* vgp = ci + C.o_vgp; const int* vge = ci + C.o_vge;
+    const int* bgp = ci + C.o_bgp; const int* bge = ci + C.o_bge;
+    const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+    for (int r = tid; r < n; r += nth) {
+      double f = 0.0, q = 0.0, fl = 0.0, ql = 0.0;
+      for (int e = vgp[r]; e < vgp[r + 1]; ++e) {
+        const int ent = vge[e];
+        const int dev = I.dev_off + (ent >> 6), plane = ent & 63;
+        const double v = A.rec[(size_t)plane * NT + dev];
+        const int vid = d_rec_vec_id(plane);
+        if (vid == 0) f += v; else if (vid == 1) q += v; else if (vid == 2) fl += v; else ql += v;
+      }
+      double sf = 0.0, sq = 0.0;
+      for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        const double xv = xs[cc[k]];
+        if (linF[k] != 0.0) sf += linF[k] * xv;
+        if (linQ[k] != 0.0) sq += linQ[k] * xv;
+      }
+      f += sf; q += sq;
+      double b = 0.0;
+      for (int e = bgp[r]; e < bgp[r + 1]; ++e) {
+        const int ent = bge[e];
+        const int w = A.src_wave[I.src_off + (ent >> 1)];
+        const int wo = A.wave_off[w];
+        const double v = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, tsrc, A.tstep, A.tstop);
+        if (ent & 1) b -= v; else b += v;
+      }
+      A.F[off + r] = f; A.Q[off + r] = q; A.B[off + r] = b;
+      const double res = residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
+      rhs[r] = res;
+      n2 += res * res;
+      ninf = fmax(ninf, fabs(res));
+    }
+  }
+  n2 = block_sum(n2, red);
+  ninf = block_max(ninf, red);
+  // ---- 4. Newton bookkeeping (DampedNewton::solve / converged_)
+  if (tid == 0) {
+    S.n_loads++;
+    const double normRHS = sqrt(n2);
+    int status = 0;
+    if (S.newton_iter == 0) {
+      S.norm_init = normRHS; S.norm_old = normRHS; S.stall_count = 0; S.tmp_rate = 1.0;
+      status = isfinite(normRHS) ? 0 : -6;
+    } else {
+      const int maxstep = dc ? kDcMaxNewton : kMaxNewton;
+      const double rhstol = dc ? kDcRhsTol : kRhsTol, dxtol = dc ? kDcDxTol : kDxTol;
+      const int nl = S.newton_iter;
+      if (S.last_status < 0) status = S.last_status;          // linear failure / NaN from the solve
+      else if (limited && nl < maxstep) status = 0;
+      else if (limited) status = -1;
+      else if (!isfinite(normRHS)) status = -6;
+      else if (normRHS < 2.220446049250313e-16) status = 1;
+      else {
+        const double rel = normRHS / S.norm_init, rate = normRHS / S.norm_old;
+        S.norm_old = normRHS;
+        const double upd = S.wt_dx;
+        if (ninf <= rhstol && upd <= dxtol) status = 2;
+        else if (!dc && nl >= maxstep && rel <= 0.9 && rate <= 1.0) status = -3;
+        else if (upd <= 1e-6) status = 4;
+        else if (nl >= maxstep) status = -1;
+        else if (rate > 0.5 * 1.7976931348623157e308) status = -2;
+        else if (!dc) {
+          if (fabs(rate - 1.0) <= 1.0e-3) {
+            if (S.stall_count == 0 || rate < S.tmp_rate) S.tmp_rate = rate;
+            S.stall_count++;
+          } else S.stall_count = 0;
+          if (S.stall_count == 5) status = -3;
+        }
+      }
+    }
+    sh_int[1] = status;
+  }
+  __syncthreads();
+  const int status = sh_int[1];
+
+  if (status == 0) {
+    // ---- 5a. Jacobian, LU refactor, solve, update
+    const int nlu = C.nlu;
+    if (S.newton_iter == 0 && !dc) {
+      // transient weights from the predictor and the last accepted x (updateWeights_)
+      for (int i = tid; i < n; i += nth)
+        A.wt[off + i] = kNlRel * fmax(fabs(A.xH0[off + i]), fabs(xs[i])) + kNlAbs;
+    }
+    const double ja = dc ? 1.0e-20 : 1.0 / h;
+    const double jb = dc ? 1.0 : (S.order == 2 ? 0.5 : 1.0);
+    bool use_dense = dc;
+    if (!dc) {
+      for (int p = tid; p < nlu; p += nth) lu[p] = 0.0;
+      __syncthreads();
+      {
+        const int* mgp = ci + C.o_mgp; const int* mge = ci + C.o_mge; const int* a2lu = ci + C.o_a2lu;
+        const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+        for (int s = tid; s < C.nnz; s += nth) {
+          double df = 0.0, dq = 0.0;
+          for (int e = mgp[s]; e < mgp[s + 1]; ++e) {
+            const int ent = mge[e];
+            const double v = A.rec[(size_t)(ent & 63) * NT + I.dev_off + (ent >> 6)];
+            if ((ent & 63) < R_JQ0) df += v; else dq += v;
+          }
+          if (linF[s] != 0.0) df += linF[s];
+          if (linQ[s] != 0.0) dq += linQ[s];
+          lu[a2lu[s]] = jac_combine(ja, dq, jb, df);
+        }
+      }
+      __syncthreads();
+      {
+        const int* flp = ci + C.o_flp; const int* ftg = ci + C.o_ftg; const int* fop = ci + C.o_fop;
+        const int* fops = ci + C.o_fops; const int* fdiv = ci + C.o_fdiv;
+        for (int l = 0; l < C.nflev; ++l) {
+          for (int t = flp[l] + tid; t < flp[l + 1]; t += nth) {
+            const int p = ftg[t];
+            double v = lu[p];
+            for (int o = fop[t]; o < fop[t + 1]; ++o) v -= lu[fops[2 * o]] * lu[fops[2 * o + 1]];
+            const int dv = fdiv[t];
+            if (dv >= 0) v /= lu[dv];
+            lu[p] = v;
+          }
+          __syncthreads();
+        }
+      }
+      // KLU's threshold test on the static pivots: |l_ij| <= 1/0.001, finite, nonzero pivot
+      {
+        const int* lurp = ci + C.o_lurp; const int* dg = ci + C.o_diag;
+        int bad = 0;
+        for (int i = tid; i < n; i += nth) {
+          const double dgv = lu[dg[i]];
+          if (!(fabs(dgv) > 0.0) || !isfinite(dgv)) bad = 1;
+          for (int p = lurp[i]; p < dg[i]; ++p)
+            if (!(fabs(lu[p]) <= 1.0e3)) bad = 1;
+        }
+        if (bad) atomicOr(&sh_int[6], 1);
+        __syncthreads();
+        use_dense = sh_int[6] != 0;
+      }
+      if (!use_dense) {
+        const int* prow = ci + C.o_prow; const int* pcol = ci + C.o_pcol;
+        const int* lurp = ci + C.o_lurp; const int* luci = ci + C.o_luci; const int* dg = ci + C.o_diag;
+        const int* fsp = ci + C.o_fsp; const int* fsr = ci + C.o_fsr; const int* bsp = ci + C.o_bsp; const int* bsr = ci + C.o_bsr;
+        for (int i = tid; i < n; i += nth) y[i] = rhs[prow[i]];
+        __syncthreads();
+        for (int l = 0; l < C.nfs; ++l) {
+          for (int t = fsp[l] + tid; t < fsp[l + 1]; t += nth) {
+            const int i = fsr[t];
+            double v = y[i];
+            for (int p = lurp[i]; p < dg[i]; ++p) v -= lu[p] * y[luci[p]];
+            y[i] = v;
+          }
+          __syncthreads();
+        }
+        for (int l = 0; l < C.nbs; ++l) {
+          for (int t = bsp[l] + tid; t < bsp[l + 1]; t += nth) {
+            const int i = bsr[t];
+            double v = y[i];
+            for (int p = dg[i] + 1; p < lurp[i + 1]; ++p) v -= lu[p] * y[luci[p]];
+            y[i] = v / lu[dg[i]];
+          }
+          __syncthreads();
+        }
+        for (int j = tid; j < n; j += nth) rhs[pcol[j]] = y[j];
+        __syncthreads();
+      }
+    }
+    if (use_dense) {
+      if (tid == 0) S.n_dense++;
+      const int sing = dense_pivot_solve(A, C, I, ci, ja, jb, rhs, red, red_i);
+      if (sing && tid == 0) sh_int[2] |= 1;
+      __syncthreads();
+    }
+    // x += dx, weighted update norm; a failed pivot gives a zero update and
+    // a linear-solver failure (AmesosSolver: zero solution + status)
+    const bool pivot_bad = sh_int[2] & 1;
+    double wdx = 0.0;
+    int nanf = 0;
+    for (int i = tid; i < n; i += nth) {
+      const double d = pivot_bad ? 0.0 : rhs[i];
+      const double xn = xs[i] + d;
+      if (!isfinite(d)) nanf = 1;
+      A.x[off + i] = xn;
+      const double w = dc ? kDcRel * fmax(0.0, fabs(xn)) + kDcAbs : A.wt[off + i];
+      wdx = fmax(wdx, fabs(d) / w);
+    }
+    if (nanf) atomicOr(&sh_int[2], 2);
+    wdx = block_max(wdx, red);
+    if (tid == 0) {
+      S.wt_dx = wdx;
+      S.newton_iter++;
+      S.n_iters++;
+      S.last_status = sh_int[2] & 1 ? -9 : (sh_int[2] & 2 ? -6 : 0);
+      A.st[inst] = S;
+    }
+    return;
+  }
+
+  // ---- 5b. Newton finished for this step
+  if (dc) {
+    if (status < 0) {
+      if (tid == 0) { S.mode = -1; A.st[inst] = S; }
+      return;
+    }
+    // tranop -> transient: OneStep::initialize (N_TIA_OneStep.C:1684)
+    for (int d = tid; d < C.n_mos; d += nth) {
+      const int g = I.dev_off + d;
+      for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
+      for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
+    }
+    for (int i = tid; i < n; i += nth) {
+      A.xH0[off + i] = xs[i];
+      A.qH0[off + i] = A.Q[off + i];
+      A.xH1[off + i] = 0.0;
+      A.xH2[off + i] = 0.0;
+      A.qH2[off + i] = 0.0;
+    }
+    if (tid == 0) {
+      S.mode = 1;
+      S.t = 0.0;
+      const double stop = next_bp(A, S);
+      double hh = 0.005 * fabs(stop - S.t);
+      if (A.tstep > 0.0) hh = fmin(A.tstep, hh);
+      if (A.hmax > 0 && hh > A.hmax) hh = A.hmax;
+      S.h = fmax(hh, kHmin);
+      S.last_h = S.h; S.h_sum = 2.0 * S.h;
+      S.order = 1; S.psi0 = S.h; S.psi1 = 0; S.psi2 = 0;
+      S.beginning = 1;
+      S.steps_taken = 0;
+      sh_d[0] = S.h;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nth) A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[0];
+    if (A.rec_max > 0 && tid == 0 && S.n_rec < A.rec_max) A.rec_t[(size_t)I.rec_off + S.n_rec] = 0.0;
+    if (A.rec_max > 0 && S.n_rec < A.rec_max)
+      for (int i = tid; i < n; i += nth) A.rec_x[((size_t)I.rec_off + S.n_rec) * A.max_n + i] = xs[i];
+    __syncthreads();
+    if (tid == 0 && A.rec_max > 0) S.n_rec++;
+    __syncthreads();
+    begin_step(A, S, C, I, red);
+    if (tid == 0) A.st[inst] = S;
+    return;
+  }
+
+  // transient: local error test (StepErrorControl::evaluateStepError)
+  bool ok = status > 0;
+  if (ok && !S.beginning) {
+    double s2 = 0.0;
+    for (int i = tid; i < n; i += nth) {
+      const double q = (xs[i] - A.xn0[off + i]) / S.errwt;
+      s2 += q * q;
+    }
+    s2 = block_sum(s2, red);
+    if (tid == 0) { S.est = S.ck * sqrt(s2 / n); sh_int[3] = S.est <= 1.0; }
+    __syncthreads();
+    ok = sh_int[3];
+  }
+  if (ok) {
+    // completeStep (N_TIA_OneStep.C:2124) + updateHistory (:1390)
+    if (tid == 0) {
+      S.n_steps++; S.nef = 0;
+      S.last_h = S.h;
+      double newh = S.h;
+      double rr = pow(0.5 / (S.est + 0.0001), 1.0 / (S.order + 1.0));
+      S.steps_taken += 1;
+      if (S.steps_taken >= 2 && S.order == 1) {
+        S.order = 2;
+        rr = pow(0.5 / (S.est + 0.0001), 1.0 / (S.order + 1.0));
+        if (rr <= 1.05) S.order = 1;
+      }
+      if (rr >= 2.0) newh = 2.0 * S.h;
+      else if (rr <= 1) newh = fmax(0.25, fmin(0.9, rr)) * S.h;
+      sh_d[1] = newh;
+      sh_int[4] = S.order;
+    }
+    __syncthreads();
+    const int new_order = sh_int[4];
+    for (int i = tid; i < n; i += nth) {
+      if (new_order == 2) {
+        A.xH2[off + i] = A.xH1[off + i];
+        A.qH2[off + i] = 1.0 * A.F[off + i] + -1.0 * A.B[off + i];
+      }
+      const double xv = xs[i], qv = A.Q[off + i];
+      A.xH1[off + i] = 1.0 * xv + -1.0 * A.xH0[off + i];
+      A.qH1[off + i] = 1.0 * qv + -1.0 * A.qH0[off + i];
+      A.xH0[off + i] = xv;
+      A.qH0[off + i] = qv;
+    }
+    for (int d = tid; d < C.n_mos; d += nth) {
+      const int g = I.dev_off + d;
+      for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
+      for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
+    }
+    if (tid == 0) S.t = S.t + S.h;
+    __syncthreads();
+    if (A.rec_max > 0 && S.n_rec < A.rec_max) {
+      if (tid == 0) A.rec_t[(size_t)I.rec_off + S.n_rec] = S.t;
+      for (int i = tid; i < n; i += nth) A.rec_x[((size_t)I.rec_off + S.n_rec) * A.max_n + i] = xs[i];
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (A.rec_max > 0 && S.n_rec < A.rec_max) S.n_rec++;
+      double newh = fmax(sh_d[1], kHmin);
+      if (A.hmax > 0) newh = fmin(newh, A.hmax);
+      S.beginning = 0;
+      sh_int[5] = 0;
+      if (S.t >= A.tstop * (1 - 1e-12)) {
+        S.mode = 2;
+      } else if (S.bpi < A.n_bp && fabs(S.t - A.bp[S.bpi]) <= 1e-13 * fmax(1.0, fabs(S.t)) + 1e-24) {
+        // breakpoint: restart at order 1 (OneStep::initialize, not at t = 0)
+        S.saved_h = newh;
+        ++S.bpi;
+        const double stop = next_bp(A, S);
+        double hh = 0.1 * fmin(S.saved_h, fabs(stop - S.t));
+        if (A.hmax > 0 && hh > A.hmax) hh = A.hmax;
+        S.h = fmax(hh, kHmin);
+        S.last_h = S.h; S.h_sum = 2.0 * S.h;
+        S.order = 1; S.psi0 = S.h; S.steps_taken = 0; S.beginning = 1;
+        sh_int[5] = 1;
+        sh_d[2] = S.h;
+      } else {
+        S.h_sum = newh + S.last_h;
+        S.h = newh;
+      }
+    }
+    __syncthreads();
+    if (S.mode == 2) { if (tid == 0) A.st[inst] = S; return; }
+    if (sh_int[5]) {
+      for (int i = tid; i < n; i += nth) {
+        A.xH1[off + i] = 0.0;
+        A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[2];
+      }
+      __syncthreads();
+    }
+    begin_step(A, S, C, I, red);
+    if (tid == 0) A.st[inst] = S;
+    return;
+  }
+  // rejectStep (N_TIA_OneStep.C:1946)
+  if (tid == 0) {
+    S.n_rejects++;
+    S.nef++;
+    const int ord = S.order;
+    if (ord >= 1) S.psi0 = S.psi1;
+    if (ord >= 2) S.psi1 = S.psi2;
+    double newh;
+    if (status <= 0) { newh = S.h / 8; S.order = 1; }
+    else if (S.nef == 1) {
+      S.order = 1;
+      const double rr = fmax(0.25, fmin(0.9, pow(0.5 / (S.est + 0.0001), 1.0 / (S.order + 1.0))));
+      newh = rr * S.h;
+    } else { newh = 0.25 * S.h; S.order = 1; }
+    newh = fmax(newh, kHmin);
+    if (A.hmax > 0) newh = fmin(newh, A.hmax);
+    if (S.nef >= 15 || newh <= kHmin) S.mode = -2;
+    S.h_sum = newh + S.last_h;
+    S.h = newh;
+  }
+  __syncthreads();
+  if (S.mode < 0) { if (tid == 0) A.st[inst] = S; return; }
+  begin_step(A, S, C, I, red);
+  if (tid == 0) A.st[inst] = S;
+}
+
+__global__ void k_init_state(int n_inst, InstState* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  InstState s;
+  memset(&s, 0, sizeof(s));
+  s.mode = 0; s.order = 1; s.h = 1e-12;
+  st[i] = s;
+}
+
+}  // namespace xg
+
+using namespace xg;
+
+template <class T>
+struct DArr {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DArr() { if (p) (void)hipFree(p); }
+  hipError_t up(const std::vector<T>& v) {
+    n = v.size();
+    if (!n) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(p, v.data(), n * sizeof(T), hipMemcpyHostToDevice);
+  }
+  hipError_t zeros(size_t cnt) {
+    n = cnt;
+    if (!n) return hipSuccess;
+    hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemset(p, 0, n * sizeof(T));
+  }
+};
+
+struct xg_batch {
+  int n_inst = 0;
+  std::vector<ClassDesc> classes;
+  std::vector<InstDesc> insts;
+  std::vector<int> cls_n;         // per instance n
+  DArr<int> d_order, d_ci, d_mcls, d_src_wave, d_wave_type, d_wave_off;
+  DArr<InstDesc> d_inst;
+  DArr<ClassDesc> d_cls;
+  DArr<InstState> d_st;
+  DArr<double> d_kcls, d_tvbi, d_beta, d_tvto, d_sto_cur, d_sto_nxt, d_sta_cur, d_sta_nxt, d_rec;
+  DArr<unsigned long long> d_mask;
+  DArr<double> d_x, d_xn0, d_xH0, d_xH1, d_xH2, d_qH0, d_qH1, d_qH2, d_F, d_Q, d_B, d_F0, d_B0, d_wt;
+  DArr<double> d_linF, d_linQ, d_wave_data, d_bp, d_rec_t, d_rec_x, d_dense;
+  BatchArgs args{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int block = 256;
+  size_t shmem = 0;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_len = 0;
+  long long launches = 0;
+  double last_seconds = 0;
+  double tune_us = 0;
+  int order_mode = 1;
+  std::vector<double> bps;
+  ~xg_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  void set_order(int mode) {
+    std::vector<int> ord(n_inst);
+    std::iota(ord.begin(), ord.end(), 0);
+    if (mode == 1)  // largest instances first (longest-processing-time order)
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cls_n[a] > cls_n[b]; });
+    (void)hipMemcpy(d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice);
+    order_mode = mode;
+  }
+  void drop_graph() {
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    graph_len = 0;
+  }
+  void launch() {
+    hipLaunchKernelGGL(k_batch_step, dim3(n_inst), dim3(block), shmem, stream, args);
+  }
+};
+
+extern "C" {
+
+int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, double tstep, double hmax,
+                    int record_max, xg_batch** out) {
+  if (n_inst <= 0 || !descs || !out) return XG_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
+  auto b = std::make_unique<xg_batch>();
+  b->n_inst = n_inst;
+  std::vector<int> ci;                              // concatenated class arrays
+  std::map<std::vector<int>, int> cls_of;           // structure key -> class
+  std::vector<double> kcls, tvbi, beta, tvto, linF, linQ, wave_data;
+  std::vector<int> mcls, src_wave, wave_type, wave_off{0};
+  std::vector<unsigned long long> mask;
+  std::map<std::vector<double>, int> kindex;
+  int dev_off = 0, vec_off = 0, nnz_off = 0, src_off = 0;
+  long long dense_total = 0;
+  int max_n = 0, max_nlu = 0;
+  std::vector<double> all_bp;
+  for (int i = 0; i < n_inst; ++i) {
+    HostCircuit hc;
+    int st = build_host_circuit(&descs[i], hc);
+    if (st != XG_OK) return st;
+    // structure key: pattern + gather maps + node indices + source rows
+    std::vector<int> key;
+    key.push_back(hc.n); key.push_back(hc.nnz); key.push_back(hc.n_mos); key.push_back((int)hc.srcs.size());
+    key.insert(key.end(), hc.rowptr.begin(), hc.rowptr.end());
+    key.insert(key.end(), hc.colind.begin(), hc.colind.end());
+    key.insert(key.end(), hc.vg_ptr.begin(), hc.vg_ptr.end());
+    key.insert(key.end(), hc.vg_ent.begin(), hc.vg_ent.end());
+    key.insert(key.end(), hc.mg_ptr.begin(), hc.mg_ptr.end());
+    key.insert(key.end(), hc.mg_ent.begin(), hc.mg_ent.end());
+    key.insert(key.end(), hc.mos_lid.begin(), hc.mos_lid.end());
+    for (auto& s : hc.srcs) { key.push_back(s.kind); key.push_back(s.li_p); key.push_back(s.li_n); }
+    int cid;
+    auto it = cls_of.find(key);
+    if (it == cls_of.end()) {
+      cid = (int)b->classes.size();
+      cls_of.emplace(key, cid);
+      ClassDesc C{};
+      C.n = hc.n; C.nnz = hc.nnz; C.n_mos = hc.n_mos; C.n_src = (int)hc.srcs.size();
+      LuSymbolic sym;
+      // the linear values decide the transversal only through structure here:
+      // pass the first instance's Jacobian-like magnitudes (linF + linQ + 1)
+      std::vector<double> guide(hc.nnz, 1.0);
+      for (int k = 0; k < hc.nnz; ++k) guide[k] = std::fabs(hc.linF[k]) + std::fabs(hc.linQ[k]) * 1e11 + 1e-3;
+      try {
+        if (lu_analyse(hc.n, hc.rowptr.data(), hc.colind.data(), guide.data(), sym, true) != XG_OK) return XG_E_SETUP;
+      } catch (...) {
+        return XG_E_SETUP;
+      }
+      C.nlu = sym.nlu;
+      C.nflev = (int)sym.f_lvl_ptr.size() - 1;
+      C.nfs = (int)sym.fs_lvl_ptr.size() - 1;
+      C.nbs = (int)sym.bs_lvl_ptr.size() - 1;
+      auto push = [&](const std::vector<int>& v) { int o = (int)ci.size(); ci.insert(ci.end(), v.begin(), v.end()); return o; };
+      C.o_rowptr = push(hc.rowptr); C.o_colind = push(hc.colind);
+      C.o_vgp = push(hc.vg_ptr); C.o_vge = push(hc.vg_ent); C.o_mgp = push(hc.mg_ptr); C.o_mge = push(hc.mg_ent);
+      // per-row source lists (local source index)
+      {
+        const int n = hc.n;
+        std::vector<int> cnt(n + 1, 0);
+        for (auto& e : hc.srcs) {
+          if (e.li_p < n) cnt[e.li_p]++;
+          if (e.kind == 1 && e.li_n < n) cnt[e.li_n]++;
+        }
+        std::vector<int> ptr(n + 1, 0);
+        for (int r = 0; r < n; ++r) ptr[r + 1] = ptr[r] + cnt[r];
+        std::vector<int> ent(ptr[n]), fill(ptr.begin(), ptr.end() - 1);
+        for (size_t si = 0; si < hc.srcs.size(); ++si) {
+          const auto& e = hc.srcs[si];
+          if (e.kind == 0) { if (e.li_p < n) ent[fill[e.li_p]++] = (int)si * 2; }
+          else {
+            if (e.li_p < n) ent[fill[e.li_p]++] = (int)si * 2 + 1;
+            if (e.li_n < n) ent[fill[e.li_n]++] = (int)si * 2;
+          }
+        }
+        C.o_bgp = push(ptr); C.o_bge = push(ent);
+      }
+      C.o_lid = push(hc.mos_lid);
+      C.o_a2lu = push(sym.a2lu); C.o_lurp = push(sym.lu_rowptr); C.o_luci = push(sym.lu_colind);
+      C.o_diag = push(sym.lu_diag); C.o_flp = push(sym.f_lvl_ptr); C.o_ftg = push(sym.f_tgt);
+      C.o_fop = push(sym.f_op_ptr); C.o_fops = push(sym.f_ops); C.o_fdiv = push(sym.f_div);
+      C.o_fsp = push(sym.fs_lvl_ptr); C.o_fsr = push(sym.fs_rows); C.o_bsp = push(sym.bs_lvl_ptr);
+      C.o_bsr = push(sym.bs_rows); C.o_prow = push(sym.perm_row); C.o_pcol = push(sym.perm_col);
+      // Every index the kernel reads is checked here, on the host, before any launch.
+      {
+        auto in = [](const std::vector<int>& v, int lo, int hi) {
+          for (int x : v) if (x < lo || x >= hi) return false;
+          return true;
+        };
+        bool ok = in(hc.colind, 0, C.n) && in(hc.mos_lid, 0, C.n + 1) && in(sym.a2lu, 0, C.nlu) &&
+                  in(sym.lu_colind, 0, C.n) && in(sym.lu_diag, 0, C.nlu) && in(sym.f_tgt, 0, C.nlu) &&
+                  in(sym.f_ops, 0, C.nlu) && in(sym.f_div, -1, C.nlu) && in(sym.fs_rows, 0, C.n) &&
+                  in(sym.bs_rows, 0, C.n) && in(sym.perm_row, 0, C.n) && in(sym.perm_col, 0, C.n);
+        for (int e : hc.vg_ent) ok = ok && (e >> 6) < C.n_mos && (e & 63) < R_JF0;
+        for (int e : hc.mg_ent) ok = ok && (e >> 6) < C.n_mos && (e & 63) >= R_JF0 && (e & 63) < R_COUNT;
+        ok = ok && (int)hc.vg_ptr.size() == C.n + 1 && hc.vg_ptr[C.n] == (int)hc.vg_ent.size();
+        ok = ok && (int)hc.mg_ptr.size() == C.nnz + 1 && hc.mg_ptr[C.nnz] == (int)hc.mg_ent.size();
+        ok = ok && (int)sym.f_op_ptr.size() == C.nlu + 1 && sym.f_op_ptr[C.nlu] * 2 == (int)sym.f_ops.size();
+        ok = ok && sym.f_lvl_ptr.back() == C.nlu && sym.fs_lvl_ptr.back() == C.n && sym.bs_lvl_ptr.back() == C.n;
+        ok = ok && (int)hc.rowptr.size() == C.n + 1 && hc.rowptr[C.n] == C.nnz;
+        for (auto& e : hc.srcs) ok = ok && e.li_p <= C.n && e.li_n <= C.n && e.wave >= 0 && e.wave < (int)hc.wave_type.size();
+        if (!ok) return XG_E_SETUP;
+      }
+      b->classes.push_back(C);
+      max_n = std::max(max_n, C.n);
+      max_nlu = std::max(max_nlu, C.nlu);
+    } else {
+      cid = it->second;
+    }
+    InstDesc D{};
+    D.cls = cid; D.dev_off = dev_off; D.vec_off = vec_off; D.nnz_off = nnz_off; D.src_off = src_off;
+    D.rec_off = i * std::max(record_max, 0);
+    D.dense_off = dense_total;
+    dense_total += (long long)hc.n * hc.n;
+    b->insts.push_back(D);
+    b->cls_n.push_back(hc.n_mos + hc.n);
+    // devices
+    for (int d = 0; d < hc.n_mos; ++d) {
+      std::vector<double> kk(hc.cls.begin() + (size_t)hc.mos_cls[d] * KSTRIDE, hc.cls.begin() + (size_t)(hc.mos_cls[d] + 1) * KSTRIDE);
+      auto kt = kindex.find(kk);
+      int kid;
+      if (kt == kindex.end()) { kid = (int)kindex.size(); kindex.emplace(kk, kid); kcls.insert(kcls.end(), kk.begin(), kk.end()); }
+      else kid = kt->second;
+      mcls.push_back(kid);
+    }
+    tvbi.insert(tvbi.end(), hc.mos_tvbi.begin(), hc.mos_tvbi.end());
+    beta.insert(beta.end(), hc.mos_beta.begin(), hc.mos_beta.end());
+    tvto.insert(tvto.end(), hc.mos_tvto.begin(), hc.mos_tvto.end());
+    for (auto m : hc.mos_mask) mask.push_back(m);
+    linF.insert(linF.end(), hc.linF.begin(), hc.linF.end());
+    linQ.insert(linQ.end(), hc.linQ.begin(), hc.linQ.end());
+    // waves (renumbered into the batch table) and sources
+    const int wbase = (int)wave_type.size();
+    for (size_t w = 0; w < hc.wave_type.size(); ++w) {
+      wave_type.push_back(hc.wave_type[w]);
+      wave_data.insert(wave_data.end(), hc.wave_data.begin() + hc.wave_off[w], hc.wave_data.begin() + hc.wave_off[w + 1]);
+      wave_off.push_back((int)wave_data.size());
+    }
+    for (auto& s : hc.srcs) src_wave.push_back(wbase + s.wave);
+    for (size_t si = 0; si < hc.srcs.size(); ++si) wave_breakpoints(hc, hc.srcs[si].wave, tstep, tstop, all_bp);
+    dev_off += hc.n_mos; vec_off += hc.n; nnz_off += hc.nnz; src_off += (int)hc.srcs.size();
+  }
+  // common breakpoint list (the union over instances; every instance stops at all of them)
+  all_bp.push_back(tstop);
+  std::sort(all_bp.begin(), all_bp.end());
+  all_bp.erase(std::unique(all_bp.begin(), all_bp.end()), all_bp.end());
+  b->bps = all_bp;
+  const size_t NT = dev_off;
+  if (hipStreamCreate(&b->stream) != hipSuccess) return XG_E_HIP;
+  HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1));
+  HIPCHK(b->d_ci.up(ci)); HIPCHK(b->d_cls.up(b->classes)); HIPCHK(b->d_inst.up(b->insts));
+  HIPCHK(b->d_kcls.up(kcls)); HIPCHK(b->d_mcls.up(mcls)); HIPCHK(b->d_tvbi.up(tvbi)); HIPCHK(b->d_beta.up(beta));
+  HIPCHK(b->d_tvto.up(tvto)); HIPCHK(b->d_mask.up(mask));
+  HIPCHK(b->d_sto_cur.zeros(6 * NT)); HIPCHK(b->d_sto_nxt.zeros(6 * NT));
+  HIPCHK(b->d_sta_cur.zeros(8 * NT)); HIPCHK(b->d_sta_nxt.zeros(8 * NT));
+  HIPCHK(b->d_rec.zeros((size_t)R_COUNT * NT));
+  const size_t NV = vec_off;
+  for (auto* v : {&b->d_x, &b->d_xn0, &b->d_xH0, &b->d_xH1, &b->d_xH2, &b->d_qH0, &b->d_qH1, &b->d_qH2, &b->d_F,
+                  &b->d_Q, &b->d_B, &b->d_F0, &b->d_B0, &b->d_wt})
+    HIPCHK(v->zeros(NV));
+  HIPCHK(b->d_linF.up(linF)); HIPCHK(b->d_linQ.up(linQ));
+  HIPCHK(b->d_src_wave.up(src_wave)); HIPCHK(b->d_wave_type.up(wave_type)); HIPCHK(b->d_wave_off.up(wave_off));
+  HIPCHK(b->d_wave_data.up(wave_data)); HIPCHK(b->d_bp.up(all_bp));
+  HIPCHK(b->d_st.zeros(n_inst));
+  HIPCHK(b->d_dense.zeros((size_t)dense_total));
+  k_init_state<<<(n_inst + 255) / 256, 256>>>(n_inst, b->d_st.p);
+  HIPCHK(b->d_order.zeros(n_inst));
+  if (record_max > 0) {
+    HIPCHK(b->d_rec_t.zeros((size_t)n_inst * record_max));
+    HIPCHK(b->d_rec_x.zeros((size_t)n_inst * record_max * max_n));
+  }
+  HIPCHK(hipDeviceSynchronize());
+  BatchArgs& A = b->args;
+  A.n_inst = n_inst; A.order = b->d_order.p; A.inst = b->d_inst.p; A.st = b->d_st.p; A.cls = b->d_cls.p; A.ci = b->d_ci.p;
+  A.nt_mos = (int)NT; A.kcls = b->d_kcls.p; A.mcls = b->d_mcls.p; A.tvbi = b->d_tvbi.p; A.beta = b->d_beta.p;
+  A.tvto = b->d_tvto.p; A.mask = b->d_mask.p; A.sto_cur = b->d_sto_cur.p; A.sto_nxt = b->d_sto_nxt.p;
+  A.sta_cur = b->d_sta_cur.p; A.sta_nxt = b->d_sta_nxt.p; A.rec = b->d_rec.p;
+  A.x = b->d_x.p; A.xn0 = b->d_xn0.p; A.xH0 = b->d_xH0.p; A.xH1 = b->d_xH1.p; A.xH2 = b->d_xH2.p; A.qH0 = b->d_qH0.p;
+  A.qH1 = b->d_qH1.p; A.qH2 = b->d_qH2.p; A.F = b->d_F.p; A.Q = b->d_Q.p; A.B = b->d_B.p; A.F0 = b->d_F0.p;
+  A.B0 = b->d_B0.p; A.wt = b->d_wt.p; A.linF = b->d_linF.p; A.linQ = b->d_linQ.p; A.src_wave = b->d_src_wave.p;
+  A.wave_type = b->d_wave_type.p; A.wave_off = b->d_wave_off.p; A.wave_data = b->d_wave_data.p;
+  A.n_bp = (int)all_bp.size(); A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
+  A.rec_max = std::max(record_max, 0); A.rec_t = b->d_rec_t.p; A.rec_x = b->d_rec_x.p;
+  A.max_n = max_n; A.max_nlu = max_nlu;
+  A.dense = b->d_dense.p; A.ipiv_scratch = nullptr;
+  b->shmem = sizeof(double) * ((size_t)3 * (max_n + 1) + max_nlu + 8);
+  if (b->shmem > 150 * 1024) return XG_E_ARG;
+  b->set_order(1);
+  *out = b.release();
+  return XG_OK;
+}
+
+int xg_batch_destroy(xg_batch* b) {
+  if (!b) return XG_E_ARG;
+  delete b;
+  return XG_OK;
+}
+
+int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
+  if (!b || launches < 0) return XG_E_ARG;
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  int left = launches;
+  if (graph_chunk > 0 && launches >= graph_chunk) {
+    if (!b->gexec || b->graph_len != graph_chunk) {
+      b->drop_graph();
+      HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < graph_chunk; ++i) b->launch();
+      HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+      HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->graph_len = graph_chunk;
+    }
+    while (left >= graph_chunk) {
+      HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+      left -= graph_chunk;
+    }
+  }
+  for (; left > 0; --left) b->launch();
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  b->last_seconds = ms * 1e-3;
+  b->launches += launches;
+  return XG_OK;
+}
+
+int xg_batch_counters(xg_batch* b, double* c) {
+  if (!b || !c) return XG_E_ARG;
+  std::vector<InstState> st(b->n_inst);
+  HIPCHK(hipMemcpy(st.data(), b->d_st.p, sizeof(InstState) * b->n_inst, hipMemcpyDeviceToHost));
+  double it = 0, ld = 0, acc = 0, rej = 0, done = 0, fail = 0, dense = 0;
+  for (auto& s : st) {
+    it += s.n_iters; ld += s.n_loads; acc += s.n_steps; rej += s.n_rejects; dense += s.n_dense;
+    done += s.mode == 2; fail += s.mode < 0;
+  }
+  c[0] = it; c[1] = ld; c[2] = acc; c[3] = rej; c[4] = done; c[5] = fail; c[6] = (double)b->launches;
+  c[7] = b->last_seconds;
+  c[8] = dense;
+  return XG_OK;
+}
+
+int xg_batch_instance(xg_batch* b, int inst, double* ts, double* x) {
+  if (!b || inst < 0 || inst >= b->n_inst) return XG_E_ARG;
+  InstState s;
+  HIPCHK(hipMemcpy(&s, b->d_st.p + inst, sizeof(s), hipMemcpyDeviceToHost));
+  if (ts) {
+    ts[0] = s.t; ts[1] = s.h; ts[2] = s.order; ts[3] = s.mode; ts[4] = (double)s.n_iters; ts[5] = (double)s.n_steps;
+    ts[6] = (double)s.n_rejects; ts[7] = (double)s.n_loads;
+  }
+  if (x) {
+    const ClassDesc& C = b->classes[b->insts[inst].cls];
+    HIPCHK(hipMemcpy(x, b->d_x.p + b->insts[inst].vec_off, sizeof(double) * C.n, hipMemcpyDeviceToHost));
+  }
+  return XG_OK;
+}
+
+int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x) {
+  if (!b || inst < 0 || inst >= b->n_inst || b->args.rec_max == 0) return XG_E_ARG;
+  InstState s;
+  HIPCHK(hipMemcpy(&s, b->d_st.p + inst, sizeof(s), hipMemcpyDeviceToHost));
+  const int cnt = (int)std::min<long long>(s.n_rec, std::min(max_points, b->args.rec_max));
+  const ClassDesc& C = b->classes[b->insts[inst].cls];
+  const size_t base = b->insts[inst].rec_off;
+  HIPCHK(hipMemcpy(t, b->d_rec_t.p + base, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+  std::vector<double> tmp((size_t)cnt * b->args.max_n);
+  if (cnt) HIPCHK(hipMemcpy(tmp.data(), b->d_rec_x.p + base * b->args.max_n, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+  for (int p = 0; p < cnt; ++p) std::copy(tmp.begin() + (size_t)p * b->args.max_n, tmp.begin() + (size_t)p * b->args.max_n + C.n, x + (size_t)p * C.n);
+  return cnt;
+}
+
+int xg_batch_autotune(xg_batch* b, double* result) {
+  if (!b) return XG_E_ARG;
+  // Candidates: workgroup size x instance order.  Each candidate runs a few
+  // real launches (they advance the simulation; every launch is a valid
+  // Newton iteration) and the fastest per-launch time wins.
+  const int blocks[3] = {128, 256, 512};
+  double best = 1e30;
+  int best_b = b->block, best_o = b->order_mode;
+  for (int o = 0; o < 2; ++o) {
+    b->set_order(o);
+    for (int bs : blocks) {
+      b->block = bs;
+      b->drop_graph();
+      int st = xg_batch_advance(b, 2, 0);
+      if (st != XG_OK) return st;
+      st = xg_batch_advance(b, 8, 0);
+      if (st != XG_OK) return st;
+      const double us = b->last_seconds / 8 * 1e6;
+      if (us < best) { best = us; best_b = bs; best_o = o; }
+    }
+  }
+  b->block = best_b;
+  b->set_order(best_o);
+  b->drop_graph();
+  b->tune_us = best;
+  if (result) { result[0] = best_b; result[1] = best_o; result[2] = best; }
+  return XG_OK;
+}
+
+int xg_tran_run(xg_circuit*, double, double, double, int, double*) { return XG_E_ARG; }
+int xg_tran_points(xg_circuit*, double*, double*) { return XG_E_ARG; }
+
+}  // extern "C"

@@ -1,0 +1,714 @@
+// Drop-in device-side and linear-side kernels for one circuit, and the C ABI
+// that a Xyce DeviceMaster / Linear::Solver shim binds (include/xyce_amd.h).
+//
+// Data layout in HBM (one circuit):
+//   MOSFET constants   class table [n_cls][32] + per device tVbi, Beta, tVto
+//   node indices       int32 [n_mos][6] (d g s b d' s'; n = ground slot)
+//   store / state      SoA planes [6][n_mos] / [8][n_mos]
+//   record terms       SoA planes [48][n_mos], only planes whose destination
+//                      is not ground are written (per-device 64-bit mask)
+//   CSR values         dFdx, dQdx, J [nnz]; LU values [nnz(L+U)]
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "mos1_device.h"
+#include "xg_internal.h"
+
+#define HIPCHK(x)                                    \
+  do {                                               \
+    hipError_t e_ = (x);                             \
+    if (e_ != hipSuccess) return XG_E_HIP;           \
+  } while (0)
+
+namespace xg {
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t cnt) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = cnt;
+    if (cnt == 0) return hipSuccess;
+    return hipMalloc((void**)&p, cnt * sizeof(T));
+  }
+  hipError_t upload(const std::vector<T>& v, hipStream_t s) {
+    hipError_t e = alloc(v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct EvalArgs {
+  int n_mos;
+  const double* cls; const int* mcls; const double* tvbi; const double* beta; const double* tvto;
+  const int* lid; const unsigned long long* mask;
+  const double* x;
+  const double* sto_cur; const double* sto_nxt_in; const double* sta_cur;
+  double* sto_out; double* sta_out;
+  double* rec;
+  int* limited;
+  MosFlags fl;
+};
+
+__global__ void __launch_bounds__(256) k_mos1_eval(EvalArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n_mos) return;
+  const int N = a.n_mos;
+  const double* k = a.cls + (size_t)a.mcls[i] * KSTRIDE;
+  double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+  for (int j = 0; j < 6; ++j) volt[j] = a.x[a.lid[6 * i + j]];
+  for (int j = 0; j < 6; ++j) { sc[j] = a.sto_cur[(size_t)j * N + i]; sn[j] = a.sto_nxt_in[(size_t)j * N + i]; }
+  for (int j = 0; j < 8; ++j) ac[j] = a.sta_cur[(size_t)j * N + i];
+  const unsigned long long m = a.mask[i];
+  double* rec = a.rec;
+  auto emit = [&](int plane, double v) { if (m >> plane & 1ull) rec[(size_t)plane * N + i] = v; };
+  const int lim = mos1_eval(k, a.tvbi[i], a.beta[i], a.tvto[i], volt, sc, sn, ac, a.fl, so, ao, emit);
+  for (int j = 0; j < 6; ++j) a.sto_out[(size_t)j * N + i] = so[j];
+  for (int j = 0; j < 8; ++j) a.sta_out[(size_t)j * N + i] = ao[j];
+  if (lim) atomicOr(a.limited, 1);
+}
+
+struct SrcArgs {
+  const int* bg_ptr; const int* bg_ent;  // per row: entries (src*2 + neg)
+  const int* src_wave; const int* wave_type; const int* wave_off; const double* wave_data;
+  double time, tstep, tstop;
+};
+
+__device__ __forceinline__ double src_val(const SrcArgs& s, int src) {
+  const int w = s.src_wave[src];
+  const int off = s.wave_off[w];
+  return wave_value(s.wave_type[w], s.wave_data + off, s.wave_off[w + 1] - off, s.time, s.tstep, s.tstop);
+}
+
+// Per row: F/Q/dFdxdVp/dQdxdVp from the record terms (device order), then the
+// linear part's row sums (FilteredMatrix::axpy, N_LAS_FilteredMatrix.C:511-536),
+// then B from the sources (DeviceMgr::loadBVectorsforSources).
+__global__ void k_gather_vectors(int n, int n_mos, const int* __restrict__ vg_ptr, const int* __restrict__ vg_ent,
+                                 const double* __restrict__ rec, const int* __restrict__ rowptr,
+                                 const int* __restrict__ colind, const double* __restrict__ linF,
+                                 const double* __restrict__ linQ, const double* __restrict__ x, SrcArgs s,
+                                 double* F, double* Q, double* B, double* FL, double* QL) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double f = 0.0, q = 0.0, fl = 0.0, ql = 0.0;
+  for (int e = vg_ptr[r]; e < vg_ptr[r + 1]; ++e) {
+    const int ent = vg_ent[e];
+    const int dev = ent >> 6, plane = ent & 63;
+    const double v = rec[(size_t)plane * n_mos + dev];
+    const int vid = d_rec_vec_id(plane);
+    if (vid == 0) f += v; else if (vid == 1) q += v; else if (vid == 2) fl += v; else ql += v;
+  }
+  double sf = 0.0, sq = 0.0;
+  for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+    const double xv = x[colind[k]];
+    if (linF[k] != 0.0) sf += linF[k] * xv;
+    if (linQ[k] != 0.0) sq += linQ[k] * xv;
+  }
+  f += sf; q += sq;
+  double b = 0.0;
+  for (int e = s.bg_ptr[r]; e < s.bg_ptr[r + 1]; ++e) {
+    const int ent = s.bg_ent[e];
+    const double v = src_val(s, ent >> 1);
+    if (ent & 1) b -= v; else b += v;
+  }
+  F[r] = f; Q[r] = q; B[r] = b; FL[r] = fl; QL[r] = ql;
+}
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ double jac_combine_k(double a, double dq, double b, double df) { return a * dq + b * df; }
+#pragma clang fp contract(fast)
+
+// Per CSR slot: dFdx, dQdx from the record terms, + the linear matrices
+// (FilteredMatrix::addToMatrix), optionally J = a*dQdx + b*dFdx
+// (EpetraMatrix::linearCombo N_LAS_EpetraMatrix.C:629) scattered to LU storage.
+__global__ void k_gather_matrices(int nnz, int n_mos, const int* __restrict__ mg_ptr, const int* __restrict__ mg_ent,
+                                  const double* __restrict__ rec, const double* __restrict__ linF,
+                                  const double* __restrict__ linQ, double* dFdx, double* dQdx, double a, double b,
+                                  double* J, double* lu, const int* __restrict__ a2lu) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  double df = 0.0, dq = 0.0;
+  for (int e = mg_ptr[s]; e < mg_ptr[s + 1]; ++e) {
+    const int ent = mg_ent[e];
+    const int dev = ent >> 6, plane = ent & 63;
+    const double v = rec[(size_t)plane * n_mos + dev];
+    if (plane < R_JQ0) df += v; else dq += v;
+  }
+  if (linF[s] != 0.0) df += linF[s];
+  if (linQ[s] != 0.0) dq += linQ[s];
+  if (dFdx) { dFdx[s] = df; dQdx[s] = dq; }
+  if (J || lu) {
+    const double j = jac_combine_k(a, dq, b, df);
+    if (J) J[s] = j;
+    if (lu) lu[a2lu[s]] = j;
+  }
+}
+
+// OneStep::obtainResidual (N_TIA_OneStep.C:219-289) / NoTimeIntegration (DC).
+#pragma clang fp contract(off)
+__global__ void k_combine_rhs(int n, int dcop, int order, double h, const double* F, const double* Q, const double* B,
+                              const double* FL, const double* QL, const double* q0, const double* q2, double* rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r;
+  if (dcop) {
+    r = 1.0 * F[i] + -1.0 * B[i];
+    r *= -1.0;
+    r = r + 1.0 * FL[i];
+  } else {
+    r = 1.0 * Q[i] + -1.0 * q0[i];
+    if (order == 2) {
+      r = (1.0 / h) * r + 0.5 * F[i] + -0.5 * B[i];
+      r = r + 0.5 * q2[i];
+    } else {
+      r = (1.0 / h) * r + 1.0 * F[i] + -1.0 * B[i];
+    }
+    r *= -1.0;
+    const double dq = (1.0 / h) * QL[i];
+    r = r + 1.0 * dq;
+    r = r + (order == 2 ? 0.5 : 1.0) * FL[i];
+  }
+  rhs[i] = r;
+}
+
+__global__ void k_combine_jac(int nnz, double a, double b, const double* dFdx, const double* dQdx, double* J) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  J[s] = a * dQdx[s] + b * dFdx[s];
+}
+#pragma clang fp contract(fast)
+
+__global__ void k_fill(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_scatter_a(int nnz, const double* vals, const int* a2lu, double* lu) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nnz) lu[a2lu[s]] = vals[s];
+}
+
+// Entry-level scheduled LU (Doolittle order per entry: a_ij - sum_k l_ik u_kj,
+// k ascending, then / u_jj below the diagonal), one level per barrier.
+// One workgroup; LU values in global memory (L2 resident for these sizes).
+__global__ void __launch_bounds__(1024) k_lu_factor_1wg(int nlev, const int* __restrict__ lvl_ptr,
+                                                        const int* __restrict__ tgt, const int* __restrict__ op_ptr,
+                                                        const int* __restrict__ ops, const int* __restrict__ div,
+                                                        double* lu, int* bad) {
+  for (int l = 0; l < nlev; ++l) {
+    for (int t = lvl_ptr[l] + threadIdx.x; t < lvl_ptr[l + 1]; t += blockDim.x) {
+      const int p = tgt[t];
+      double v = lu[p];
+      for (int o = op_ptr[t]; o < op_ptr[t + 1]; ++o) v -= lu[ops[2 * o]] * lu[ops[2 * o + 1]];
+      const int d = div[t];
+      if (d >= 0) v /= lu[d];
+      lu[p] = v;
+    }
+    __syncthreads();
+  }
+  (void)bad;
+}
+
+__global__ void k_lu_factor_level(int t0, int t1, const int* __restrict__ tgt, const int* __restrict__ op_ptr,
+                                  const int* __restrict__ ops, const int* __restrict__ div, double* lu) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const int p = tgt[t];
+  double v = lu[p];
+  for (int o = op_ptr[t]; o < op_ptr[t + 1]; ++o) v -= lu[ops[2 * o]] * lu[ops[2 * o + 1]];
+  const int d = div[t];
+  if (d >= 0) v /= lu[d];
+  lu[p] = v;
+}
+
+// pivot check: non-finite or zero pivot, or a multiplier beyond KLU's
+// threshold (|l_ij| > 1/0.001: KLU would have re-pivoted) -> bad = row + 1
+__global__ void k_lu_pivots(int n, const int* lu_rowptr, const int* diag, const double* lu, int* bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = lu[diag[i]];
+  bool b = !(fabs(d) > 0.0) || !isfinite(d);
+  for (int p = lu_rowptr[i]; p < diag[i] && !b; ++p) b = !(fabs(lu[p]) <= 1.0e3);
+  if (b) atomicMin(bad, i + 1);
+}
+
+// Level-scheduled triangular solves; b is permuted on the way in and x on the
+// way out (B = P_r A P_c).  One workgroup.
+__global__ void __launch_bounds__(1024) k_lu_solve_1wg(int n, const int* __restrict__ lu_rowptr,
+                                                       const int* __restrict__ lu_colind, const int* __restrict__ diag,
+                                                       const double* __restrict__ lu, int nfl,
+                                                       const int* __restrict__ fl_ptr, const int* __restrict__ fl_rows,
+                                                       int nbl, const int* __restrict__ bl_ptr,
+                                                       const int* __restrict__ bl_rows, const int* __restrict__ prow,
+                                                       const int* __restrict__ pcol, const double* __restrict__ b,
+                                                       double* y, double* x) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[prow[i]];
+  __syncthreads();
+  for (int l = 0; l < nfl; ++l) {
+    for (int t = fl_ptr[l] + threadIdx.x; t < fl_ptr[l + 1]; t += blockDim.x) {
+      const int i = fl_rows[t];
+      double v = y[i];
+      for (int p = lu_rowptr[i]; p < diag[i]; ++p) v -= lu[p] * y[lu_colind[p]];
+      y[i] = v;
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < nbl; ++l) {
+    for (int t = bl_ptr[l] + threadIdx.x; t < bl_ptr[l + 1]; t += blockDim.x) {
+      const int i = bl_rows[t];
+      double v = y[i];
+      for (int p = diag[i] + 1; p < lu_rowptr[i + 1]; ++p) v -= lu[p] * y[lu_colind[p]];
+      y[i] = v / lu[diag[i]];
+    }
+    __syncthreads();
+  }
+  for (int j = threadIdx.x; j < n; j += blockDim.x) x[pcol[j]] = y[j];
+}
+
+__global__ void k_solve_fwd_level(int t0, int t1, const int* fl_rows, const int* lu_rowptr, const int* lu_colind,
+                                  const int* diag, const double* lu, double* y) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const int i = fl_rows[t];
+  double v = y[i];
+  for (int p = lu_rowptr[i]; p < diag[i]; ++p) v -= lu[p] * y[lu_colind[p]];
+  y[i] = v;
+}
+__global__ void k_solve_bwd_level(int t0, int t1, const int* bl_rows, const int* lu_rowptr, const int* lu_colind,
+                                  const int* diag, const double* lu, double* y) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const int i = bl_rows[t];
+  double v = y[i];
+  for (int p = diag[i] + 1; p < lu_rowptr[i + 1]; ++p) v -= lu[p] * y[lu_colind[p]];
+  y[i] = v / lu[diag[i]];
+}
+__global__ void k_permute_in(int n, const int* prow, const double* b, double* y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = b[prow[i]];
+}
+__global__ void k_permute_out(int n, const int* pcol, const double* y, double* x, int* nan_flag) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) {
+    x[pcol[j]] = y[j];
+    if (!isfinite(y[j])) atomicOr(nan_flag, 1);
+  }
+}
+
+// state permutation between the Xyce layout and the SoA planes
+__global__ void k_sta_to_soa(int n_mos, int nslots, const int* off0, const double* xy, double* soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mos) return;
+  for (int j = 0; j < nslots; ++j) soa[(size_t)j * n_mos + i] = xy[off0[i] + j];
+}
+__global__ void k_soa_to_sta(int n_mos, int nslots, const int* off0, const double* soa, double* xy) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_mos) return;
+  for (int j = 0; j < nslots; ++j) xy[off0[i] + j] = soa[(size_t)j * n_mos + i];
+}
+
+inline unsigned nblk(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace xg
+
+// ---------------------------------------------------------------------------
+// host objects
+// ---------------------------------------------------------------------------
+using namespace xg;
+
+struct xg_lu {
+  LuSymbolic s;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  DBuf<int> a2lu, lu_rowptr, lu_colind, lu_diag, f_lvl_ptr, f_tgt, f_op_ptr, f_ops, f_div;
+  DBuf<int> fs_ptr, fs_rows, bs_ptr, bs_rows, prow, pcol, flag;
+  DBuf<double> lu, avals, y, b, x;
+  std::vector<int> h_flvl, h_fs, h_bs;
+  bool one_wg = true;
+  int rowptr_n = 0;
+  std::vector<int> a_rowptr, a_colind;
+
+  int upload() {
+    hipStream_t st = stream;
+    HIPCHK(a2lu.upload(s.a2lu, st)); HIPCHK(lu_rowptr.upload(s.lu_rowptr, st));
+    HIPCHK(lu_colind.upload(s.lu_colind, st)); HIPCHK(lu_diag.upload(s.lu_diag, st));
+    HIPCHK(f_lvl_ptr.upload(s.f_lvl_ptr, st)); HIPCHK(f_tgt.upload(s.f_tgt, st));
+    HIPCHK(f_op_ptr.upload(s.f_op_ptr, st)); HIPCHK(f_ops.upload(s.f_ops, st)); HIPCHK(f_div.upload(s.f_div, st));
+    HIPCHK(fs_ptr.upload(s.fs_lvl_ptr, st)); HIPCHK(fs_rows.upload(s.fs_rows, st));
+    HIPCHK(bs_ptr.upload(s.bs_lvl_ptr, st)); HIPCHK(bs_rows.upload(s.bs_rows, st));
+    HIPCHK(prow.upload(s.perm_row, st)); HIPCHK(pcol.upload(s.perm_col, st));
+    HIPCHK(flag.alloc(2));
+    HIPCHK(lu.alloc(s.nlu)); HIPCHK(avals.alloc(s.nnz_a)); HIPCHK(y.alloc(s.n)); HIPCHK(b.alloc(s.n)); HIPCHK(x.alloc(s.n));
+    h_flvl = s.f_lvl_ptr; h_fs = s.fs_lvl_ptr; h_bs = s.bs_lvl_ptr;
+    const int nlev = (int)s.f_lvl_ptr.size() - 1;
+    one_wg = s.nlu <= (1 << 20) && nlev <= 4096 && s.n <= (1 << 16);
+    return XG_OK;
+  }
+
+  // numeric factorisation of the values already in `lu` storage
+  int factor_device() {
+    const int nlev = (int)h_flvl.size() - 1;
+    if (one_wg) {
+      k_lu_factor_1wg<<<1, 1024, 0, stream>>>(nlev, f_lvl_ptr.p, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, lu.p, flag.p);
+    } else {
+      for (int l = 0; l < nlev; ++l) {
+        const int t0 = h_flvl[l], t1 = h_flvl[l + 1];
+        if (t1 > t0) k_lu_factor_level<<<nblk(t1 - t0), 256, 0, stream>>>(t0, t1, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, lu.p);
+      }
+    }
+    HIPCHK(hipGetLastError());
+    return XG_OK;
+  }
+  int check_pivots(int* bad_out) {
+    int init[2] = {0x7fffffff, 0};
+    HIPCHK(hipMemcpyAsync(flag.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    k_lu_pivots<<<nblk(s.n), 256, 0, stream>>>(s.n, lu_rowptr.p, lu_diag.p, lu.p, flag.p);
+    int res[2];
+    HIPCHK(hipMemcpyAsync(res, flag.p, sizeof(res), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
+    return XG_OK;
+  }
+  // solve with device vectors b -> x
+  int solve_device(const double* db, double* dx) {
+    const int n = s.n;
+    if (one_wg) {
+      k_lu_solve_1wg<<<1, 1024, 0, stream>>>(n, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, (int)h_fs.size() - 1,
+                                             fs_ptr.p, fs_rows.p, (int)h_bs.size() - 1, bs_ptr.p, bs_rows.p, prow.p,
+                                             pcol.p, db, y.p, dx);
+    } else {
+      k_permute_in<<<nblk(n), 256, 0, stream>>>(n, prow.p, db, y.p);
+      for (size_t l = 0; l + 1 < h_fs.size(); ++l)
+        if (h_fs[l + 1] > h_fs[l])
+          k_solve_fwd_level<<<nblk(h_fs[l + 1] - h_fs[l]), 256, 0, stream>>>(h_fs[l], h_fs[l + 1], fs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, y.p);
+      for (size_t l = 0; l + 1 < h_bs.size(); ++l)
+        if (h_bs[l + 1] > h_bs[l])
+          k_solve_bwd_level<<<nblk(h_bs[l + 1] - h_bs[l]), 256, 0, stream>>>(h_bs[l], h_bs[l + 1], bs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, y.p);
+      k_permute_out<<<nblk(n), 256, 0, stream>>>(n, pcol.p, y.p, dx, flag.p + 1);
+    }
+    HIPCHK(hipGetLastError());
+    return XG_OK;
+  }
+};
+
+struct xg_circuit {
+  HostCircuit hc;
+  hipStream_t stream = nullptr;
+  int n_cls = 0;
+  DBuf<double> cls, tvbi, beta, tvto;
+  DBuf<int> mcls, lid, sta0, sto0;
+  DBuf<unsigned long long> mask;
+  DBuf<double> sto_cur, sto_nxt, sta_cur, sta_nxt, rec;
+  DBuf<double> x, F, Q, B, FL, QL, rhs, q0, q2, dFdx, dQdx, J, linF, linQ;
+  DBuf<int> rowptr, colind, vg_ptr, vg_ent, mg_ptr, mg_ent, bg_ptr, bg_ent, src_wave, wave_type, wave_off, limited;
+  DBuf<double> wave_data, xy_sta, xy_sto;
+  xg_solver_state ss{};
+  // transient run state (xg_tran_run)
+  std::vector<double> rec_t, rec_x;
+};
+
+static int circuit_upload(xg_circuit* c) {
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  c->n_cls = (int)(h.cls.size() / KSTRIDE);
+  HIPCHK(c->cls.upload(h.cls, s)); HIPCHK(c->tvbi.upload(h.mos_tvbi, s)); HIPCHK(c->beta.upload(h.mos_beta, s));
+  HIPCHK(c->tvto.upload(h.mos_tvto, s)); HIPCHK(c->mcls.upload(h.mos_cls, s)); HIPCHK(c->lid.upload(h.mos_lid, s));
+  HIPCHK(c->sta0.upload(h.mos_sta0, s)); HIPCHK(c->sto0.upload(h.mos_sto0, s));
+  std::vector<unsigned long long> m(h.mos_mask.begin(), h.mos_mask.end());
+  HIPCHK(c->mask.upload(m, s));
+  const size_t N = h.n_mos;
+  HIPCHK(c->sto_cur.alloc(6 * N)); HIPCHK(c->sto_nxt.alloc(6 * N));
+  HIPCHK(c->sta_cur.alloc(8 * N)); HIPCHK(c->sta_nxt.alloc(8 * N)); HIPCHK(c->rec.alloc((size_t)R_COUNT * N));
+  const int n = h.n, nnz = h.nnz;
+  HIPCHK(c->x.alloc(n + 1)); HIPCHK(c->F.alloc(n)); HIPCHK(c->Q.alloc(n)); HIPCHK(c->B.alloc(n));
+  HIPCHK(c->FL.alloc(n)); HIPCHK(c->QL.alloc(n)); HIPCHK(c->rhs.alloc(n)); HIPCHK(c->q0.alloc(n)); HIPCHK(c->q2.alloc(n));
+  HIPCHK(c->dFdx.alloc(nnz)); HIPCHK(c->dQdx.alloc(nnz)); HIPCHK(c->J.alloc(nnz));
+  HIPCHK(c->linF.upload(h.linF, s)); HIPCHK(c->linQ.upload(h.linQ, s));
+  HIPCHK(c->rowptr.upload(h.rowptr, s)); HIPCHK(c->colind.upload(h.colind, s));
+  HIPCHK(c->vg_ptr.upload(h.vg_ptr, s)); HIPCHK(c->vg_ent.upload(h.vg_ent, s));
+  HIPCHK(c->mg_ptr.upload(h.mg_ptr, s)); HIPCHK(c->mg_ent.upload(h.mg_ent, s));
+  // per-row source gather lists (source order)
+  std::vector<int> bcnt(n + 1, 0);
+  for (auto& e : h.srcs) {
+    if (e.kind == 0) { if (e.li_p < n) bcnt[e.li_p]++; }
+    else { if (e.li_p < n) bcnt[e.li_p]++; if (e.li_n < n) bcnt[e.li_n]++; }
+  }
+  std::vector<int> bptr(n + 1, 0);
+  for (int r = 0; r < n; ++r) bptr[r + 1] = bptr[r] + bcnt[r];
+  std::vector<int> bent(bptr[n]), fill(bptr.begin(), bptr.end() - 1), swave;
+  for (size_t si = 0; si < h.srcs.size(); ++si) {
+    const auto& e = h.srcs[si];
+    swave.push_back(e.wave);
+    if (e.kind == 0) { if (e.li_p < n) bent[fill[e.li_p]++] = (int)si * 2; }
+    else {
+      if (e.li_p < n) bent[fill[e.li_p]++] = (int)si * 2 + 1;   // B[pos] -= I
+      if (e.li_n < n) bent[fill[e.li_n]++] = (int)si * 2;       // B[neg] += I
+    }
+  }
+  HIPCHK(c->bg_ptr.upload(bptr, s)); HIPCHK(c->bg_ent.upload(bent, s)); HIPCHK(c->src_wave.upload(swave, s));
+  HIPCHK(c->wave_type.upload(h.wave_type, s)); HIPCHK(c->wave_off.upload(h.wave_off, s));
+  HIPCHK(c->wave_data.upload(h.wave_data, s));
+  HIPCHK(c->limited.alloc(1));
+  HIPCHK(c->xy_sta.alloc(std::max(1, h.nsta))); HIPCHK(c->xy_sto.alloc(std::max(1, h.nsto)));
+  k_fill<<<nblk(n + 1), 256, 0, s>>>(c->x.p, n + 1, 0.0);
+  HIPCHK(hipStreamSynchronize(s));
+  return XG_OK;
+}
+
+static SrcArgs src_args(xg_circuit* c) {
+  SrcArgs a;
+  a.bg_ptr = c->bg_ptr.p; a.bg_ent = c->bg_ent.p; a.src_wave = c->src_wave.p; a.wave_type = c->wave_type.p;
+  a.wave_off = c->wave_off.p; a.wave_data = c->wave_data.p;
+  a.time = c->ss.time; a.tstep = c->ss.tstep; a.tstop = c->ss.tstop;
+  return a;
+}
+
+extern "C" {
+
+int xg_circuit_create(const xg_circuit_desc* desc, xg_circuit** out) {
+  if (!desc || !out) return XG_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
+  auto c = std::make_unique<xg_circuit>();
+  int st = build_host_circuit(desc, c->hc);
+  if (st != XG_OK) return st;
+  if (hipStreamCreate(&c->stream) != hipSuccess) return XG_E_HIP;
+  st = circuit_upload(c.get());
+  if (st != XG_OK) return st;
+  c->ss.tran = 1; c->ss.order = 1; c->ss.h = 1e-12;
+  *out = c.release();
+  return XG_OK;
+}
+
+int xg_circuit_destroy(xg_circuit* c) {
+  if (!c) return XG_E_ARG;
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+  return XG_OK;
+}
+
+int xg_circuit_sizes(const xg_circuit* c, int* sizes) {
+  if (!c || !sizes) return XG_E_ARG;
+  sizes[0] = c->hc.n; sizes[1] = c->hc.nnz; sizes[2] = c->hc.nsta; sizes[3] = c->hc.nsto;
+  sizes[4] = c->hc.n_mos; sizes[5] = c->hc.n_nodes;
+  return XG_OK;
+}
+
+int xg_circuit_pattern(const xg_circuit* c, int* rowptr, int* colind, int* node_gid, int* mos_lid, int* mos_fslot,
+                       int* mos_qslot, int* mos_stasto) {
+  if (!c) return XG_E_ARG;
+  const HostCircuit& h = c->hc;
+  if (rowptr) std::copy(h.rowptr.begin(), h.rowptr.end(), rowptr);
+  if (colind) std::copy(h.colind.begin(), h.colind.end(), colind);
+  if (node_gid) std::copy(h.node_gid.begin(), h.node_gid.end(), node_gid);
+  if (mos_lid) std::copy(h.mos_lid.begin(), h.mos_lid.end(), mos_lid);
+  if (mos_fslot) std::copy(h.mos_fslot.begin(), h.mos_fslot.end(), mos_fslot);
+  if (mos_qslot) std::copy(h.mos_qslot.begin(), h.mos_qslot.end(), mos_qslot);
+  if (mos_stasto)
+    for (int i = 0; i < h.n_mos; ++i) { mos_stasto[2 * i] = h.mos_sta0[i]; mos_stasto[2 * i + 1] = h.mos_sto0[i]; }
+  return XG_OK;
+}
+
+int xg_set_solver_state(xg_circuit* c, const xg_solver_state* ss) {
+  if (!c || !ss) return XG_E_ARG;
+  c->ss = *ss;
+  return XG_OK;
+}
+
+int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur, double* sta_next,
+                    double* sto_next, int* limited) {
+  if (!c || !x) return XG_E_ARG;
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  const int N = h.n_mos;
+  HIPCHK(hipMemcpyAsync(c->x.p, x, sizeof(double) * h.n, hipMemcpyHostToDevice, s));
+  k_fill<<<1, 1, 0, s>>>(c->x.p + h.n, 1, 0.0);
+  if (N > 0) {
+    HIPCHK(hipMemcpyAsync(c->xy_sta.p, sta_cur, sizeof(double) * h.nsta, hipMemcpyHostToDevice, s));
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->xy_sta.p, c->sta_cur.p);
+    HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_cur, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->xy_sto.p, c->sto_cur.p);
+    HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_next, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->xy_sto.p, c->sto_nxt.p);
+  }
+  HIPCHK(hipMemsetAsync(c->limited.p, 0, sizeof(int), s));
+  if (N > 0) {
+    EvalArgs a;
+    a.n_mos = N; a.cls = c->cls.p; a.mcls = c->mcls.p; a.tvbi = c->tvbi.p; a.beta = c->beta.p; a.tvto = c->tvto.p;
+    a.lid = c->lid.p; a.mask = c->mask.p; a.x = c->x.p; a.sto_cur = c->sto_cur.p; a.sto_nxt_in = c->sto_nxt.p;
+    a.sta_cur = c->sta_cur.p; a.sto_out = c->sto_nxt.p; a.sta_out = c->sta_nxt.p; a.rec = c->rec.p;
+    a.limited = c->limited.p;
+    a.fl.dcop = c->ss.dcop; a.fl.tran = c->ss.tran; a.fl.init_jct = c->ss.init_jct; a.fl.newton_iter = c->ss.newton_iter;
+    k_mos1_eval<<<nblk(N), 256, 0, s>>>(a);
+    HIPCHK(hipGetLastError());
+    // in-place read of sto_nxt_in vs write of sto_out: each thread reads its
+    // own inputs before writing its own outputs, so aliasing is safe.
+    k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->sta_nxt.p, c->xy_sta.p);
+    HIPCHK(hipMemcpyAsync(sta_next, c->xy_sta.p, sizeof(double) * h.nsta, hipMemcpyDeviceToHost, s));
+    k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->sto_nxt.p, c->xy_sto.p);
+    HIPCHK(hipMemcpyAsync(sto_next, c->xy_sto.p, sizeof(double) * h.nsto, hipMemcpyDeviceToHost, s));
+  }
+  int lim = 0;
+  HIPCHK(hipMemcpyAsync(&lim, c->limited.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (limited) *limited = lim;
+  return XG_OK;
+}
+
+int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp) {
+  if (!c) return XG_E_ARG;
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  const int n = h.n;
+  k_gather_vectors<<<nblk(n), 256, 0, s>>>(n, h.n_mos, c->vg_ptr.p, c->vg_ent.p, c->rec.p, c->rowptr.p, c->colind.p,
+                                           c->linF.p, c->linQ.p, c->x.p, src_args(c), c->F.p, c->Q.p, c->B.p, c->FL.p, c->QL.p);
+  HIPCHK(hipGetLastError());
+  if (F) HIPCHK(hipMemcpyAsync(F, c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (Q) HIPCHK(hipMemcpyAsync(Q, c->Q.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (B) HIPCHK(hipMemcpyAsync(B, c->B.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (dFdxdVp) HIPCHK(hipMemcpyAsync(dFdxdVp, c->FL.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (dQdxdVp) HIPCHK(hipMemcpyAsync(dQdxdVp, c->QL.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return XG_OK;
+}
+
+int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
+  if (!c) return XG_E_ARG;
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  k_gather_matrices<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, h.n_mos, c->mg_ptr.p, c->mg_ent.p, c->rec.p, c->linF.p,
+                                               c->linQ.p, c->dFdx.p, c->dQdx.p, 0.0, 0.0, nullptr, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+  if (dFdx_vals) HIPCHK(hipMemcpyAsync(dFdx_vals, c->dFdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
+  if (dQdx_vals) HIPCHK(hipMemcpyAsync(dQdx_vals, c->dQdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return XG_OK;
+}
+
+int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals) {
+  if (!c) return XG_E_ARG;
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  const int n = h.n;
+  if (!c->ss.dcop) {
+    if (!q0) return XG_E_ARG;
+    HIPCHK(hipMemcpyAsync(c->q0.p, q0, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (q2) HIPCHK(hipMemcpyAsync(c->q2.p, q2, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    else k_fill<<<nblk(n), 256, 0, s>>>(c->q2.p, n, 0.0);
+  }
+  k_combine_rhs<<<nblk(n), 256, 0, s>>>(n, c->ss.dcop, c->ss.order, c->ss.h, c->F.p, c->Q.p, c->B.p, c->FL.p, c->QL.p,
+                                        c->q0.p, c->q2.p, c->rhs.p);
+  double a, b;
+  if (c->ss.dcop) { a = 1.0e-20; b = 1.0; } else { a = 1.0 / c->ss.h; b = c->ss.order == 2 ? 0.5 : 1.0; }
+  k_combine_jac<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, a, b, c->dFdx.p, c->dQdx.p, c->J.p);
+  HIPCHK(hipGetLastError());
+  if (rhs) HIPCHK(hipMemcpyAsync(rhs, c->rhs.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  if (jac_vals) HIPCHK(hipMemcpyAsync(jac_vals, c->J.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return XG_OK;
+}
+
+// ---------------- LU ----------------
+int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* values, xg_lu** out) {
+  if (n <= 0 || !rowptr || !colind || !out) return XG_E_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
+  auto lu = std::make_unique<xg_lu>();
+  try {
+    int st = lu_analyse(n, rowptr, colind, values, lu->s, true);
+    if (st != XG_OK) return st;
+  } catch (...) {
+    return XG_E_SETUP;
+  }
+  lu->a_rowptr.assign(rowptr, rowptr + n + 1);
+  lu->a_colind.assign(colind, colind + rowptr[n]);
+  if (hipStreamCreate(&lu->stream) != hipSuccess) return XG_E_HIP;
+  lu->own_stream = true;
+  int st = lu->upload();
+  if (st != XG_OK) return st;
+  *out = lu.release();
+  return XG_OK;
+}
+
+int xg_lu_destroy(xg_lu* lu) {
+  if (!lu) return XG_E_ARG;
+  hipStream_t s = lu->own_stream ? lu->stream : nullptr;
+  delete lu;
+  if (s) (void)hipStreamDestroy(s);
+  return XG_OK;
+}
+
+int xg_lu_numeric(xg_lu* lu, const double* vals) {
+  if (!lu || !vals) return XG_E_ARG;
+  hipStream_t s = lu->stream;
+  HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
+  k_fill<<<nblk(lu->s.nlu), 256, 0, s>>>(lu->lu.p, lu->s.nlu, 0.0);
+  k_scatter_a<<<nblk(lu->s.nnz_a), 256, 0, s>>>(lu->s.nnz_a, lu->avals.p, lu->a2lu.p, lu->lu.p);
+  int st = lu->factor_device();
+  if (st != XG_OK) return st;
+  int bad = 0;
+  st = lu->check_pivots(&bad);
+  if (st != XG_OK) return st;
+  if (bad) {
+    // re-pivot: analyse again steering the transversal by the values, then refactor
+    LuSymbolic s2;
+    try {
+      if (lu_analyse(lu->s.n, lu->a_rowptr.data(), lu->a_colind.data(), vals, s2, true) != XG_OK) return bad;
+    } catch (...) {
+      return bad;
+    }
+    lu->s = std::move(s2);
+    st = lu->upload();
+    if (st != XG_OK) return st;
+    k_fill<<<nblk(lu->s.nlu), 256, 0, s>>>(lu->lu.p, lu->s.nlu, 0.0);
+    k_scatter_a<<<nblk(lu->s.nnz_a), 256, 0, s>>>(lu->s.nnz_a, lu->avals.p, lu->a2lu.p, lu->lu.p);
+    st = lu->factor_device();
+    if (st != XG_OK) return st;
+    st = lu->check_pivots(&bad);
+    if (st != XG_OK) return st;
+    return bad;
+  }
+  return XG_OK;
+}
+
+int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
+  if (!lu || !b || !x) return XG_E_ARG;
+  if (transpose) return XG_E_ARG;  // Xyce's transient path never requests a transpose solve
+  hipStream_t s = lu->stream;
+  const int n = lu->s.n;
+  HIPCHK(hipMemcpyAsync(lu->b.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(lu->flag.p + 1, 0, sizeof(int), s));
+  int st = lu->solve_device(lu->b.p, lu->x.p);
+  if (st != XG_OK) return st;
+  HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+  int nanf = 0;
+  HIPCHK(hipMemcpyAsync(&nanf, lu->flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  bool bad = nanf != 0;
+  for (int i = 0; i < n && !bad; ++i) bad = !std::isfinite(x[i]);
+  if (bad) { std::fill(x, x + n, 0.0); return XG_E_NAN; }
+  return XG_OK;
+}
+
+int xg_lu_info(const xg_lu* lu, double* info) {
+  if (!lu || !info) return XG_E_ARG;
+  info[0] = lu->s.nlu;
+  info[1] = (double)lu->s.f_lvl_ptr.size() - 1;
+  info[2] = (double)(lu->s.fs_lvl_ptr.size() - 1 + lu->s.bs_lvl_ptr.size() - 1);
+  info[3] = lu->s.flops;
+  return XG_OK;
+}
+
+}  // extern "C"

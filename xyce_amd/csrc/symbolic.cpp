@@ -1,0 +1,386 @@
+// Sparse LU symbolic analysis for the GPU refactor.
+//
+// Replaces KLU's analyse step as Xyce calls it (AmesosSolver::doSolve first
+// call -> Amesos_Klu SymbolicFactorization, src/LinearAlgebraServicesPKG/
+// N_LAS_AmesosSolver.C:335): a maximum transversal gives a zero-free diagonal
+// (KLU's BTF step), then a symmetric fill-reducing order is chosen on the
+// pattern of A+A^T.  The order is chosen for the GPU: multiple-minimum-degree
+// elimination of independent sets (small systems) and nested dissection with
+// level-set separators (large systems) both give shallow elimination trees, so
+// the numeric refactor runs level by level.  Rows/columns denser than
+// 10*sqrt(n) are ordered last, as AMD does with dense rows.
+//
+// Pivots are static (the diagonal of the permuted matrix); the numeric kernels
+// check every pivot and the caller re-analyses with values when one fails,
+// which stands in for KLU's re-pivoting refactor (KLU_REPIVOT=1,
+// N_LAS_AmesosSolver.C:87,318).
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <numeric>
+#include <stdexcept>
+
+#include "xg_internal.h"
+
+namespace xg {
+
+namespace {
+
+// Maximum transversal: match[col] = row with A(row,col) != 0, preferring the
+// diagonal, then (if values are given) the largest entry of the column.
+std::vector<int> max_transversal(int n, const int* rowptr, const int* colind, const double* values) {
+  // column-wise pattern
+  std::vector<int> cptr(n + 1, 0), crow, cpos;
+  for (int r = 0; r < n; ++r) for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) cptr[colind[k] + 1]++;
+  for (int c = 0; c < n; ++c) cptr[c + 1] += cptr[c];
+  crow.resize(cptr[n]); cpos.resize(cptr[n]);
+  {
+    std::vector<int> f(cptr.begin(), cptr.end() - 1);
+    for (int r = 0; r < n; ++r)
+      for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) { int c = colind[k]; crow[f[c]] = r; cpos[f[c]++] = k; }
+  }
+  // order candidate rows of each column by |value| (descending)
+  if (values) {
+    for (int c = 0; c < n; ++c) {
+      std::vector<std::pair<double, int>> v;
+      for (int k = cptr[c]; k < cptr[c + 1]; ++k) v.push_back({-std::fabs(values[cpos[k]]), crow[k]});
+      std::stable_sort(v.begin(), v.end());
+      for (int k = cptr[c]; k < cptr[c + 1]; ++k) crow[k] = v[k - cptr[c]].second;
+    }
+  }
+  std::vector<int> match_row(n, -1), match_col(n, -1);
+  auto nonzero = [&](int r, int c) {
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k)
+      if (colind[k] == c) return !values || values[k] != 0.0;
+    return false;
+  };
+  for (int c = 0; c < n; ++c)
+    if (nonzero(c, c)) { match_col[c] = c; match_row[c] = c; }
+  // When values are given, a diagonal that is tiny relative to its column is
+  // released so the augmenting search can pick a better row.
+  if (values) {
+    for (int c = 0; c < n; ++c) {
+      if (match_col[c] != c) continue;
+      double dmax = 0, dg = 0;
+      for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+        double a = std::fabs(values[cpos[k]]);
+        dmax = std::max(dmax, a);
+      }
+      for (int k = rowptr[c]; k < rowptr[c + 1]; ++k) if (colind[k] == c) dg = std::fabs(values[k]);
+      if (dg < 1e-3 * dmax) { match_col[c] = -1; match_row[c] = -1; }
+    }
+  }
+  // augmenting paths (iterative DFS over columns; rows_on_path[i] is the row
+  // tried from column stk[i])
+  std::vector<int> visited(n, -1), rows_on_path;
+  std::vector<std::pair<int, int>> stk;
+  for (int c0 = 0; c0 < n; ++c0) {
+    if (match_col[c0] >= 0) continue;
+    stk.assign(1, {c0, cptr[c0]});
+    rows_on_path.clear();
+    bool found = false;
+    while (!stk.empty()) {
+      const int c = stk.back().first, k = stk.back().second;
+      if (k >= cptr[c + 1]) {
+        stk.pop_back();
+        if (!rows_on_path.empty()) rows_on_path.pop_back();
+        continue;
+      }
+      stk.back().second = k + 1;
+      const int r = crow[k];
+      if (visited[r] == c0) continue;
+      visited[r] = c0;
+      rows_on_path.push_back(r);
+      if (match_row[r] < 0) { found = true; break; }
+      stk.push_back({match_row[r], cptr[match_row[r]]});
+    }
+    if (!found) throw std::runtime_error("structurally singular matrix");
+    for (size_t i = 0; i < stk.size(); ++i) {
+      const int c = stk[i].first, r = rows_on_path[i];
+      match_col[c] = r; match_row[r] = c;
+    }
+  }
+  return match_col;
+}
+
+using Graph = std::vector<std::vector<int>>;
+
+// Multiple minimum degree on an explicit elimination graph: each round
+// eliminates an independent set of minimum-degree nodes, so a round is one
+// level of the elimination tree.
+void mmd_order(const Graph& g0, const std::vector<int>& nodes, std::vector<int>& order) {
+  const int m = (int)nodes.size();
+  if (m == 0) return;
+  std::vector<int> loc(g0.size(), -1);
+  for (int i = 0; i < m; ++i) loc[nodes[i]] = i;
+  Graph g(m);
+  for (int i = 0; i < m; ++i) {
+    for (int a : g0[nodes[i]]) if (loc[a] >= 0 && a != nodes[i]) g[i].push_back(loc[a]);
+    std::sort(g[i].begin(), g[i].end());
+    g[i].erase(std::unique(g[i].begin(), g[i].end()), g[i].end());
+  }
+  std::vector<char> gone(m, 0), mark(m, 0);
+  int left = m;
+  while (left > 0) {
+    int dmin = m + 1;
+    for (int i = 0; i < m; ++i) if (!gone[i]) dmin = std::min(dmin, (int)g[i].size());
+    std::vector<int> sel;
+    std::fill(mark.begin(), mark.end(), 0);
+    for (int i = 0; i < m; ++i) {
+      if (gone[i] || (int)g[i].size() != dmin || mark[i]) continue;
+      sel.push_back(i);
+      mark[i] = 1;
+      for (int a : g[i]) mark[a] = 1;
+    }
+    for (int v : sel) {
+      order.push_back(nodes[v]);
+      gone[v] = 1; --left;
+      const std::vector<int> nb = g[v];
+      for (int a : nb) {
+        auto& ga = g[a];
+        ga.erase(std::remove(ga.begin(), ga.end(), v), ga.end());
+        std::vector<int> merged;
+        merged.reserve(ga.size() + nb.size());
+        std::set_union(ga.begin(), ga.end(), nb.begin(), nb.end(), std::back_inserter(merged));
+        merged.erase(std::remove(merged.begin(), merged.end(), a), merged.end());
+        ga.swap(merged);
+      }
+      g[v].clear();
+    }
+  }
+}
+
+// Nested dissection with BFS level-set separators (George's automatic
+// nested dissection) down to leaves ordered by MMD.
+void nd_order(const Graph& g, std::vector<int> nodes, std::vector<int>& order, std::vector<int>& inset, int tag) {
+  if ((int)nodes.size() <= 256) { mmd_order(g, nodes, order); return; }
+  for (int v : nodes) inset[v] = tag;
+  // pseudo-peripheral start
+  auto bfs = [&](int s, std::vector<int>& lvl, int& last) {
+    for (int v : nodes) lvl[v] = -1;
+    std::deque<int> q{s}; lvl[s] = 0; last = s; int depth = 0;
+    while (!q.empty()) {
+      int v = q.front(); q.pop_front(); last = v; depth = lvl[v];
+      for (int a : g[v]) if (inset[a] == tag && lvl[a] < 0) { lvl[a] = lvl[v] + 1; q.push_back(a); }
+    }
+    return depth;
+  };
+  static thread_local std::vector<int> lvl;
+  if (lvl.size() < g.size()) lvl.assign(g.size(), -1);
+  int last = nodes[0];
+  int depth = bfs(nodes[0], lvl, last);
+  for (int it = 0; it < 3; ++it) { int l2; int d2 = bfs(last, lvl, l2); if (d2 <= depth) break; depth = d2; last = l2; }
+  int l2;
+  depth = bfs(last, lvl, l2);
+  // disconnected remainder goes with the far side
+  std::vector<int> a, b, sep, rest;
+  const int mid = depth / 2;
+  if (depth < 2) { for (int v : nodes) inset[v] = -1; mmd_order(g, nodes, order); return; }
+  for (int v : nodes) {
+    if (lvl[v] < 0) rest.push_back(v);
+    else if (lvl[v] < mid) a.push_back(v);
+    else if (lvl[v] > mid) b.push_back(v);
+    else sep.push_back(v);
+  }
+  b.insert(b.end(), rest.begin(), rest.end());
+  for (int v : nodes) inset[v] = -1;
+  nd_order(g, a, order, inset, tag * 2 + 1);
+  nd_order(g, b, order, inset, tag * 2 + 2);
+  mmd_order(g, sep, order);
+}
+
+}  // namespace
+
+int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s, bool want_schedule) {
+  s = LuSymbolic();
+  s.n = n; s.nnz_a = rowptr[n];
+  std::vector<int> match = max_transversal(n, rowptr, colind, values);  // A'(c,:) = A(match[c],:)
+  // A' pattern (row c of A' = row match[c] of A), symmetrised
+  Graph g(n);
+  for (int c = 0; c < n; ++c) {
+    const int r = match[c];
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+      const int j = colind[k];
+      if (j != c) { g[c].push_back(j); g[j].push_back(c); }
+    }
+  }
+  for (auto& v : g) { std::sort(v.begin(), v.end()); v.erase(std::unique(v.begin(), v.end()), v.end()); }
+  // dense nodes last
+  const int dense_thresh = std::max(16, (int)(10.0 * std::sqrt((double)n)));
+  std::vector<int> sparse_nodes, dense_nodes;
+  for (int v = 0; v < n; ++v) ((int)g[v].size() > dense_thresh ? dense_nodes : sparse_nodes).push_back(v);
+  std::vector<int> order;
+  order.reserve(n);
+  if ((int)sparse_nodes.size() <= 4096) {
+    mmd_order(g, sparse_nodes, order);
+  } else {
+    std::vector<int> inset(n, -1);
+    nd_order(g, sparse_nodes, order, inset, 1);
+  }
+  order.insert(order.end(), dense_nodes.begin(), dense_nodes.end());
+  s.dense_tail = (int)dense_nodes.size();
+  // B(i,j) = A'(order[i], order[j]) = A(match[order[i]], order[j])
+  s.perm_row.resize(n); s.perm_col.resize(n);
+  std::vector<int> pinv(n);
+  for (int i = 0; i < n; ++i) { s.perm_col[i] = order[i]; s.perm_row[i] = match[order[i]]; pinv[order[i]] = i; }
+  std::vector<int> rinv(n);
+  for (int i = 0; i < n; ++i) rinv[s.perm_row[i]] = i;
+  // symmetric pattern of B, lower part per row (for the etree / fill)
+  std::vector<std::vector<int>> blow(n);
+  for (int r = 0; r < n; ++r) {
+    const int i = rinv[r];
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+      const int j = pinv[colind[k]];
+      if (j < i) blow[i].push_back(j);
+      else if (j > i) blow[j].push_back(i);
+    }
+  }
+  // elimination tree (Liu) and row structure of L via ereach
+  std::vector<int> parent(n, -1), ancestor(n, -1);
+  for (int i = 0; i < n; ++i) {
+    for (int j : blow[i]) {
+      int k = j;
+      while (k != -1 && k < i) {
+        int next = ancestor[k];
+        ancestor[k] = i;
+        if (next == -1) { parent[k] = i; break; }
+        k = next;
+      }
+    }
+  }
+  std::vector<std::vector<int>> lrow(n);
+  {
+    std::vector<int> mark(n, -1);
+    for (int i = 0; i < n; ++i) {
+      mark[i] = i;
+      for (int j : blow[i]) {
+        int k = j;
+        while (k != -1 && k < i && mark[k] != i) { lrow[i].push_back(k); mark[k] = i; k = parent[k]; }
+      }
+      std::sort(lrow[i].begin(), lrow[i].end());
+    }
+  }
+  blow.clear(); blow.shrink_to_fit();
+  // U pattern = transpose of L pattern: urow[k] = { i : k in lrow[i] }
+  std::vector<std::vector<int>> urow(n);
+  for (int i = 0; i < n; ++i) for (int k : lrow[i]) urow[k].push_back(i);
+  // row-major L+U storage
+  s.lu_rowptr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) s.lu_rowptr[i + 1] = s.lu_rowptr[i] + (int)lrow[i].size() + 1 + (int)urow[i].size();
+  s.nlu = s.lu_rowptr[n];
+  s.lu_colind.resize(s.nlu); s.lu_diag.resize(n);
+  for (int i = 0; i < n; ++i) {
+    int p = s.lu_rowptr[i];
+    for (int k : lrow[i]) s.lu_colind[p++] = k;
+    s.lu_diag[i] = p; s.lu_colind[p++] = i;
+    for (int j : urow[i]) s.lu_colind[p++] = j;
+  }
+  auto pos = [&](int i, int j) -> int {
+    const int* b = s.lu_colind.data() + s.lu_rowptr[i];
+    const int* e = s.lu_colind.data() + s.lu_rowptr[i + 1];
+    const int* f = std::lower_bound(b, e, j);
+    if (f == e || *f != j) return -1;
+    return (int)(f - s.lu_colind.data());
+  };
+  s.a2lu.resize(s.nnz_a);
+  for (int r = 0; r < n; ++r)
+    for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+      const int p = pos(rinv[r], pinv[colind[k]]);
+      if (p < 0) throw std::runtime_error("A entry missing from LU pattern");
+      s.a2lu[k] = p;
+    }
+  // flops: sum over pivots of 2*|L(:,k)|*|U(k,:)| + |L(:,k)|
+  {
+    double fl = 0;
+    for (int k = 0; k < n; ++k) fl += 2.0 * (double)urow[k].size() * (double)urow[k].size() + (double)urow[k].size();
+    s.flops = fl;
+  }
+  if (!want_schedule) return XG_OK;
+
+  // ---- entry-level factor schedule: target (i,j) += -L(i,k)*U(k,j), k ascending
+  std::vector<int> level(s.nlu, 0);
+  std::vector<std::vector<int>> tops(s.nlu);   // not stored per target to save memory: build CSR directly
+  // count ops per target
+  std::vector<int> opcnt(s.nlu, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k : lrow[i])
+      for (int j : urow[k]) { int p = pos(i, j); if (p >= 0) opcnt[p]++; }
+  tops.clear();
+  std::vector<int> op_ptr(s.nlu + 1, 0);
+  for (int p = 0; p < s.nlu; ++p) op_ptr[p + 1] = op_ptr[p] + opcnt[p];
+  std::vector<int> ops((size_t)2 * op_ptr[s.nlu]);
+  {
+    std::vector<int> fill(op_ptr.begin(), op_ptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int k : lrow[i]) {
+        const int pik = pos(i, k);
+        for (int j : urow[k]) {
+          const int p = pos(i, j);
+          if (p < 0) continue;
+          const int f = fill[p]++;
+          ops[2 * f] = pik; ops[2 * f + 1] = pos(k, j);
+        }
+      }
+    // the diagonal U(k,k)... ops with j==k do not occur (urow excludes k)
+  }
+  // also U(k,j) for j == i with i > k: covered (urow[k] contains i)
+  // levels in row-major order
+  std::vector<int> diag_of(s.nlu, -1);
+  int maxlvl = 0;
+  for (int i = 0; i < n; ++i) {
+    for (int p = s.lu_rowptr[i]; p < s.lu_rowptr[i + 1]; ++p) {
+      const int j = s.lu_colind[p];
+      int lv = 0;
+      for (int o = op_ptr[p]; o < op_ptr[p + 1]; ++o) lv = std::max(lv, std::max(level[ops[2 * o]], level[ops[2 * o + 1]]) + 1);
+      if (j < i) { diag_of[p] = s.lu_diag[j]; lv = std::max(lv, level[s.lu_diag[j]] + 1); }
+      level[p] = lv;
+      maxlvl = std::max(maxlvl, lv);
+    }
+  }
+  // bucket targets by level (targets with no ops and no division still get
+  // scheduled at level 0: they are plain copies of A)
+  std::vector<int> lcnt(maxlvl + 2, 0);
+  for (int p = 0; p < s.nlu; ++p) lcnt[level[p] + 1]++;
+  s.f_lvl_ptr.assign(maxlvl + 2, 0);
+  for (int l = 0; l <= maxlvl; ++l) s.f_lvl_ptr[l + 1] = s.f_lvl_ptr[l] + lcnt[l + 1];
+  s.f_tgt.resize(s.nlu);
+  {
+    std::vector<int> fill(s.f_lvl_ptr.begin(), s.f_lvl_ptr.end() - 1);
+    for (int p = 0; p < s.nlu; ++p) s.f_tgt[fill[level[p]]++] = p;
+  }
+  s.f_op_ptr.assign(s.nlu + 1, 0);
+  s.f_div.resize(s.nlu);
+  s.f_ops.reserve(ops.size());
+  for (int t = 0; t < s.nlu; ++t) {
+    const int p = s.f_tgt[t];
+    for (int o = op_ptr[p]; o < op_ptr[p + 1]; ++o) { s.f_ops.push_back(ops[2 * o]); s.f_ops.push_back(ops[2 * o + 1]); }
+    s.f_op_ptr[t + 1] = (int)(s.f_ops.size() / 2);
+    s.f_div[t] = diag_of[p];
+  }
+  // ---- solve schedules
+  std::vector<int> flev(n, 0), blev(n, 0);
+  int fmax = 0, bmax = 0;
+  for (int i = 0; i < n; ++i) {
+    int lv = 0;
+    for (int k : lrow[i]) lv = std::max(lv, flev[k] + 1);
+    flev[i] = lv; fmax = std::max(fmax, lv);
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    int lv = 0;
+    for (int j : urow[i]) lv = std::max(lv, blev[j] + 1);
+    blev[i] = lv; bmax = std::max(bmax, lv);
+  }
+  auto bucket = [&](const std::vector<int>& lev, int mx, std::vector<int>& ptr, std::vector<int>& rows) {
+    ptr.assign(mx + 2, 0);
+    for (int i = 0; i < n; ++i) ptr[lev[i] + 1]++;
+    for (int l = 0; l <= mx; ++l) ptr[l + 1] += ptr[l];
+    rows.resize(n);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
+  };
+  bucket(flev, fmax, s.fs_lvl_ptr, s.fs_rows);
+  bucket(blev, bmax, s.bs_lvl_ptr, s.bs_rows);
+  return XG_OK;
+}
+
+}  // namespace xg

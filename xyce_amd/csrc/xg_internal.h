@@ -1,0 +1,101 @@
+// Internal layout shared by the host setup code and the HIP kernels.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/xyce_amd.h"
+
+namespace xg {
+
+// ---- per-class MOSFET level-1 constants (everything but VTO/KP-derived terms)
+enum MosClassField {
+  K_DTYPE = 0, K_GAMMA, K_LAMBDA, K_TPHI, K_VT, K_DSAT, K_SSAT, K_DVCRIT, K_SVCRIT,
+  K_CBD, K_CBDSW, K_CBS, K_CBSSW, K_TDEPCAP, K_TBULKPOT, K_MJ, K_MJSW,
+  K_F2D, K_F3D, K_F4D, K_F2S, K_F3S, K_F4S, K_OXCAP, K_GSO, K_GDO, K_GBO,
+  K_GD, K_GS, K_M, K_GMIN, K_COUNT
+};
+constexpr int KSTRIDE = 32;   // doubles per class record
+
+// ---- per-device contribution record planes (SoA: plane * n_dev + dev)
+// Each plane holds one term exactly as the reference adds it (sign included),
+// so a gather that sums the terms in device order reproduces the reference's
+// sequential "+=" into Epetra vectors / raw matrix pointers.
+enum RecPlane {
+  R_F_D = 0, R_F_G, R_F_S, R_F_B, R_F_DP, R_F_SP,     // F vector (6)
+  R_Q_G, R_Q_B, R_Q_DP, R_Q_SP,                       // Q vector (4)
+  R_FL_B, R_FL_DP, R_FL_SP,                           // dFdxdVp (3)
+  R_QL_G, R_QL_B, R_QL_DP, R_QL_SP,                   // dQdxdVp (4)
+  R_JF0,                                              // dFdx pointer terms 0..16
+  R_JQ0 = R_JF0 + 17,                                 // dQdx pointer terms 0..13
+  R_COUNT = R_JQ0 + 14                                // 48
+};
+// Vector id of a vector plane: 0 F, 1 Q, 2 dFdxdVp, 3 dQdxdVp.
+inline int rec_vec_id(int plane) {
+  return plane < R_Q_G ? 0 : plane < R_FL_B ? 1 : plane < R_QL_G ? 2 : 3;
+}
+
+// State / store slot order (MOSFET1 registerStateLIDs / registerStoreLIDs,
+// N_DEV_MOSFET1.C:1556-1609).
+enum StaSlot { S_QGS = 0, S_QGD, S_QGB, S_CAPGS, S_CAPGD, S_CAPGB, S_QBD, S_QBS, S_COUNT };
+enum StoSlot { O_VBD = 0, O_VBS, O_VGS, O_VDS, O_VON, O_GM, O_COUNT };
+
+// ---- source list entry (B vector)
+struct SrcEntry { int wave; int kind; int li_p; int li_n; };  // kind 0 = V (B[bra] += v), 1 = I
+// li_p for V sources is the branch row.
+
+// ---- host-side circuit after setup
+struct HostCircuit {
+  int n = 0, nnz = 0, n_mos = 0, n_nodes = 0, nsta = 0, nsto = 0;
+  double gmin = 1e-12;
+  // CSR pattern
+  std::vector<int> rowptr, colind;
+  std::vector<int> node_gid;
+  std::vector<char> var_type;          // 'V' / 'I' per unknown
+  // MOSFETs
+  std::vector<double> cls;             // n_cls * KSTRIDE
+  std::vector<int> mos_cls;
+  std::vector<double> mos_tvbi, mos_beta, mos_tvto;
+  std::vector<int> mos_lid;            // 6 per device: d g s b dp sp  (n = ground)
+  std::vector<int> mos_fslot, mos_qslot;
+  std::vector<int> mos_sta0, mos_sto0; // Xyce-layout offsets
+  std::vector<uint64_t> mos_mask;      // used record planes
+  // linear part (CSR-aligned) and sources
+  std::vector<double> linF, linQ;
+  std::vector<SrcEntry> srcs;
+  std::vector<int> wave_type, wave_off;
+  std::vector<double> wave_data;
+  std::vector<double> breakpoints;     // filled per tran run
+  // gather maps: vectors per row, matrices per slot; entry = dev * 64 + plane
+  std::vector<int> vg_ptr, vg_ent;     // n + 1
+  std::vector<int> mg_ptr, mg_ent;     // nnz + 1
+  std::string err;
+};
+
+int build_host_circuit(const xg_circuit_desc* d, HostCircuit& hc);
+void wave_breakpoints(const HostCircuit& hc, int w, double tstep, double tstop, std::vector<double>& out);
+
+// ---- sparse LU symbolic analysis (host)
+struct LuSymbolic {
+  int n = 0, nnz_a = 0, nlu = 0;
+  std::vector<int> perm_row;     // B row i  <- A row perm_row[i]
+  std::vector<int> perm_col;     // B col j  <- A col perm_col[j]
+  std::vector<int> a2lu;         // A CSR slot -> LU position
+  std::vector<int> lu_rowptr, lu_colind, lu_diag;   // row-major L+U pattern of B
+  // factor schedule (entry level)
+  std::vector<int> f_lvl_ptr;    // levels + 1 -> index into f_tgt
+  std::vector<int> f_tgt;        // LU position of target
+  std::vector<int> f_op_ptr;     // per target (same order as f_tgt) + 1
+  std::vector<int> f_ops;        // pairs (pos_l, pos_u)
+  std::vector<int> f_div;        // per target: diag position to divide by, or -1
+  // solve schedules
+  std::vector<int> fs_lvl_ptr, fs_rows;   // forward
+  std::vector<int> bs_lvl_ptr, bs_rows;   // backward
+  double flops = 0;
+  int dense_tail = 0;
+};
+
+int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s,
+               bool want_schedule = true);
+
+}  // namespace xg
