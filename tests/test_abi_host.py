@@ -1,0 +1,135 @@
+"""The C ABI library and its host-side logic (no GPU calls).
+
+* every entry point declared in include/xyce_amd.h is exported;
+* the host setup (node order, CSR pattern, stamp slots, state offsets) is
+  bit-identical to the CPU oracle's for rings, inverter chains with internal
+  drain/source nodes, power-grid meshes and gate DAGs;
+* a hand-enumerated 3-stage ring pins both against the reference's stamp
+  rules (jacStamp with RD=RS=0 merged, Vsrc branch row, ground dropped);
+* the symbolic LU: valid permutations, zero-free diagonal, fill that equals
+  a brute-force boolean elimination.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import xyce_amd as X
+from oracle.pyoracle import Oracle
+from xyce_amd import circuits as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "xyce_amd.h")).read()
+    declared = set(re.findall(r"^int (xg_\w+)\(", hdr, flags=re.M))
+    assert declared, "no declarations found"
+    L = X.lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    assert declared == set(X.EXPORTS)
+
+
+def _same_pattern(c):
+    hp = X.host_pattern(c)
+    op = Oracle(c).pattern()
+    for k in ("rowptr", "colind", "node_gid", "mos_lid", "mos_fslot", "mos_qslot", "mos_stasto"):
+        assert np.array_equal(hp[k], op[k]), k
+    return hp
+
+
+@pytest.mark.parametrize("stages", [3, 5, 11, 53, 201])
+def test_ring_pattern_matches_oracle(stages):
+    _same_pattern(C.ring(stages, seed=stages, mc_seed=stages))
+
+
+def test_internal_drain_source_nodes():
+    c = C.inverter_chain(3, RD=25.0, RS=40.0, TOX=2e-8)
+    hp = _same_pattern(c)
+    # each MOSFET gains d' and s' unknowns: 6 MOSFETs x 2 + nodes + 2 branches
+    assert hp["n"] == 6 * 2 + (c.n_nodes - 1) + 2
+    assert np.all(hp["mos_lid"][:, 4] != hp["mos_lid"][:, 0])
+
+
+def test_power_grid_and_dag_patterns_match_oracle():
+    _same_pattern(C.power_grid(8, 8, seed=2, pad_pitch=4, n_sinks=5, tstop=1e-9))
+    _same_pattern(C.gate_dag(60, seed=4))
+
+
+def test_three_stage_ring_hand_enumerated():
+    """Rows by the reference's stamp rules, independent of node numbering:
+    out_k: {out_k-1, out_k, out_k+1, vdd};  vdd: {vdd, all outs, branch};
+    branch: {vdd}."""
+    c = C.ring(3, seed=1)
+    hp = X.host_pattern(c)
+    gid = hp["node_gid"]
+    vdd = gid[1]
+    outs = [gid[2 + k] for k in range(3)]
+    bra = ({*range(hp["n"])} - {vdd, *outs}).pop()
+    rows = {r: set(hp["colind"][hp["rowptr"][r]:hp["rowptr"][r + 1]]) for r in range(hp["n"])}
+    for k in range(3):
+        assert rows[outs[k]] == {outs[k - 1], outs[k], outs[(k + 1) % 3], vdd}
+    assert rows[vdd] == {vdd, *outs, bra}
+    assert rows[bra] == {vdd}
+    assert hp["nnz"] == 5 * 3 + 3
+    # NMOS of stage 1: drain out_1, gate out_3, source/bulk ground (dummy slot -1)
+    lid = hp["mos_lid"][0]
+    assert lid[0] == outs[0] and lid[1] == outs[2] and lid[2] == hp["n"] and lid[3] == hp["n"]
+    fs = hp["mos_fslot"][0]
+    rp, ci = hp["rowptr"], hp["colind"]
+    slot = lambda r, col: rp[r] + int(np.where(ci[rp[r]:rp[r + 1]] == col)[0][0])
+    # f_DrainPrimeEquGateNodePtr (index 8) -> (out_1, out_3); source rows -> ground
+    assert fs[8] == slot(outs[0], outs[2])
+    assert fs[10] == slot(outs[0], outs[0])
+    assert fs[12] == -1 and fs[16] == -1
+
+
+def _dense_of(n, rp, ci, vals=None):
+    A = np.zeros((n, n))
+    for r in range(n):
+        for k in range(rp[r], rp[r + 1]):
+            A[r, ci[k]] = 1.0 if vals is None else vals[k]
+    return A
+
+
+@pytest.mark.parametrize("maker", [lambda: C.ring(7, seed=1), lambda: C.ring(41, seed=2),
+                                   lambda: C.power_grid(6, 6, seed=2, pad_pitch=3, n_sinks=3),
+                                   lambda: C.gate_dag(40, seed=1)])
+def test_symbolic_lu(maker):
+    c = maker()
+    hp = X.host_pattern(c)
+    n = hp["n"]
+    s = X.host_lu_analyse(n, hp["rowptr"], hp["colind"])
+    pr, pc = s["perm_row"], s["perm_col"]
+    assert sorted(pr) == list(range(n)) and sorted(pc) == list(range(n))
+    A = _dense_of(n, hp["rowptr"], hp["colind"]) != 0
+    B = A[np.ix_(pr, pc)]
+    assert np.all(np.diag(B)), "transversal left a structural zero on the diagonal"
+    # brute-force fill of the symmetric pattern of B
+    F = B | B.T
+    for k in range(n):
+        rows = np.where(F[k + 1:, k])[0] + k + 1
+        cols = np.where(F[k, k + 1:])[0] + k + 1
+        F[np.ix_(rows, cols)] = True
+    P = np.zeros((n, n), bool)
+    lrp, lci = s["lu_rowptr"], s["lu_colind"]
+    for i in range(n):
+        P[i, lci[lrp[i]:lrp[i + 1]]] = True
+    assert np.array_equal(P, F)
+    assert s["nnz_lu"] == int(F.sum())
+
+
+def test_value_steered_transversal_avoids_tiny_diagonal():
+    """With values, a diagonal far below its column maximum is not used as a
+    pivot (the ring at its DC point: gmin-sized diagonals, gm off-diagonals)."""
+    n = 4
+    # cyclic matrix with tiny diagonal
+    rp = np.array([0, 2, 4, 6, 8], np.int32)
+    ci = np.array([0, 3, 0, 1, 1, 2, 2, 3], np.int32)
+    vals = np.array([1e-12, 1e-4, 1e-4, 1e-12, 1e-4, 1e-12, 1e-4, 1e-12])
+    s = X.host_lu_analyse(n, rp, ci, values=vals)
+    A = _dense_of(n, rp, ci, vals)
+    B = A[np.ix_(s["perm_row"], s["perm_col"])]
+    assert np.all(np.abs(np.diag(B)) >= 1e-4)

@@ -75,6 +75,36 @@ def test_load_parity(gpu, stages, seed, newton_iter, order):
     assert entry_tol(J_g, J_o, s_j, abs_floor=1e-30) <= 1
 
 
+CAPS = dict(TOX=2e-8, CJ=2e-4, CJSW=1e-9, CGSO=3e-10, CGDO=3e-10, CGBO=2e-10, GAMMA=0.4, LAMBDA=0.03)
+
+
+@pytest.mark.parametrize("newton_iter,order", [(0, 1), (2, 2)])
+def test_load_parity_with_caps(gpu, newton_iter, order):
+    """Meyer gate capacitances, junction depletion charges and overlap caps on."""
+    c = C.inverter_chain(4, **CAPS)
+    d = X.DeviceMgr(c)
+    o = Oracle(c)
+    x, sta0, sto0 = _loaded_state(o, 11)
+    kw = dict(dcop=0, tran=1, init_jct=0, newton_iter=newton_iter, order=order, h=3e-12, time=0.4e-9,
+              tstep=1e-12, tstop=5e-9)
+    sta0 = sta0 * 1.01
+    ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    d.set_solver_state(**kw)
+    sta_n, sto_n, lim = d.updateState(x, sta0, sto0, sto_next=sto0)
+    assert lim == ref["limited"]
+    assert entry_tol(sta_n, ref["sta_next"], np.abs(ref["sta_next"]) + 1e-300) <= 1
+    vec = d.loadDAEVectors()
+    for k in ("F", "Q", "B", "dFdxdVp", "dQdxdVp"):
+        a, b = vec[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1, k
+    assert np.max(np.abs(ref["Q"])) > 0 and np.max(np.abs(ref["dQdx"])) > 0
+    dF, dQ = d.loadDAEMatrices()
+    for a, b in ((dF, ref["dFdx"]), (dQ, ref["dQdx"])):
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
+
+
 def test_dc_load_parity(gpu):
     c = C.ring(7, seed=9, mc_seed=9)
     d = X.DeviceMgr(c)
@@ -135,6 +165,40 @@ def test_batch_single_transient_matches_oracle(gpu, stages, tstop):
     st, _ = b.instance(0, o.n)
     # same algorithm: same step count up to rounding-driven differences
     assert abs(st["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
-    volt = [j for j in range(o.n) if j != int(np.argmax(np.abs(r["x"][-1])) * 0 - 1)]
-    err = waveform_error(t_g, x_g[:, :], r["t"], r["x"][:, :])
-    assert err <= 1e-3 * 1.8 + 1e-6 + 5e-3, err
+    err = waveform_error(t_g, x_g, r["t"], r["x"])
+    print(f"steps gpu {st['steps']} oracle {r['steps']}  max waveform diff {err:.3e} V")
+    assert err <= 1e-3 * 1.8 + 1e-6, err
+
+
+def test_batch_with_caps_matches_oracle(gpu):
+    """Driven inverter chain with every capacitance on: Meyer back-averaging,
+    depletion charges, overlap caps, PULSE input breakpoints."""
+    c = C.inverter_chain(4, **CAPS)
+    tstop = 4e-9
+    b = X.Batch([c], tstop, 1e-12, 1e-11, record_max=20000)
+    cnt = b.run_to_end(chunk=512)
+    assert cnt["done"] == 1 and cnt["failed"] == 0, cnt
+    o = Oracle(c)
+    r = o.tran(tstop, 1e-12, 1e-11)
+    assert r["status"] == 0
+    t_g, x_g = b.points(0, o.n)
+    err = waveform_error(t_g, x_g, r["t"], r["x"])
+    print(f"caps: steps gpu {len(t_g)} oracle {len(r['t'])}  max waveform diff {err:.3e}")
+    assert err <= 1e-3 * 1.8 + 1e-6, err
+
+
+def test_batch_uneven_instances_match_individual_runs(gpu):
+    """A batch of rings of different sizes gives each instance the same
+    trajectory as running it alone (instances are independent)."""
+    cs = [C.ring(s, seed=100 + s, mc_seed=100 + s) for s in (11, 31, 7, 61)]
+    tstop = 1e-9
+    b = X.Batch(cs, tstop, 1e-12, 1e-11, record_max=4000)
+    cnt = b.run_to_end(chunk=256)
+    assert cnt["done"] == 4 and cnt["failed"] == 0, cnt
+    for i, c in enumerate(cs):
+        b1 = X.Batch([c], tstop, 1e-12, 1e-11, record_max=4000)
+        b1.run_to_end(chunk=256)
+        n = X.host_pattern(c)["n"]
+        t_a, x_a = b.points(i, n)
+        t_b, x_b = b1.points(0, n)
+        assert np.array_equal(t_a, t_b) and np.array_equal(x_a, x_b)

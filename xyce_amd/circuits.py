@@ -348,21 +348,23 @@ def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5) -> C
     return c
 
 
-def inverter_chain(stages: int = 3, vin_wave=None) -> Circuit:
-    """Small driven inverter chain used by unit tests."""
+def inverter_chain(stages: int = 3, vin_wave=None, **model_extra) -> Circuit:
+    """Small driven inverter chain used by unit tests; ``model_extra`` adds
+    .model parameters to both cards (e.g. TOX, CJ, CGSO to turn on the Meyer
+    and junction capacitances)."""
     c = Circuit()
     vdd = c.node("vdd")
     inp = c.node("in")
-    mn = c.add_model(model_card(+1, **NMOS_RING))
-    mp = c.add_model(model_card(-1, **PMOS_RING))
+    mn = c.add_model(model_card(+1, **NMOS_RING, **model_extra))
+    mp = c.add_model(model_card(-1, **PMOS_RING, **model_extra))
     c.add_vsrc(vdd, 0, c.add_wave(W_DC, [VDD]))
     w = c.add_wave(W_PULSE, [0.0, VDD, 0.2e-9, 50e-12, 50e-12, 1e-9, 3e-9]) if vin_wave is None else vin_wave
     c.add_vsrc(inp, 0, w)
     prev = inp
     for k in range(stages):
         out = c.node(f"o{k}")
-        c.add_mos(out, prev, 0, 0, mn, inst_card(W=2e-6, L=0.5e-6))
-        c.add_mos(out, prev, vdd, vdd, mp, inst_card(W=4e-6, L=0.5e-6))
+        c.add_mos(out, prev, 0, 0, mn, inst_card(W=2e-6, L=0.5e-6, AD=4e-12, AS=4e-12, PD=8e-6, PS=8e-6))
+        c.add_mos(out, prev, vdd, vdd, mp, inst_card(W=4e-6, L=0.5e-6, AD=8e-12, AS=8e-12, PD=12e-6, PS=12e-6))
         c.add_cap(out, 0, 2e-15)
         prev = out
     return c

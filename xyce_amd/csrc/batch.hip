@@ -44,10 +44,14 @@ namespace xg {
 
 // ---- per structure class (offsets into the concatenated int arrays)
 struct ClassDesc {
-  int n, nnz, nlu, n_mos, n_src;
+  int n, nnz, nlu, n_mos, n_src, nops, nterm;
   int nflev, nfs, nbs;
-  int o_rowptr, o_colind, o_vgp, o_vge, o_mgp, o_mge, o_bgp, o_bge, o_lid;
-  int o_a2lu, o_lurp, o_luci, o_diag, o_flp, o_ftg, o_fop, o_fops, o_fdiv, o_fsp, o_fsr, o_bsp, o_bsr, o_prow, o_pcol;
+  // int32 arrays (global): device node indices, term positions, term ranges, sources
+  int o_lid, o_tpos, o_vptr, o_mptr, o_bgp, o_bge;
+  // uint16 program blob (staged into LDS each launch); offsets in uint16 units
+  int blob_off, blob_len;
+  int b_flp, b_ftg, b_fop, b_fdiv, b_fops, b_lurp, b_luci, b_diag, b_fsp, b_fsr, b_bsp, b_bsr, b_prow, b_pcol,
+      b_rp, b_cc, b_a2lu;
 };
 
 // ---- per instance state (one 512-byte record in HBM)
@@ -70,6 +74,7 @@ struct InstState {
 struct InstDesc {
   int cls, dev_off, vec_off, nnz_off, src_off, rec_off;
   long long dense_off;   // n*n scratch for the partial-pivoting fallback
+  long long term_off;    // this instance's slot-ordered term array
 };
 
 struct BatchArgs {
@@ -83,7 +88,9 @@ struct BatchArgs {
   int nt_mos;
   const double* kcls; const int* mcls; const double* tvbi; const double* beta; const double* tvto;
   const unsigned long long* mask;
-  double* sto_cur; double* sto_nxt; double* sta_cur; double* sta_nxt; double* rec;
+  double* sto_cur; double* sto_nxt; double* sta_cur; double* sta_nxt;
+  double* terms;               // per-instance term arrays (slot order)
+  const unsigned short* blob;  // per-class uint16 programs
   // per-instance vectors (offset vec_off, length n)
   double* x; double* xn0; double* xH0; double* xH1; double* xH2; double* qH0; double* qH1; double* qH2;
   double* F; double* Q; double* B; double* F0; double* B0; double* wt;
@@ -96,7 +103,7 @@ struct BatchArgs {
   double tstop, tstep, hmax;
   // recording
   int rec_max; double* rec_t; double* rec_x;
-  int max_n, max_nlu;
+  int max_n, max_nlu, max_blob;
   double* dense;           // per-instance n*n scratch
   int* ipiv_scratch;       // unused (row swaps are applied in place)
 };
@@ -214,23 +221,20 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
 // (N_LAS_AmesosSolver.C:318) would pick different pivots.  Returns nonzero if
 // a zero pivot is met (singular).
 __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const InstDesc& I, const int* ci,
-                                 double ja, double jb, double* rhs, double* red, int* sh_i) {
+                                 const unsigned short* prog, double ja, double jb, double* rhs, double* red, int* sh_i) {
   const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
   double* D = A.dense + I.dense_off;
-  const int* rp = ci + C.o_rowptr; const int* cc = ci + C.o_colind;
-  const int* mgp = ci + C.o_mgp; const int* mge = ci + C.o_mge;
+  const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
+  const int* mptr = ci + C.o_mptr;
+  const double* T = A.terms + I.term_off;
   const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
-  const int NT = A.nt_mos;
   for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
   __syncthreads();
   for (int r = tid; r < n; r += nth)
     for (int s = rp[r]; s < rp[r + 1]; ++s) {
       double df = 0.0, dq = 0.0;
-      for (int e = mgp[s]; e < mgp[s + 1]; ++e) {
-        const int ent = mge[e];
-        const double v = A.rec[(size_t)(ent & 63) * NT + I.dev_off + (ent >> 6)];
-        if ((ent & 63) < R_JQ0) df += v; else dq += v;
-      }
+      for (int e = mptr[2 * s]; e < mptr[2 * s + 1]; ++e) df += T[e];
+      for (int e = mptr[2 * s + 1]; e < mptr[2 * s + 2]; ++e) dq += T[e];
       if (linF[s] != 0.0) df += linF[s];
       if (linQ[s] != 0.0) dq += linQ[s];
       D[(size_t)r * n + cc[s]] = jac_combine(ja, dq, jb, df);
@@ -302,6 +306,8 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   double* rhs = xs + (A.max_n + 1);         // n
   double* y = rhs + A.max_n;                // n
   double* lu = y + A.max_n;                 // nlu
+  // class program after the doubles, on a 16-byte boundary (uint4 copies)
+  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.max_n + 1) + A.max_nlu + 1) & ~1));
   const int tid = threadIdx.x, nth = blockDim.x;
   if (tid == 0) S = A.st[inst];
   if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
@@ -311,15 +317,24 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   const int* ci = A.ci;
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
-  // ---- 1. x into LDS
+  double* T = A.terms + I.term_off;
+  // ---- 1. stage the class program (uint16, 16-byte chunks) and x into LDS
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(A.blob + C.blob_off);
+    uint4* dst = reinterpret_cast<uint4*>(prog);
+    const int chunks = C.blob_len / 8;
+    for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
+  }
   for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
   if (tid == 0) xs[n] = 0.0;
   __syncthreads();
-  // ---- 2. device evaluation (one thread per MOSFET)
+  // ---- 2. device evaluation (one thread per MOSFET); each term goes to its
+  //         slot-ordered position in the instance's term array
   {
     MosFlags fl;
     fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0; fl.newton_iter = S.newton_iter;
     const int* lid = ci + C.o_lid;
+    const int* tpos = ci + C.o_tpos;
     int lim_any = 0;
     for (int d = tid; d < C.n_mos; d += nth) {
       const int g = I.dev_off + d;
@@ -328,9 +343,8 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       for (int j = 0; j < 6; ++j) volt[j] = xs[lid[6 * d + j]];
       for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
       for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
-      const unsigned long long m = A.mask[g];
-      double* rec = A.rec;
-      auto emit = [&](int plane, double v) { if (m >> plane & 1ull) rec[(size_t)plane * NT + g] = v; };
+      const int* tp = tpos + (size_t)d * R_COUNT;
+      auto emit = [&](int plane, double v) { const int p = tp[plane]; if (p >= 0) T[p] = v; };
       lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
       for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
       for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
@@ -339,24 +353,24 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   }
   __syncthreads();
   const int limited = sh_int[0];
-  // ---- 3. vectors per row and the residual
+  const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
+  // ---- 3. vectors per row (contiguous term ranges) and the residual
   const double h = S.h;
   const double tsrc = dc ? 0.0 : S.t + S.h;
   double n2 = 0.0, ninf = 0.0;
   {
-    const int* rp = ci + C.o_rowptr; const int* cc = ci + C.o_colind;
-    const int* vgp = ci + C.o_vgp; const int* vge = ci + C.o_vge;
+    const int* vptr = ci + C.o_vptr;
     const int* bgp = ci + C.o_bgp; const int* bge = ci + C.o_bge;
     const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
     for (int r = tid; r < n; r += nth) {
-      double f = 0.0, q = 0.0, fl = 0.0, ql = 0.0;
-      for (int e = vgp[r]; e < vgp[r + 1]; ++e) {
-        const int ent = vge[e];
-        const int dev = I.dev_off + (ent >> 6), plane = ent & 63;
-        const double v = A.rec[(size_t)plane * NT + dev];
-        const int vid = d_rec_vec_id(plane);
-        if (vid == 0) f += v; else if (vid == 1) q += v; else if (vid == 2) fl += v; else ql += v;
+      double acc[4];
+      for (int v = 0; v < 4; ++v) {
+        double s = 0.0;
+        for (int e = vptr[4 * r + v]; e < vptr[4 * r + v + 1]; ++e) s += T[e];
+        acc[v] = s;
       }
+      double f = acc[0], q = acc[1];
+      const double fl = acc[2], ql = acc[3];
       double sf = 0.0, sq = 0.0;
       for (int k = rp[r]; k < rp[r + 1]; ++k) {
         const double xv = xs[cc[k]];
@@ -436,15 +450,13 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       for (int p = tid; p < nlu; p += nth) lu[p] = 0.0;
       __syncthreads();
       {
-        const int* mgp = ci + C.o_mgp; const int* mge = ci + C.o_mge; const int* a2lu = ci + C.o_a2lu;
+        const int* mptr = ci + C.o_mptr;
+        const unsigned short* a2lu = prog + C.b_a2lu;
         const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
         for (int s = tid; s < C.nnz; s += nth) {
           double df = 0.0, dq = 0.0;
-          for (int e = mgp[s]; e < mgp[s + 1]; ++e) {
-            const int ent = mge[e];
-            const double v = A.rec[(size_t)(ent & 63) * NT + I.dev_off + (ent >> 6)];
-            if ((ent & 63) < R_JQ0) df += v; else dq += v;
-          }
+          for (int e = mptr[2 * s]; e < mptr[2 * s + 1]; ++e) df += T[e];
+          for (int e = mptr[2 * s + 1]; e < mptr[2 * s + 2]; ++e) dq += T[e];
           if (linF[s] != 0.0) df += linF[s];
           if (linQ[s] != 0.0) dq += linQ[s];
           lu[a2lu[s]] = jac_combine(ja, dq, jb, df);
@@ -452,15 +464,16 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       }
       __syncthreads();
       {
-        const int* flp = ci + C.o_flp; const int* ftg = ci + C.o_ftg; const int* fop = ci + C.o_fop;
-        const int* fops = ci + C.o_fops; const int* fdiv = ci + C.o_fdiv;
+        const unsigned short* flp = prog + C.b_flp; const unsigned short* ftg = prog + C.b_ftg;
+        const unsigned short* fop = prog + C.b_fop; const unsigned short* fops = prog + C.b_fops;
+        const unsigned short* fdiv = prog + C.b_fdiv;
         for (int l = 0; l < C.nflev; ++l) {
           for (int t = flp[l] + tid; t < flp[l + 1]; t += nth) {
             const int p = ftg[t];
             double v = lu[p];
             for (int o = fop[t]; o < fop[t + 1]; ++o) v -= lu[fops[2 * o]] * lu[fops[2 * o + 1]];
             const int dv = fdiv[t];
-            if (dv >= 0) v /= lu[dv];
+            if (dv != 0xFFFF) v /= lu[dv];
             lu[p] = v;
           }
           __syncthreads();
@@ -468,7 +481,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       }
       // KLU's threshold test on the static pivots: |l_ij| <= 1/0.001, finite, nonzero pivot
       {
-        const int* lurp = ci + C.o_lurp; const int* dg = ci + C.o_diag;
+        const unsigned short* lurp = prog + C.b_lurp; const unsigned short* dg = prog + C.b_diag;
         int bad = 0;
         for (int i = tid; i < n; i += nth) {
           const double dgv = lu[dg[i]];
@@ -481,9 +494,11 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
         use_dense = sh_int[6] != 0;
       }
       if (!use_dense) {
-        const int* prow = ci + C.o_prow; const int* pcol = ci + C.o_pcol;
-        const int* lurp = ci + C.o_lurp; const int* luci = ci + C.o_luci; const int* dg = ci + C.o_diag;
-        const int* fsp = ci + C.o_fsp; const int* fsr = ci + C.o_fsr; const int* bsp = ci + C.o_bsp; const int* bsr = ci + C.o_bsr;
+        const unsigned short* prow = prog + C.b_prow; const unsigned short* pcol = prog + C.b_pcol;
+        const unsigned short* lurp = prog + C.b_lurp; const unsigned short* luci = prog + C.b_luci;
+        const unsigned short* dg = prog + C.b_diag;
+        const unsigned short* fsp = prog + C.b_fsp; const unsigned short* fsr = prog + C.b_fsr;
+        const unsigned short* bsp = prog + C.b_bsp; const unsigned short* bsr = prog + C.b_bsr;
         for (int i = tid; i < n; i += nth) y[i] = rhs[prow[i]];
         __syncthreads();
         for (int l = 0; l < C.nfs; ++l) {
@@ -510,7 +525,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     }
     if (use_dense) {
       if (tid == 0) S.n_dense++;
-      const int sing = dense_pivot_solve(A, C, I, ci, ja, jb, rhs, red, red_i);
+      const int sing = dense_pivot_solve(A, C, I, ci, prog, ja, jb, rhs, red, red_i);
       if (sing && tid == 0) sh_int[2] |= 1;
       __syncthreads();
     }
@@ -748,7 +763,8 @@ struct xg_batch {
   DArr<InstDesc> d_inst;
   DArr<ClassDesc> d_cls;
   DArr<InstState> d_st;
-  DArr<double> d_kcls, d_tvbi, d_beta, d_tvto, d_sto_cur, d_sto_nxt, d_sta_cur, d_sta_nxt, d_rec;
+  DArr<double> d_kcls, d_tvbi, d_beta, d_tvto, d_sto_cur, d_sto_nxt, d_sta_cur, d_sta_nxt, d_terms;
+  DArr<unsigned short> d_blob;
   DArr<unsigned long long> d_mask;
   DArr<double> d_x, d_xn0, d_xH0, d_xH1, d_xH2, d_qH0, d_qH1, d_qH2, d_F, d_Q, d_B, d_F0, d_B0, d_wt;
   DArr<double> d_linF, d_linQ, d_wave_data, d_bp, d_rec_t, d_rec_x, d_dense;
@@ -806,15 +822,36 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   std::vector<unsigned long long> mask;
   std::map<std::vector<double>, int> kindex;
   int dev_off = 0, vec_off = 0, nnz_off = 0, src_off = 0;
-  long long dense_total = 0;
+  long long dense_total = 0, term_total = 0;
+  int max_blob = 0;
+  std::vector<unsigned short> blob;
   int max_n = 0, max_nlu = 0;
   std::vector<double> all_bp;
   for (int i = 0; i < n_inst; ++i) {
     HostCircuit hc;
     int st = build_host_circuit(&descs[i], hc);
     if (st != XG_OK) return st;
-    // structure key: pattern + gather maps + node indices + source rows
+    // Effective term masks: the record planes that can be nonzero for each
+    // device given its class constants (identically-zero terms are neither
+    // written nor summed; adding +0.0 would not change any sum).
+    std::vector<unsigned long long> emask(hc.n_mos);
+    for (int d = 0; d < hc.n_mos; ++d) {
+      const double* k = &hc.cls[(size_t)hc.mos_cls[d] * KSTRIDE];
+      unsigned long long m = hc.mos_mask[d];
+      m &= ~(1ull << R_F_G);                               // old-Meyer gate current term is 0
+      if (k[K_GD] == 0.0) m &= ~((1ull << R_F_D) | (1ull << (R_JF0 + 0)) | (1ull << (R_JF0 + 1)) | (1ull << (R_JF0 + 7)));
+      if (k[K_GS] == 0.0) m &= ~((1ull << R_F_S) | (1ull << (R_JF0 + 2)) | (1ull << (R_JF0 + 3)) | (1ull << (R_JF0 + 13)));
+      const bool no_caps = k[K_OXCAP] == 0.0 && k[K_GSO] == 0.0 && k[K_GDO] == 0.0 && k[K_GBO] == 0.0 &&
+                           k[K_CBD] == 0.0 && k[K_CBDSW] == 0.0 && k[K_CBS] == 0.0 && k[K_CBSSW] == 0.0;
+      if (no_caps) {
+        for (int p : {R_Q_G, R_Q_B, R_Q_DP, R_Q_SP, R_QL_G, R_QL_B, R_QL_DP, R_QL_SP}) m &= ~(1ull << p);
+        for (int j = 0; j < 14; ++j) m &= ~(1ull << (R_JQ0 + j));
+      }
+      emask[d] = m;
+    }
+    // structure key: pattern + gather maps + node indices + term masks + source rows
     std::vector<int> key;
+    for (auto m : emask) { key.push_back((int)(m & 0xffffffffu)); key.push_back((int)(m >> 32)); }
     key.push_back(hc.n); key.push_back(hc.nnz); key.push_back(hc.n_mos); key.push_back((int)hc.srcs.size());
     key.insert(key.end(), hc.rowptr.begin(), hc.rowptr.end());
     key.insert(key.end(), hc.colind.begin(), hc.colind.end());
@@ -846,11 +883,37 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       C.nfs = (int)sym.fs_lvl_ptr.size() - 1;
       C.nbs = (int)sym.bs_lvl_ptr.size() - 1;
       auto push = [&](const std::vector<int>& v) { int o = (int)ci.size(); ci.insert(ci.end(), v.begin(), v.end()); return o; };
-      C.o_rowptr = push(hc.rowptr); C.o_colind = push(hc.colind);
-      C.o_vgp = push(hc.vg_ptr); C.o_vge = push(hc.vg_ent); C.o_mgp = push(hc.mg_ptr); C.o_mge = push(hc.mg_ent);
+      const int n = hc.n;
+      // ---- slot-ordered term layout: vector terms grouped by (row, vector),
+      //      matrix terms by (slot, dFdx/dQdx), each group in device order
+      static const int vrow[R_JF0] = {0, 1, 2, 3, 4, 5, 1, 3, 4, 5, 3, 4, 5, 1, 3, 4, 5};
+      std::vector<int> vcnt(4 * n + 1, 0), mcnt(2 * hc.nnz + 1, 0);
+      auto vgroup = [&](int d, int p) { return 4 * hc.mos_lid[(size_t)6 * d + vrow[p]] + rec_vec_id(p); };
+      auto mgroup = [&](int d, int p) {
+        return p < R_JQ0 ? 2 * hc.mos_fslot[(size_t)17 * d + (p - R_JF0)] : 2 * hc.mos_qslot[(size_t)14 * d + (p - R_JQ0)] + 1;
+      };
+      for (int d = 0; d < hc.n_mos; ++d)
+        for (int p = 0; p < R_COUNT; ++p) {
+          if (!(emask[d] >> p & 1ull)) continue;
+          if (p < R_JF0) vcnt[vgroup(d, p)]++; else mcnt[mgroup(d, p)]++;
+        }
+      std::vector<int> vptr(4 * n + 1, 0), mptr(2 * hc.nnz + 1, 0);
+      for (int g = 0; g < 4 * n; ++g) vptr[g + 1] = vptr[g] + vcnt[g];
+      mptr[0] = vptr[4 * n];
+      for (int g = 0; g < 2 * hc.nnz; ++g) mptr[g + 1] = mptr[g] + mcnt[g];
+      C.nterm = mptr[2 * hc.nnz];
+      std::vector<int> tpos((size_t)hc.n_mos * R_COUNT, -1);
+      {
+        std::vector<int> vf(vptr.begin(), vptr.end() - 1), mf(mptr.begin(), mptr.end() - 1);
+        for (int d = 0; d < hc.n_mos; ++d)
+          for (int p = 0; p < R_COUNT; ++p) {
+            if (!(emask[d] >> p & 1ull)) continue;
+            tpos[(size_t)d * R_COUNT + p] = p < R_JF0 ? vf[vgroup(d, p)]++ : mf[mgroup(d, p)]++;
+          }
+      }
+      C.o_tpos = push(tpos); C.o_vptr = push(vptr); C.o_mptr = push(mptr);
       // per-row source lists (local source index)
       {
-        const int n = hc.n;
         std::vector<int> cnt(n + 1, 0);
         for (auto& e : hc.srcs) {
           if (e.li_p < n) cnt[e.li_p]++;
@@ -870,11 +933,26 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         C.o_bgp = push(ptr); C.o_bge = push(ent);
       }
       C.o_lid = push(hc.mos_lid);
-      C.o_a2lu = push(sym.a2lu); C.o_lurp = push(sym.lu_rowptr); C.o_luci = push(sym.lu_colind);
-      C.o_diag = push(sym.lu_diag); C.o_flp = push(sym.f_lvl_ptr); C.o_ftg = push(sym.f_tgt);
-      C.o_fop = push(sym.f_op_ptr); C.o_fops = push(sym.f_ops); C.o_fdiv = push(sym.f_div);
-      C.o_fsp = push(sym.fs_lvl_ptr); C.o_fsr = push(sym.fs_rows); C.o_bsp = push(sym.bs_lvl_ptr);
-      C.o_bsr = push(sym.bs_rows); C.o_prow = push(sym.perm_row); C.o_pcol = push(sym.perm_col);
+      C.nops = (int)sym.f_ops.size() / 2;
+      // ---- uint16 program blob, staged into LDS by every launch
+      {
+        std::vector<int> fdiv16(sym.f_div.size());
+        for (size_t t = 0; t < sym.f_div.size(); ++t) fdiv16[t] = sym.f_div[t] < 0 ? 0xFFFF : sym.f_div[t];
+        const size_t b0 = blob.size();
+        auto put = [&](const std::vector<int>& v) {
+          int o = (int)(blob.size() - b0);
+          for (int x : v) blob.push_back((unsigned short)x);
+          return o;
+        };
+        C.b_flp = put(sym.f_lvl_ptr); C.b_ftg = put(sym.f_tgt); C.b_fop = put(sym.f_op_ptr); C.b_fdiv = put(fdiv16);
+        C.b_fops = put(sym.f_ops); C.b_lurp = put(sym.lu_rowptr); C.b_luci = put(sym.lu_colind);
+        C.b_diag = put(sym.lu_diag); C.b_fsp = put(sym.fs_lvl_ptr); C.b_fsr = put(sym.fs_rows);
+        C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(sym.bs_rows); C.b_prow = put(sym.perm_row);
+        C.b_pcol = put(sym.perm_col); C.b_rp = put(hc.rowptr); C.b_cc = put(hc.colind); C.b_a2lu = put(sym.a2lu);
+        while ((blob.size() - b0) % 8) blob.push_back(0);
+        C.blob_off = (int)b0;
+        C.blob_len = (int)(blob.size() - b0);
+      }
       // Every index the kernel reads is checked here, on the host, before any launch.
       {
         auto in = [](const std::vector<int>& v, int lo, int hi) {
@@ -884,17 +962,18 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         bool ok = in(hc.colind, 0, C.n) && in(hc.mos_lid, 0, C.n + 1) && in(sym.a2lu, 0, C.nlu) &&
                   in(sym.lu_colind, 0, C.n) && in(sym.lu_diag, 0, C.nlu) && in(sym.f_tgt, 0, C.nlu) &&
                   in(sym.f_ops, 0, C.nlu) && in(sym.f_div, -1, C.nlu) && in(sym.fs_rows, 0, C.n) &&
-                  in(sym.bs_rows, 0, C.n) && in(sym.perm_row, 0, C.n) && in(sym.perm_col, 0, C.n);
-        for (int e : hc.vg_ent) ok = ok && (e >> 6) < C.n_mos && (e & 63) < R_JF0;
-        for (int e : hc.mg_ent) ok = ok && (e >> 6) < C.n_mos && (e & 63) >= R_JF0 && (e & 63) < R_COUNT;
-        ok = ok && (int)hc.vg_ptr.size() == C.n + 1 && hc.vg_ptr[C.n] == (int)hc.vg_ent.size();
-        ok = ok && (int)hc.mg_ptr.size() == C.nnz + 1 && hc.mg_ptr[C.nnz] == (int)hc.mg_ent.size();
+                  in(sym.bs_rows, 0, C.n) && in(sym.perm_row, 0, C.n) && in(sym.perm_col, 0, C.n) &&
+                  in(tpos, -1, C.nterm);
+        // every value of the uint16 program must fit (0xFFFF is the "no division" sentinel)
+        ok = ok && C.nlu < 0xFFFF && (int)sym.f_ops.size() / 2 < 0xFFFF && C.nnz < 0xFFFF && C.n < 0xFFFF;
         ok = ok && (int)sym.f_op_ptr.size() == C.nlu + 1 && sym.f_op_ptr[C.nlu] * 2 == (int)sym.f_ops.size();
         ok = ok && sym.f_lvl_ptr.back() == C.nlu && sym.fs_lvl_ptr.back() == C.n && sym.bs_lvl_ptr.back() == C.n;
         ok = ok && (int)hc.rowptr.size() == C.n + 1 && hc.rowptr[C.n] == C.nnz;
+        ok = ok && vptr[0] == 0 && mptr[2 * hc.nnz] == C.nterm;
         for (auto& e : hc.srcs) ok = ok && e.li_p <= C.n && e.li_n <= C.n && e.wave >= 0 && e.wave < (int)hc.wave_type.size();
         if (!ok) return XG_E_SETUP;
       }
+      max_blob = std::max(max_blob, C.blob_len);
       b->classes.push_back(C);
       max_n = std::max(max_n, C.n);
       max_nlu = std::max(max_nlu, C.nlu);
@@ -906,6 +985,8 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
     D.rec_off = i * std::max(record_max, 0);
     D.dense_off = dense_total;
     dense_total += (long long)hc.n * hc.n;
+    D.term_off = term_total;
+    term_total += b->classes[cid].nterm;
     b->insts.push_back(D);
     b->cls_n.push_back(hc.n_mos + hc.n);
     // devices
@@ -947,7 +1028,8 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   HIPCHK(b->d_tvto.up(tvto)); HIPCHK(b->d_mask.up(mask));
   HIPCHK(b->d_sto_cur.zeros(6 * NT)); HIPCHK(b->d_sto_nxt.zeros(6 * NT));
   HIPCHK(b->d_sta_cur.zeros(8 * NT)); HIPCHK(b->d_sta_nxt.zeros(8 * NT));
-  HIPCHK(b->d_rec.zeros((size_t)R_COUNT * NT));
+  HIPCHK(b->d_terms.zeros((size_t)std::max(1LL, term_total)));
+  HIPCHK(b->d_blob.up(blob));
   const size_t NV = vec_off;
   for (auto* v : {&b->d_x, &b->d_xn0, &b->d_xH0, &b->d_xH1, &b->d_xH2, &b->d_qH0, &b->d_qH1, &b->d_qH2, &b->d_F,
                   &b->d_Q, &b->d_B, &b->d_F0, &b->d_B0, &b->d_wt})
@@ -968,16 +1050,17 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.n_inst = n_inst; A.order = b->d_order.p; A.inst = b->d_inst.p; A.st = b->d_st.p; A.cls = b->d_cls.p; A.ci = b->d_ci.p;
   A.nt_mos = (int)NT; A.kcls = b->d_kcls.p; A.mcls = b->d_mcls.p; A.tvbi = b->d_tvbi.p; A.beta = b->d_beta.p;
   A.tvto = b->d_tvto.p; A.mask = b->d_mask.p; A.sto_cur = b->d_sto_cur.p; A.sto_nxt = b->d_sto_nxt.p;
-  A.sta_cur = b->d_sta_cur.p; A.sta_nxt = b->d_sta_nxt.p; A.rec = b->d_rec.p;
+  A.sta_cur = b->d_sta_cur.p; A.sta_nxt = b->d_sta_nxt.p; A.terms = b->d_terms.p; A.blob = b->d_blob.p;
   A.x = b->d_x.p; A.xn0 = b->d_xn0.p; A.xH0 = b->d_xH0.p; A.xH1 = b->d_xH1.p; A.xH2 = b->d_xH2.p; A.qH0 = b->d_qH0.p;
   A.qH1 = b->d_qH1.p; A.qH2 = b->d_qH2.p; A.F = b->d_F.p; A.Q = b->d_Q.p; A.B = b->d_B.p; A.F0 = b->d_F0.p;
   A.B0 = b->d_B0.p; A.wt = b->d_wt.p; A.linF = b->d_linF.p; A.linQ = b->d_linQ.p; A.src_wave = b->d_src_wave.p;
   A.wave_type = b->d_wave_type.p; A.wave_off = b->d_wave_off.p; A.wave_data = b->d_wave_data.p;
   A.n_bp = (int)all_bp.size(); A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
   A.rec_max = std::max(record_max, 0); A.rec_t = b->d_rec_t.p; A.rec_x = b->d_rec_x.p;
-  A.max_n = max_n; A.max_nlu = max_nlu;
+  A.max_n = max_n; A.max_nlu = max_nlu; A.max_blob = max_blob;
   A.dense = b->d_dense.p; A.ipiv_scratch = nullptr;
-  b->shmem = sizeof(double) * ((size_t)3 * (max_n + 1) + max_nlu + 8);
+  b->shmem = sizeof(double) * (((size_t)3 * (max_n + 1) + max_nlu + 1) & ~(size_t)1) +
+             sizeof(unsigned short) * (size_t)max_blob;
   if (b->shmem > 150 * 1024) return XG_E_ARG;
   b->set_order(1);
   *out = b.release();
