@@ -72,8 +72,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--cpu-tstop", type=float, default=2e-9)
+    ap.add_argument("--cpu-sample", type=int, default=512)
+    ap.add_argument("--cpu-tstop", type=float, default=3e-9)
+    ap.add_argument("--iters", type=int, default=1, help="Newton iterations per launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the batched engine shards "
@@ -95,6 +96,8 @@ def main():
     b = X.Batch(ckts, tstop, tstep, hmax)
     log(f"batch setup {time.time() - t0:.1f}s")
     tune = b.autotune()
+    b.config(iters_per_launch=args.iters)
+    tune["iters_per_launch"] = args.iters
     log(f"autotune: {tune}")
     b.advance(args.warmup, graph_chunk=50)
     c0 = b.counters()
@@ -116,6 +119,7 @@ def main():
     # single-instance FP64 GPU path: the same engine on the 101-stage ring
     r1 = C.ring(101, seed=1)
     b1 = X.Batch([r1], 50e-9, tstep, hmax)
+    b1.config(iters_per_launch=args.iters)
     b1.advance(args.warmup, graph_chunk=50)
     s0 = b1.counters()
     torch.cuda.synchronize()
@@ -127,7 +131,7 @@ def main():
     single = (s1["newton_iters"] - s0["newton_iters"]) / wall1
     log(f"single instance: {single:.1f} Newton steps/s ({wall1 / args.steps * 1e6:.1f} us/launch)")
 
-    byts = algorithmic_bytes(ckts)
+    byts = algorithmic_bytes(ckts) * args.iters
     us_launch = dev_s / args.steps * 1e6
     achieved = byts / (dev_s / args.steps) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -170,6 +170,18 @@ int xg_batch_counters(xg_batch* b, double* counters);
 int xg_batch_instance(xg_batch* b, int inst, double* tstate, double* x);
 /* Recorded accepted points of one instance: t[max], x[max*n]; returns count. */
 int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x);
+/* Workgroup size (64/128/256/512, 0 = keep) and Newton iterations per
+ * launch (1 = one launch per batched Newton iteration; K > 1 keeps each
+ * instance's state and program in LDS across K iterations, 0 = keep). */
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch);
+/* Per-phase device timers of the first-scheduled instance (the counterpart of
+ * the reference's Residual / Jacobian / Linear-solve StatTop timers,
+ * src/UtilityPKG/N_UTL_Stats.h:481-511): enable = 1 resets and starts,
+ * 0 stops, -1 only reads.  phase_us[0..14] accumulated microseconds of:
+ * x load, device eval, vector gather + residual, norms, Newton decision,
+ * Jacobian gather, LU refactor, pivot threshold test, triangular solves,
+ * update, step control; phase_us[15] = iterations timed. */
+int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
 /* Autotuner: times candidate block sizes / instance orders on the real batch
  * and keeps the fastest; result[0..2] = block size, order mode, us/launch. */
 int xg_batch_autotune(xg_batch* b, double* result);

@@ -202,3 +202,18 @@ def test_batch_uneven_instances_match_individual_runs(gpu):
         t_a, x_a = b.points(i, n)
         t_b, x_b = b1.points(0, n)
         assert np.array_equal(t_a, t_b) and np.array_equal(x_a, x_b)
+
+
+def test_iterations_per_launch_do_not_change_results(gpu):
+    """K Newton iterations per launch (state kept in LDS) give bit-identical
+    trajectories to one iteration per launch."""
+    cs = [C.ring(s, seed=7 + s, mc_seed=7 + s) for s in (9, 23)]
+    out = []
+    for k in (1, 4):
+        b = X.Batch(cs, 0.5e-9, 1e-12, 1e-11, record_max=2000)
+        b.config(iters_per_launch=k)
+        b.run_to_end(chunk=64)
+        n0 = X.host_pattern(cs[0])["n"]; n1 = X.host_pattern(cs[1])["n"]
+        out.append((b.points(0, n0), b.points(1, n1)))
+    for a, c in zip(out[0], out[1]):
+        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])

@@ -41,7 +41,7 @@ EXPORTS = [
     "xg_lu_symbolic", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
     "xg_tran_run", "xg_tran_points",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
-    "xg_batch_points", "xg_batch_autotune", "xg_host_pattern", "xg_host_lu_analyse",
+    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse",
 ]
 
 
@@ -326,6 +326,22 @@ class Batch:
             if c["done"] + c["failed"] >= self.n_inst:
                 break
         return self.counters()
+
+    def config(self, block=0, iters_per_launch=0):
+        _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch)), "xg_batch_config")
+
+    PHASES = ("x_load", "device_eval", "vector_gather_residual", "norms", "newton_decision", "jacobian_gather",
+              "lu_refactor", "pivot_check", "triangular_solves", "update", "step_control")
+
+    def profile(self, enable=None):
+        """enable=True/False starts/stops the per-phase timers; returns the
+        accumulated microseconds per phase of one instance."""
+        us = np.zeros(16)
+        flag = -1 if enable is None else (1 if enable else 0)
+        _check(lib().xg_batch_profile(self.h, C.c_int(flag), _p(us)), "xg_batch_profile")
+        out = {k: float(us[i]) for i, k in enumerate(self.PHASES)}
+        out["iterations"] = int(us[15])
+        return out
 
     def autotune(self):
         r = np.zeros(3)

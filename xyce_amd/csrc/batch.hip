@@ -48,6 +48,9 @@ struct ClassDesc {
   int nflev, nfs, nbs;
   // int32 arrays (global): device node indices, term positions, term ranges, sources
   int o_lid, o_tpos, o_vptr, o_mptr, o_bgp, o_bge;
+  // groups of >= kLongSeg terms, summed by a whole wavefront: vector groups
+  // (4*row + vector) then matrix groups (2*slot + F/Q), at o_long
+  int n_vlong, n_mlong, o_long;
   // uint16 program blob (staged into LDS each launch); offsets in uint16 units
   int blob_off, blob_len;
   int b_flp, b_ftg, b_fop, b_fdiv, b_fops, b_lurp, b_luci, b_diag, b_fsp, b_fsr, b_bsp, b_bsr, b_prow, b_pcol,
@@ -104,6 +107,9 @@ struct BatchArgs {
   // recording
   int rec_max; double* rec_t; double* rec_x;
   int max_n, max_nlu, max_blob;
+  int n_kcls;                  // entries of the MOSFET class table
+  int iters;                   // Newton iterations per launch
+  unsigned long long* prof;    // per-phase wall-clock ticks of one instance (null = off)
   double* dense;           // per-instance n*n scratch
   int* ipiv_scratch;       // unused (row swaps are applied in place)
 };
@@ -114,28 +120,27 @@ constexpr int kMaxNewton = 20;
 constexpr double kDcRel = 1e-3, kDcAbs = 1e-12, kDcDxTol = 1.0, kDcRhsTol = 1e-6;
 constexpr int kDcMaxNewton = 200;
 constexpr double kHmin = 1e-18;
+constexpr int kLongSeg = 32;     // term groups this long are reduced by a wavefront
+constexpr int kMaxLong = 16;     // per class
 
+// Block reductions: 64-lane wave shuffles, then one LDS exchange across waves.
 __device__ __forceinline__ double block_sum(double v, double* red) {
-  const int t = threadIdx.x;
-  red[t] = v;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) red[w] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] += red[t + s];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = 0.0;
+  for (int i = 0; i < nw; ++i) r += red[i];
   __syncthreads();
   return r;
 }
 __device__ __forceinline__ double block_max(double v, double* red) {
-  const int t = threadIdx.x;
-  red[t] = v;
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) red[w] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (t < s) red[t] = fmax(red[t], red[t + s]);
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = red[0];
+  for (int i = 1; i < nw; ++i) r = fmax(r, red[i]);
   __syncthreads();
   return r;
 }
@@ -291,43 +296,35 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
   return 0;
 }
 
-__global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
-  if ((int)blockIdx.x >= A.n_inst) return;
-  const int inst = A.order[blockIdx.x];
-  const InstDesc I = A.inst[inst];
-  const ClassDesc C = A.cls[I.cls];
-  __shared__ InstState S;
-  __shared__ double red[512];
-  __shared__ int red_i[512];
-  __shared__ int sh_int[8];
-  __shared__ double sh_d[8];
-  extern __shared__ double dyn[];
-  double* xs = dyn;                         // n + 1
-  double* rhs = xs + (A.max_n + 1);         // n
-  double* y = rhs + A.max_n;                // n
-  double* lu = y + A.max_n;                 // nlu
-  // class program after the doubles, on a 16-byte boundary (uint4 copies)
-  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.max_n + 1) + A.max_nlu + 1) & ~1));
+// One Newton iteration of one instance (the body of every launch; a launch
+// runs A.iters of them back to back with the instance state kept in LDS).
+__device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S,
+                                double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
+                                double* y, double* lu, const unsigned short* prog, const double* kcls) {
   const int tid = threadIdx.x, nth = blockDim.x;
-  if (tid == 0) S = A.st[inst];
-  if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
-  __syncthreads();
-  if (S.mode >= 2 || S.mode < 0) return;
   const int n = C.n, off = I.vec_off;
   const int* ci = A.ci;
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
   double* T = A.terms + I.term_off;
-  // ---- 1. stage the class program (uint16, 16-byte chunks) and x into LDS
-  {
-    const uint4* src = reinterpret_cast<const uint4*>(A.blob + C.blob_off);
-    uint4* dst = reinterpret_cast<uint4*>(prog);
-    const int chunks = C.blob_len / 8;
-    for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
-  }
+  // phase timers (one instance, thread 0, after each phase's barrier)
+  const bool prof_on = A.prof != nullptr && blockIdx.x == 0 && tid == 0;
+  unsigned long long prof_t = prof_on ? wall_clock64() : 0;
+#define XG_PHASE(k)                                              \
+  do {                                                           \
+    if (prof_on) {                                               \
+      const unsigned long long t_ = wall_clock64();              \
+      A.prof[k] += t_ - prof_t;                                  \
+      prof_t = t_;                                               \
+    }                                                            \
+  } while (0)
+  if (prof_on) A.prof[15] += 1;
+  if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
+  __syncthreads();
   for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
   if (tid == 0) xs[n] = 0.0;
   __syncthreads();
+  XG_PHASE(0);
   // ---- 2. device evaluation (one thread per MOSFET); each term goes to its
   //         slot-ordered position in the instance's term array
   {
@@ -338,7 +335,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     int lim_any = 0;
     for (int d = tid; d < C.n_mos; d += nth) {
       const int g = I.dev_off + d;
-      const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
+      const double* k = kcls + (size_t)A.mcls[g] * KSTRIDE;
       double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
       for (int j = 0; j < 6; ++j) volt[j] = xs[lid[6 * d + j]];
       for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
@@ -352,7 +349,28 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     if (lim_any) atomicOr(&sh_int[0], 1);
   }
   __syncthreads();
+  XG_PHASE(1);
   const int limited = sh_int[0];
+  // long term groups (e.g. a supply node shared by every PMOS): one wavefront
+  // each, lane-strided loads and a fixed shuffle tree (deterministic)
+  __shared__ double lsum[kMaxLong];
+  {
+    const int nl = C.n_vlong + C.n_mlong;
+    const int* lg = ci + C.o_long;
+    const int* vptr = ci + C.o_vptr;
+    const int* mptr = ci + C.o_mptr;
+    const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
+    for (int j = wave; j < nl; j += nwaves) {
+      const int g = lg[j];
+      const int beg = j < C.n_vlong ? vptr[g] : mptr[g];
+      const int end = j < C.n_vlong ? vptr[g + 1] : mptr[g + 1];
+      double sacc = 0.0;
+      for (int e = beg + lane; e < end; e += 64) sacc += T[e];
+      for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
+      if (lane == 0) lsum[j] = sacc;
+    }
+    if (nl) __syncthreads();
+  }
   const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
   // ---- 3. vectors per row (contiguous term ranges) and the residual
   const double h = S.h;
@@ -365,8 +383,14 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     for (int r = tid; r < n; r += nth) {
       double acc[4];
       for (int v = 0; v < 4; ++v) {
+        const int beg = vptr[4 * r + v], end = vptr[4 * r + v + 1];
         double s = 0.0;
-        for (int e = vptr[4 * r + v]; e < vptr[4 * r + v + 1]; ++e) s += T[e];
+        if (end - beg >= kLongSeg) {
+          const int* lg = ci + C.o_long;
+          for (int j = 0; j < C.n_vlong; ++j) if (lg[j] == 4 * r + v) s = lsum[j];
+        } else {
+          for (int e = beg; e < end; ++e) s += T[e];
+        }
         acc[v] = s;
       }
       double f = acc[0], q = acc[1];
@@ -393,8 +417,10 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       ninf = fmax(ninf, fabs(res));
     }
   }
+  XG_PHASE(2);
   n2 = block_sum(n2, red);
   ninf = block_max(ninf, red);
+  XG_PHASE(3);
   // ---- 4. Newton bookkeeping (DampedNewton::solve / converged_)
   if (tid == 0) {
     S.n_loads++;
@@ -434,6 +460,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   }
   __syncthreads();
   const int status = sh_int[1];
+  XG_PHASE(4);
 
   if (status == 0) {
     // ---- 5a. Jacobian, LU refactor, solve, update
@@ -453,16 +480,27 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
         const int* mptr = ci + C.o_mptr;
         const unsigned short* a2lu = prog + C.b_a2lu;
         const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+        const int* lg = ci + C.o_long;
         for (int s = tid; s < C.nnz; s += nth) {
-          double df = 0.0, dq = 0.0;
-          for (int e = mptr[2 * s]; e < mptr[2 * s + 1]; ++e) df += T[e];
-          for (int e = mptr[2 * s + 1]; e < mptr[2 * s + 2]; ++e) dq += T[e];
+          double part[2];
+          for (int fq = 0; fq < 2; ++fq) {
+            const int g = 2 * s + fq, beg = mptr[g], end = mptr[g + 1];
+            double acc = 0.0;
+            if (end - beg >= kLongSeg) {
+              for (int j = 0; j < C.n_mlong; ++j) if (lg[C.n_vlong + j] == g) acc = lsum[C.n_vlong + j];
+            } else {
+              for (int e = beg; e < end; ++e) acc += T[e];
+            }
+            part[fq] = acc;
+          }
+          double df = part[0], dq = part[1];
           if (linF[s] != 0.0) df += linF[s];
           if (linQ[s] != 0.0) dq += linQ[s];
           lu[a2lu[s]] = jac_combine(ja, dq, jb, df);
         }
       }
       __syncthreads();
+      XG_PHASE(5);
       {
         const unsigned short* flp = prog + C.b_flp; const unsigned short* ftg = prog + C.b_ftg;
         const unsigned short* fop = prog + C.b_fop; const unsigned short* fops = prog + C.b_fops;
@@ -479,6 +517,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
           __syncthreads();
         }
       }
+      XG_PHASE(6);
       // KLU's threshold test on the static pivots: |l_ij| <= 1/0.001, finite, nonzero pivot
       {
         const unsigned short* lurp = prog + C.b_lurp; const unsigned short* dg = prog + C.b_diag;
@@ -493,6 +532,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
         __syncthreads();
         use_dense = sh_int[6] != 0;
       }
+      XG_PHASE(7);
       if (!use_dense) {
         const unsigned short* prow = prog + C.b_prow; const unsigned short* pcol = prog + C.b_pcol;
         const unsigned short* lurp = prog + C.b_lurp; const unsigned short* luci = prog + C.b_luci;
@@ -522,6 +562,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
         for (int j = tid; j < n; j += nth) rhs[pcol[j]] = y[j];
         __syncthreads();
       }
+      XG_PHASE(8);
     }
     if (use_dense) {
       if (tid == 0) S.n_dense++;
@@ -544,12 +585,12 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     }
     if (nanf) atomicOr(&sh_int[2], 2);
     wdx = block_max(wdx, red);
+    XG_PHASE(9);
     if (tid == 0) {
       S.wt_dx = wdx;
       S.newton_iter++;
       S.n_iters++;
       S.last_status = sh_int[2] & 1 ? -9 : (sh_int[2] & 2 ? -6 : 0);
-      A.st[inst] = S;
     }
     return;
   }
@@ -557,7 +598,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   // ---- 5b. Newton finished for this step
   if (dc) {
     if (status < 0) {
-      if (tid == 0) { S.mode = -1; A.st[inst] = S; }
+      if (tid == 0) S.mode = -1;
       return;
     }
     // tranop -> transient: OneStep::initialize (N_TIA_OneStep.C:1684)
@@ -596,7 +637,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     if (tid == 0 && A.rec_max > 0) S.n_rec++;
     __syncthreads();
     begin_step(A, S, C, I, red);
-    if (tid == 0) A.st[inst] = S;
+    XG_PHASE(10);
     return;
   }
 
@@ -682,7 +723,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       }
     }
     __syncthreads();
-    if (S.mode == 2) { if (tid == 0) A.st[inst] = S; return; }
+    if (S.mode == 2) { return; }
     if (sh_int[5]) {
       for (int i = tid; i < n; i += nth) {
         A.xH1[off + i] = 0.0;
@@ -691,7 +732,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
       __syncthreads();
     }
     begin_step(A, S, C, I, red);
-    if (tid == 0) A.st[inst] = S;
+    XG_PHASE(10);
     return;
   }
   // rejectStep (N_TIA_OneStep.C:1946)
@@ -715,8 +756,52 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     S.h = newh;
   }
   __syncthreads();
-  if (S.mode < 0) { if (tid == 0) A.st[inst] = S; return; }
+  if (S.mode < 0) { return; }
   begin_step(A, S, C, I, red);
+  XG_PHASE(10);
+#undef XG_PHASE
+}
+
+__global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
+  if ((int)blockIdx.x >= A.n_inst) return;
+  const int inst = A.order[blockIdx.x];
+  const InstDesc I = A.inst[inst];
+  const ClassDesc C = A.cls[I.cls];
+  __shared__ InstState S;
+  __shared__ double red[512];
+  __shared__ int red_i[512];
+  __shared__ int sh_int[8];
+  __shared__ double sh_d[8];
+  __shared__ double kc_s[16 * KSTRIDE];
+  extern __shared__ double dyn[];
+  double* xs = dyn;                         // n + 1
+  double* rhs = xs + (A.max_n + 1);         // n
+  double* y = rhs + A.max_n;                // n
+  double* lu = y + A.max_n;                 // nlu
+  // class program after the doubles, on a 16-byte boundary (uint4 copies)
+  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.max_n + 1) + A.max_nlu + 1) & ~1));
+  const int tid = threadIdx.x, nth = blockDim.x;
+  if (tid == 0) S = A.st[inst];
+  __syncthreads();
+  if (S.mode >= 2 || S.mode < 0) return;
+  // stage the class program (uint16, 16-byte chunks) and the MOSFET class table into LDS
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(A.blob + C.blob_off);
+    uint4* dst = reinterpret_cast<uint4*>(prog);
+    const int chunks = C.blob_len / 8;
+    for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
+  }
+  const double* kcls = A.kcls;
+  if (A.n_kcls <= 16) {
+    for (int i = tid; i < A.n_kcls * KSTRIDE; i += nth) kc_s[i] = A.kcls[i];
+    kcls = kc_s;
+  }
+  __syncthreads();
+  for (int it = 0; it < A.iters; ++it) {
+    batch_iteration(A, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls);
+    __syncthreads();
+    if (S.mode >= 2 || S.mode < 0) break;
+  }
   if (tid == 0) A.st[inst] = S;
 }
 
@@ -768,6 +853,7 @@ struct xg_batch {
   DArr<unsigned long long> d_mask;
   DArr<double> d_x, d_xn0, d_xH0, d_xH1, d_xH2, d_qH0, d_qH1, d_qH2, d_F, d_Q, d_B, d_F0, d_B0, d_wt;
   DArr<double> d_linF, d_linQ, d_wave_data, d_bp, d_rec_t, d_rec_x, d_dense;
+  DArr<unsigned long long> d_prof;
   BatchArgs args{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -912,6 +998,20 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           }
       }
       C.o_tpos = push(tpos); C.o_vptr = push(vptr); C.o_mptr = push(mptr);
+      {
+        std::vector<int> lg;
+        for (int g = 0; g < 4 * n && (int)lg.size() < kMaxLong; ++g) if (vptr[g + 1] - vptr[g] >= kLongSeg) lg.push_back(g);
+        C.n_vlong = (int)lg.size();
+        for (int g = 0; g < 2 * hc.nnz && (int)lg.size() < kMaxLong; ++g) if (mptr[g + 1] - mptr[g] >= kLongSeg) lg.push_back(g);
+        C.n_mlong = (int)lg.size() - C.n_vlong;
+        // a class with more long groups than kMaxLong cannot use this path
+        int total_long = 0;
+        for (int g = 0; g < 4 * n; ++g) total_long += vptr[g + 1] - vptr[g] >= kLongSeg;
+        for (int g = 0; g < 2 * hc.nnz; ++g) total_long += mptr[g + 1] - mptr[g] >= kLongSeg;
+        if (total_long > kMaxLong) return XG_E_ARG;
+        lg.push_back(0);
+        C.o_long = push(lg);
+      }
       // per-row source lists (local source index)
       {
         std::vector<int> cnt(n + 1, 0);
@@ -1058,6 +1158,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.n_bp = (int)all_bp.size(); A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
   A.rec_max = std::max(record_max, 0); A.rec_t = b->d_rec_t.p; A.rec_x = b->d_rec_x.p;
   A.max_n = max_n; A.max_nlu = max_nlu; A.max_blob = max_blob;
+  A.n_kcls = (int)(kcls.size() / KSTRIDE); A.iters = 1; A.prof = nullptr;
   A.dense = b->d_dense.p; A.ipiv_scratch = nullptr;
   b->shmem = sizeof(double) * (((size_t)3 * (max_n + 1) + max_nlu + 1) & ~(size_t)1) +
              sizeof(unsigned short) * (size_t)max_blob;
@@ -1172,6 +1273,37 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   b->drop_graph();
   b->tune_us = best;
   if (result) { result[0] = best_b; result[1] = best_o; result[2] = best; }
+  return XG_OK;
+}
+
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch) {
+  if (!b || iters_per_launch < 0) return XG_E_ARG;
+  if (block) {
+    if (block != 64 && block != 128 && block != 256 && block != 512) return XG_E_ARG;
+    b->block = block;
+  }
+  if (iters_per_launch) b->args.iters = iters_per_launch;
+  b->drop_graph();
+  return XG_OK;
+}
+
+int xg_batch_profile(xg_batch* b, int enable, double* phase_us) {
+  if (!b) return XG_E_ARG;
+  if (enable && !b->d_prof.p) {
+    HIPCHK(b->d_prof.zeros(16));
+  }
+  if (phase_us && b->d_prof.p) {
+    unsigned long long t[16];
+    HIPCHK(hipMemcpy(t, b->d_prof.p, sizeof(t), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    for (int k = 0; k < 15; ++k) phase_us[k] = khz > 0 ? (double)t[k] / (double)khz * 1e3 : 0.0;
+    phase_us[15] = (double)t[15];
+  }
+  if (enable >= 0) b->args.prof = enable ? b->d_prof.p : nullptr;
+  if (enable > 0) HIPCHK(hipMemset(b->d_prof.p, 0, 16 * sizeof(unsigned long long)));
+  b->drop_graph();
   return XG_OK;
 }
 
