@@ -138,17 +138,19 @@ int xg_lu_numeric(xg_lu* lu, const double* vals);
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
 /* info[0..3] = nnz(L+U), number of factor levels, number of solve levels, flops per factorization */
 int xg_lu_info(const xg_lu* lu, double* info);
-
-/* ---------------- device-resident Newton / transient ---------------- */
-
-/* Runs a full transient (DC operating point, then OneStep trapezoid with
- * DampedNewton) for one circuit with every Newton step on the GPU
- * (eval, stamp, combine, LU refactor, solve, update, norms), driven from the
- * host one Newton iteration at a time.  stats[0..7] = accepted steps,
- * rejected steps, Newton iterations, residual loads, seconds, status,
- * recorded points, mean Newton-iteration time (s). */
-int xg_tran_run(xg_circuit* c, double tstop, double tstep, double hmax, int record, double* stats);
-int xg_tran_points(xg_circuit* c, double* t, double* x);
+/* Factor precision (the mixed-precision form of AmesosSolver::doSolve):
+ * 0 = FP64 factors (default); 1 = FP32 factors, FP32 triangular solves and
+ * FP64 iterative refinement x += A^-1_32 (b - A x) until
+ * max|b - A x| <= ir_tol * max|b|, at most ir_max corrections.  If the
+ * residual stops decreasing, ir_max corrections do not reach ir_tol, or the
+ * FP32 pivots fail the threshold test, the same call finishes with FP64
+ * factors, so a solve is never less accurate than the FP64 path. */
+int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max);
+/* stats[0..6] = corrections of the last solve, its relative residual,
+ * solves, corrections in total, solves finished by FP64 factors, refactors
+ * whose FP32 pivots failed the threshold test, current precision
+ * (2 = FP32 switched off for this pattern after a failure). */
+int xg_lu_precision_stats(const xg_lu* lu, double* stats);
 
 /* ---------------- batched Monte Carlo engine ---------------- */
 

@@ -1,0 +1,100 @@
+"""Mixed-precision LU on the GPU: FP32 factors and triangular solves with FP64
+residuals and iterative refinement, against the all-FP64 answer.
+
+Systems: transient Jacobians (trapezoid, h = 2 ps) of level-1 INV/NAND2 gate
+arrays wired as a random DAG, at a perturbed DC operating point of the
+oracle.  The FP64 answer is SuperLU (scipy) on the oracle's CSR values.
+
+Bars: relative residual max|b - A x| / max|b| <= 1e-12 (the refinement
+target), and per-node error |x - x64| <= 1e-9 * (RELTOL |x64| + ABSTOL) with
+RELTOL 1e-3, ABSTOL 1e-6, i.e. far inside the transient tolerances.  The
+per-node error is printed.  Hand-made ill-conditioned systems check the FP64
+fallbacks (FP32 pivot failure, refinement that stops contracting).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import xyce_amd as X
+from oracle.pyoracle import Oracle
+from xyce_amd import circuits as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _tran_system(c, seed):
+    o = Oracle(c)
+    st, x0, sta0, sto0, Q, F, B = o.dcop()
+    assert st > 0
+    rng = np.random.default_rng(seed)
+    x = x0 + rng.normal(0.0, 0.05, size=o.n)
+    ref = o.load(x, sta0, sto0, h=2e-12, order=2)
+    rhs, J = o.combine(ref, ref["Q"], ref["F"] - ref["B"])
+    return o, J, rhs
+
+
+@pytest.mark.parametrize("gates", [300, 3000])
+def test_fp32_refined_solve_matches_fp64(gpu, gates):
+    c = C.gate_dag(gates, seed=4)
+    o, J, rhs = _tran_system(c, gates)
+    p = o.pattern()
+    A = sp.csr_matrix((J, p["colind"], p["rowptr"]), shape=(o.n, o.n))
+    x64 = spla.spsolve(A.tocsc(), rhs)
+    s = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    s.set_precision(1, ir_tol=1e-12, ir_max=10)
+    for rep in range(2):       # refactor + solve twice (the Newton loop's pattern)
+        st, x = s.doSolve(J * (1.0 + 0.01 * rep), rhs)
+        assert st == 0
+    ps = s.precision_stats()
+    assert ps["precision"] == 1 and ps["fp64_fallbacks"] == 0 and ps["fp32_pivot_failures"] == 0, ps
+    x64 = spla.spsolve((A * 1.01).tocsc(), rhs)
+    res = np.max(np.abs(A * 1.01 @ x - rhs)) / np.max(np.abs(rhs))
+    err = np.abs(x - x64) / (1e-3 * np.abs(x64) + 1e-6)
+    print(f"n={o.n}: corrections {ps['last_corrections']}, rel residual {res:.2e}, "
+          f"max per-node error {err.max():.2e} of RELTOL|x|+ABSTOL")
+    assert res <= 1e-12
+    assert err.max() <= 1e-9
+    # the FP64 path on the same system agrees too
+    s64 = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    st, xd = s64.doSolve(J * 1.01, rhs)
+    assert st == 0
+    assert np.max(np.abs(xd - x) / (1e-3 * np.abs(x64) + 1e-6)) <= 1e-9
+
+
+def _two_by_two(delta):
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    v = np.array([1.0, 1.0, 1.0, 1.0 + delta])
+    return rp, ci, v
+
+
+def test_fp32_singular_pivot_falls_back(gpu):
+    # 1 + 1e-9 rounds to 1 in FP32: the FP32 pivot is exactly zero
+    rp, ci, v = _two_by_two(1e-9)
+    b = np.array([1.0, 2.0])
+    s = X.DirectSolver(2, rp, ci)
+    s.set_precision(1)
+    st, x = s.doSolve(v, b)
+    assert st == 0
+    ps = s.precision_stats()
+    assert ps["fp32_pivot_failures"] >= 1
+    xn = np.linalg.solve(np.array([[1.0, 1.0], [1.0, 1.0 + 1e-9]]), b)
+    np.testing.assert_allclose(x, xn, rtol=1e-6)
+
+
+def test_refinement_that_stalls_falls_back(gpu):
+    """cond ~ 1e8: the FP32 factors' error times the condition number is
+    near 1 and FP64 itself cannot reach 1e-12, so the residual stops
+    decreasing and the solve finishes with the FP64 factors."""
+    A = np.array([[1.0, 1.0 / 3.0], [1.0, 1.0 / 3.0 + 5e-8]])
+    rp = np.array([0, 2, 4], np.int32)
+    ci = np.array([0, 1, 0, 1], np.int32)
+    b = np.array([1.0, 2.0])
+    s = X.DirectSolver(2, rp, ci)
+    s.set_precision(1, ir_tol=1e-12, ir_max=10)
+    st, x = s.doSolve(A.ravel(), b)
+    assert st == 0
+    ps = s.precision_stats()
+    assert ps["fp32_pivot_failures"] == 0 and ps["fp64_fallbacks"] == 1, ps
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-6)

@@ -39,7 +39,7 @@ EXPORTS = [
     "xg_circuit_create", "xg_circuit_destroy", "xg_circuit_sizes", "xg_circuit_pattern", "xg_set_solver_state",
     "xg_update_state", "xg_load_vectors", "xg_load_matrices", "xg_combine",
     "xg_lu_symbolic", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
-    "xg_tran_run", "xg_tran_points",
+    "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
     "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse",
 ]
@@ -242,6 +242,18 @@ class DirectSolver:
         i = np.zeros(4)
         lib().xg_lu_info(self.h, _p(i))
         return dict(nnz_lu=int(i[0]), factor_levels=int(i[1]), solve_levels=int(i[2]), flops=float(i[3]))
+
+    def set_precision(self, precision, ir_tol=1e-12, ir_max=10):
+        """0 = FP64 factors; 1 = FP32 factors with FP64 iterative refinement."""
+        _check(lib().xg_lu_set_precision(self.h, int(precision), C.c_double(ir_tol), int(ir_max)),
+               "xg_lu_set_precision")
+
+    def precision_stats(self):
+        s = np.zeros(7)
+        _check(lib().xg_lu_precision_stats(self.h, _p(s)), "xg_lu_precision_stats")
+        return dict(last_corrections=int(s[0]), last_rel_residual=float(s[1]), solves=int(s[2]),
+                    corrections=int(s[3]), fp64_fallbacks=int(s[4]), fp32_pivot_failures=int(s[5]),
+                    precision=int(s[6]))
 
     def numeric(self, values):
         v = np.ascontiguousarray(values, np.float64)

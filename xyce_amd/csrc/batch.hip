@@ -1307,7 +1307,5 @@ int xg_batch_profile(xg_batch* b, int enable, double* phase_us) {
   return XG_OK;
 }
 
-int xg_tran_run(xg_circuit*, double, double, double, int, double*) { return XG_E_ARG; }
-int xg_tran_points(xg_circuit*, double*, double*) { return XG_E_ARG; }
 
 }  // extern "C"

@@ -189,27 +189,32 @@ __global__ void k_combine_jac(int nnz, double a, double b, const double* dFdx, c
 }
 #pragma clang fp contract(fast)
 
-__global__ void k_fill(double* p, size_t n, double v) {
+template <class T>
+__global__ void k_fill(T* p, size_t n, T v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void k_scatter_a(int nnz, const double* vals, const int* a2lu, double* lu) {
+// LU kernels are templated on the factor type T: double for the plain path,
+// float for the FP32 factorisation that FP64 iterative refinement corrects.
+template <class T>
+__global__ void k_scatter_a(int nnz, const double* vals, const int* a2lu, T* lu) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nnz) lu[a2lu[s]] = vals[s];
+  if (s < nnz) lu[a2lu[s]] = (T)vals[s];
 }
 
 // Entry-level scheduled LU (Doolittle order per entry: a_ij - sum_k l_ik u_kj,
 // k ascending, then / u_jj below the diagonal), one level per barrier.
 // One workgroup; LU values in global memory (L2 resident for these sizes).
+template <class T>
 __global__ void __launch_bounds__(1024) k_lu_factor_1wg(int nlev, const int* __restrict__ lvl_ptr,
                                                         const int* __restrict__ tgt, const int* __restrict__ op_ptr,
                                                         const int* __restrict__ ops, const int* __restrict__ div,
-                                                        double* lu, int* bad) {
+                                                        T* lu) {
   for (int l = 0; l < nlev; ++l) {
     for (int t = lvl_ptr[l] + threadIdx.x; t < lvl_ptr[l + 1]; t += blockDim.x) {
       const int p = tgt[t];
-      double v = lu[p];
+      T v = lu[p];
       for (int o = op_ptr[t]; o < op_ptr[t + 1]; ++o) v -= lu[ops[2 * o]] * lu[ops[2 * o + 1]];
       const int d = div[t];
       if (d >= 0) v /= lu[d];
@@ -217,15 +222,15 @@ __global__ void __launch_bounds__(1024) k_lu_factor_1wg(int nlev, const int* __r
     }
     __syncthreads();
   }
-  (void)bad;
 }
 
+template <class T>
 __global__ void k_lu_factor_level(int t0, int t1, const int* __restrict__ tgt, const int* __restrict__ op_ptr,
-                                  const int* __restrict__ ops, const int* __restrict__ div, double* lu) {
+                                  const int* __restrict__ ops, const int* __restrict__ div, T* lu) {
   const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= t1) return;
   const int p = tgt[t];
-  double v = lu[p];
+  T v = lu[p];
   for (int o = op_ptr[t]; o < op_ptr[t + 1]; ++o) v -= lu[ops[2 * o]] * lu[ops[2 * o + 1]];
   const int d = div[t];
   if (d >= 0) v /= lu[d];
@@ -234,31 +239,33 @@ __global__ void k_lu_factor_level(int t0, int t1, const int* __restrict__ tgt, c
 
 // pivot check: non-finite or zero pivot, or a multiplier beyond KLU's
 // threshold (|l_ij| > 1/0.001: KLU would have re-pivoted) -> bad = row + 1
-__global__ void k_lu_pivots(int n, const int* lu_rowptr, const int* diag, const double* lu, int* bad) {
+template <class T>
+__global__ void k_lu_pivots(int n, const int* lu_rowptr, const int* diag, const T* lu, int* bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double d = lu[diag[i]];
+  const double d = (double)lu[diag[i]];
   bool b = !(fabs(d) > 0.0) || !isfinite(d);
-  for (int p = lu_rowptr[i]; p < diag[i] && !b; ++p) b = !(fabs(lu[p]) <= 1.0e3);
+  for (int p = lu_rowptr[i]; p < diag[i] && !b; ++p) b = !(fabs((double)lu[p]) <= 1.0e3);
   if (b) atomicMin(bad, i + 1);
 }
 
 // Level-scheduled triangular solves; b is permuted on the way in and x on the
-// way out (B = P_r A P_c).  One workgroup.
+// way out (B = P_r A P_c).  One workgroup; arithmetic in T.
+template <class T>
 __global__ void __launch_bounds__(1024) k_lu_solve_1wg(int n, const int* __restrict__ lu_rowptr,
                                                        const int* __restrict__ lu_colind, const int* __restrict__ diag,
-                                                       const double* __restrict__ lu, int nfl,
+                                                       const T* __restrict__ lu, int nfl,
                                                        const int* __restrict__ fl_ptr, const int* __restrict__ fl_rows,
                                                        int nbl, const int* __restrict__ bl_ptr,
                                                        const int* __restrict__ bl_rows, const int* __restrict__ prow,
                                                        const int* __restrict__ pcol, const double* __restrict__ b,
-                                                       double* y, double* x) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[prow[i]];
+                                                       T* y, double* x, int* nan_flag) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = (T)b[prow[i]];
   __syncthreads();
   for (int l = 0; l < nfl; ++l) {
     for (int t = fl_ptr[l] + threadIdx.x; t < fl_ptr[l + 1]; t += blockDim.x) {
       const int i = fl_rows[t];
-      double v = y[i];
+      T v = y[i];
       for (int p = lu_rowptr[i]; p < diag[i]; ++p) v -= lu[p] * y[lu_colind[p]];
       y[i] = v;
     }
@@ -267,43 +274,86 @@ __global__ void __launch_bounds__(1024) k_lu_solve_1wg(int n, const int* __restr
   for (int l = 0; l < nbl; ++l) {
     for (int t = bl_ptr[l] + threadIdx.x; t < bl_ptr[l + 1]; t += blockDim.x) {
       const int i = bl_rows[t];
-      double v = y[i];
+      T v = y[i];
       for (int p = diag[i] + 1; p < lu_rowptr[i + 1]; ++p) v -= lu[p] * y[lu_colind[p]];
       y[i] = v / lu[diag[i]];
     }
     __syncthreads();
   }
-  for (int j = threadIdx.x; j < n; j += blockDim.x) x[pcol[j]] = y[j];
+  int bad = 0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double v = (double)y[j];
+    if (!isfinite(v)) bad = 1;
+    x[pcol[j]] = v;
+  }
+  if (bad) atomicOr(nan_flag, 1);
 }
 
+template <class T>
 __global__ void k_solve_fwd_level(int t0, int t1, const int* fl_rows, const int* lu_rowptr, const int* lu_colind,
-                                  const int* diag, const double* lu, double* y) {
+                                  const int* diag, const T* lu, T* y) {
   const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= t1) return;
   const int i = fl_rows[t];
-  double v = y[i];
+  T v = y[i];
   for (int p = lu_rowptr[i]; p < diag[i]; ++p) v -= lu[p] * y[lu_colind[p]];
   y[i] = v;
 }
+template <class T>
 __global__ void k_solve_bwd_level(int t0, int t1, const int* bl_rows, const int* lu_rowptr, const int* lu_colind,
-                                  const int* diag, const double* lu, double* y) {
+                                  const int* diag, const T* lu, T* y) {
   const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= t1) return;
   const int i = bl_rows[t];
-  double v = y[i];
+  T v = y[i];
   for (int p = diag[i] + 1; p < lu_rowptr[i + 1]; ++p) v -= lu[p] * y[lu_colind[p]];
   y[i] = v / lu[diag[i]];
 }
-__global__ void k_permute_in(int n, const int* prow, const double* b, double* y) {
+template <class T>
+__global__ void k_permute_in(int n, const int* prow, const double* b, T* y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = b[prow[i]];
+  if (i < n) y[i] = (T)b[prow[i]];
 }
-__global__ void k_permute_out(int n, const int* pcol, const double* y, double* x, int* nan_flag) {
+template <class T>
+__global__ void k_permute_out(int n, const int* pcol, const T* y, double* x, int* nan_flag) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n) {
-    x[pcol[j]] = y[j];
-    if (!isfinite(y[j])) atomicOr(nan_flag, 1);
+    const double v = (double)y[j];
+    x[pcol[j]] = v;
+    if (!isfinite(v)) atomicOr(nan_flag, 1);
   }
+}
+
+// Iterative refinement (FP64): r = b - A x over the caller's CSR (one row per
+// thread), with per-block max|r| and max|b| folded into red[0..1] as ordered
+// non-negative doubles (their bit patterns compare like the values).
+__global__ void k_residual_f64(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                               const double* __restrict__ a, const double* __restrict__ x,
+                               const double* __restrict__ b, double* __restrict__ r,
+                               unsigned long long* red) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ar = 0.0, ab = 0.0;
+  if (i < n) {
+    double s = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s += a[k] * x[ci[k]];
+    const double v = b[i] - s;
+    r[i] = v;
+    ar = fabs(v);
+    ab = fabs(b[i]);
+    if (!isfinite(ar)) ar = INFINITY;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ar = fmax(ar, __shfl_xor(ar, o, 64));
+    ab = fmax(ab, __shfl_xor(ab, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&red[0], (unsigned long long)__double_as_longlong(ar));
+    atomicMax(&red[1], (unsigned long long)__double_as_longlong(ab));
+  }
+}
+__global__ void k_axpy_f64(int n, const double* d, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
 }
 
 // state permutation between the Xyce layout and the SoA planes
@@ -332,12 +382,19 @@ struct xg_lu {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   DBuf<int> a2lu, lu_rowptr, lu_colind, lu_diag, f_lvl_ptr, f_tgt, f_op_ptr, f_ops, f_div;
-  DBuf<int> fs_ptr, fs_rows, bs_ptr, bs_rows, prow, pcol, flag;
-  DBuf<double> lu, avals, y, b, x;
+  DBuf<int> fs_ptr, fs_rows, bs_ptr, bs_rows, prow, pcol, flag, arp, aci;
+  DBuf<double> lu, avals, y, b, x, r, d;
+  DBuf<float> lu32, y32;
+  DBuf<unsigned long long> red;
   std::vector<int> h_flvl, h_fs, h_bs;
   bool one_wg = true;
-  int rowptr_n = 0;
   std::vector<int> a_rowptr, a_colind;
+  // precision: 0 = FP64 factors; 1 = FP32 factors + FP64 iterative refinement
+  int mode = 0;
+  double ir_tol = 1e-12;
+  int ir_max = 10;
+  bool have64 = false;    // FP64 factors of the current values exist (refinement fallback)
+  double stats[6] = {0, 0, 0, 0, 0, 0};  // last solve: refinements, rel. residual; totals: solves, refinements, fallbacks, FP32 pivot failures
 
   int upload() {
     hipStream_t st = stream;
@@ -348,56 +405,117 @@ struct xg_lu {
     HIPCHK(fs_ptr.upload(s.fs_lvl_ptr, st)); HIPCHK(fs_rows.upload(s.fs_rows, st));
     HIPCHK(bs_ptr.upload(s.bs_lvl_ptr, st)); HIPCHK(bs_rows.upload(s.bs_rows, st));
     HIPCHK(prow.upload(s.perm_row, st)); HIPCHK(pcol.upload(s.perm_col, st));
-    HIPCHK(flag.alloc(2));
-    HIPCHK(lu.alloc(s.nlu)); HIPCHK(avals.alloc(s.nnz_a)); HIPCHK(y.alloc(s.n)); HIPCHK(b.alloc(s.n)); HIPCHK(x.alloc(s.n));
+    HIPCHK(arp.upload(a_rowptr, st)); HIPCHK(aci.upload(a_colind, st));
+    HIPCHK(flag.alloc(2)); HIPCHK(red.alloc(2));
+    HIPCHK(lu.alloc(s.nlu)); HIPCHK(avals.alloc(s.nnz_a)); HIPCHK(y.alloc(s.n)); HIPCHK(b.alloc(s.n));
+    HIPCHK(x.alloc(s.n)); HIPCHK(r.alloc(s.n)); HIPCHK(d.alloc(s.n));
+    if (mode == 1) { HIPCHK(lu32.alloc(s.nlu)); HIPCHK(y32.alloc(s.n)); }
     h_flvl = s.f_lvl_ptr; h_fs = s.fs_lvl_ptr; h_bs = s.bs_lvl_ptr;
     const int nlev = (int)s.f_lvl_ptr.size() - 1;
     one_wg = s.nlu <= (1 << 20) && nlev <= 4096 && s.n <= (1 << 16);
+    have64 = false;
     return XG_OK;
   }
 
-  // numeric factorisation of the values already in `lu` storage
-  int factor_device() {
+  // numeric factorisation of the values in `avals`, into FP64 or FP32 storage
+  template <class T>
+  int factor_device(T* f) {
     const int nlev = (int)h_flvl.size() - 1;
+    k_fill<T><<<nblk(s.nlu), 256, 0, stream>>>(f, s.nlu, (T)0);
+    k_scatter_a<T><<<nblk(s.nnz_a), 256, 0, stream>>>(s.nnz_a, avals.p, a2lu.p, f);
     if (one_wg) {
-      k_lu_factor_1wg<<<1, 1024, 0, stream>>>(nlev, f_lvl_ptr.p, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, lu.p, flag.p);
+      k_lu_factor_1wg<T><<<1, 1024, 0, stream>>>(nlev, f_lvl_ptr.p, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, f);
     } else {
       for (int l = 0; l < nlev; ++l) {
         const int t0 = h_flvl[l], t1 = h_flvl[l + 1];
-        if (t1 > t0) k_lu_factor_level<<<nblk(t1 - t0), 256, 0, stream>>>(t0, t1, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, lu.p);
+        if (t1 > t0) k_lu_factor_level<T><<<nblk(t1 - t0), 256, 0, stream>>>(t0, t1, f_tgt.p, f_op_ptr.p, f_ops.p, f_div.p, f);
       }
     }
     HIPCHK(hipGetLastError());
     return XG_OK;
   }
-  int check_pivots(int* bad_out) {
+  template <class T>
+  int check_pivots(const T* f, int* bad_out) {
     int init[2] = {0x7fffffff, 0};
     HIPCHK(hipMemcpyAsync(flag.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
-    k_lu_pivots<<<nblk(s.n), 256, 0, stream>>>(s.n, lu_rowptr.p, lu_diag.p, lu.p, flag.p);
+    k_lu_pivots<T><<<nblk(s.n), 256, 0, stream>>>(s.n, lu_rowptr.p, lu_diag.p, f, flag.p);
     int res[2];
     HIPCHK(hipMemcpyAsync(res, flag.p, sizeof(res), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
     return XG_OK;
   }
-  // solve with device vectors b -> x
-  int solve_device(const double* db, double* dx) {
+  // solve with the factors f and device vectors db -> dx (arithmetic in T);
+  // flag[1] collects non-finite results
+  template <class T>
+  int solve_device(const T* f, T* yy, const double* db, double* dx) {
     const int n = s.n;
     if (one_wg) {
-      k_lu_solve_1wg<<<1, 1024, 0, stream>>>(n, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, (int)h_fs.size() - 1,
-                                             fs_ptr.p, fs_rows.p, (int)h_bs.size() - 1, bs_ptr.p, bs_rows.p, prow.p,
-                                             pcol.p, db, y.p, dx);
+      k_lu_solve_1wg<T><<<1, 1024, 0, stream>>>(n, lu_rowptr.p, lu_colind.p, lu_diag.p, f, (int)h_fs.size() - 1,
+                                                fs_ptr.p, fs_rows.p, (int)h_bs.size() - 1, bs_ptr.p, bs_rows.p,
+                                                prow.p, pcol.p, db, yy, dx, flag.p + 1);
     } else {
-      k_permute_in<<<nblk(n), 256, 0, stream>>>(n, prow.p, db, y.p);
+      k_permute_in<T><<<nblk(n), 256, 0, stream>>>(n, prow.p, db, yy);
       for (size_t l = 0; l + 1 < h_fs.size(); ++l)
         if (h_fs[l + 1] > h_fs[l])
-          k_solve_fwd_level<<<nblk(h_fs[l + 1] - h_fs[l]), 256, 0, stream>>>(h_fs[l], h_fs[l + 1], fs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, y.p);
+          k_solve_fwd_level<T><<<nblk(h_fs[l + 1] - h_fs[l]), 256, 0, stream>>>(h_fs[l], h_fs[l + 1], fs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, f, yy);
       for (size_t l = 0; l + 1 < h_bs.size(); ++l)
         if (h_bs[l + 1] > h_bs[l])
-          k_solve_bwd_level<<<nblk(h_bs[l + 1] - h_bs[l]), 256, 0, stream>>>(h_bs[l], h_bs[l + 1], bs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, lu.p, y.p);
-      k_permute_out<<<nblk(n), 256, 0, stream>>>(n, pcol.p, y.p, dx, flag.p + 1);
+          k_solve_bwd_level<T><<<nblk(h_bs[l + 1] - h_bs[l]), 256, 0, stream>>>(h_bs[l], h_bs[l + 1], bs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, f, yy);
+      k_permute_out<T><<<nblk(n), 256, 0, stream>>>(n, pcol.p, yy, dx, flag.p + 1);
     }
     HIPCHK(hipGetLastError());
+    return XG_OK;
+  }
+  // r = b - A x; returns max|r| / max|b| (0/0 -> 0)
+  int residual(const double* db, const double* dx, double* rel) {
+    HIPCHK(hipMemsetAsync(red.p, 0, 2 * sizeof(unsigned long long), stream));
+    k_residual_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, arp.p, aci.p, avals.p, dx, db, r.p, red.p);
+    unsigned long long h[2];
+    HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    double rn, bn;
+    std::memcpy(&rn, &h[0], 8);
+    std::memcpy(&bn, &h[1], 8);
+    *rel = bn > 0.0 ? rn / bn : (rn > 0.0 ? INFINITY : 0.0);
+    return XG_OK;
+  }
+  // FP32 solve + FP64 refinement of device b -> x; falls back to FP64
+  // factors when the residual stops decreasing or ir_max corrections did
+  // not reach ir_tol (an ill-conditioned system: the FP32 factors' error
+  // times the condition number is near 1, or FP64 itself cannot reach ir_tol)
+  int solve_refined(const double* db, double* dx) {
+    int st = solve_device<float>(lu32.p, y32.p, db, dx);
+    if (st != XG_OK) return st;
+    double rel = 0.0, prev = INFINITY;
+    int it = 0;
+    bool fell_back = false;
+    for (;;) {
+      st = residual(db, dx, &rel);
+      if (st != XG_OK) return st;
+      if (rel <= ir_tol) break;
+      if (it >= ir_max || !(rel < prev)) {
+        // refinement does not converge with the FP32 factors: FP64 factors
+        HIPCHK(hipMemsetAsync(flag.p + 1, 0, sizeof(int), stream));
+        if (!have64) {
+          st = factor_device<double>(lu.p);
+          if (st != XG_OK) return st;
+          have64 = true;
+        }
+        st = solve_device<double>(lu.p, y.p, db, dx);
+        if (st != XG_OK) return st;
+        st = residual(db, dx, &rel);
+        if (st != XG_OK) return st;
+        fell_back = true;
+        break;
+      }
+      prev = rel;
+      st = solve_device<float>(lu32.p, y32.p, r.p, d.p);
+      if (st != XG_OK) return st;
+      k_axpy_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, d.p, dx);
+      ++it;
+    }
+    stats[0] = it; stats[1] = rel; stats[2] += 1; stats[3] += it; stats[4] += fell_back ? 1 : 0;
     return XG_OK;
   }
 };
@@ -414,8 +532,6 @@ struct xg_circuit {
   DBuf<int> rowptr, colind, vg_ptr, vg_ent, mg_ptr, mg_ent, bg_ptr, bg_ent, src_wave, wave_type, wave_off, limited;
   DBuf<double> wave_data, xy_sta, xy_sto;
   xg_solver_state ss{};
-  // transient run state (xg_tran_run)
-  std::vector<double> rec_t, rec_x;
 };
 
 static int circuit_upload(xg_circuit* c) {
@@ -654,12 +770,19 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   if (!lu || !vals) return XG_E_ARG;
   hipStream_t s = lu->stream;
   HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
-  k_fill<<<nblk(lu->s.nlu), 256, 0, s>>>(lu->lu.p, lu->s.nlu, 0.0);
-  k_scatter_a<<<nblk(lu->s.nnz_a), 256, 0, s>>>(lu->s.nnz_a, lu->avals.p, lu->a2lu.p, lu->lu.p);
-  int st = lu->factor_device();
+  lu->have64 = false;
+  int bad = 0, st;
+  if (lu->mode == 1) {
+    st = lu->factor_device<float>(lu->lu32.p);
+    if (st != XG_OK) return st;
+    st = lu->check_pivots<float>(lu->lu32.p, &bad);
+    if (st != XG_OK) return st;
+    if (!bad) return XG_OK;
+    lu->stats[5] += 1;    // FP32 pivots fail the test: use the FP64 path for these values
+  }
+  st = lu->factor_device<double>(lu->lu.p);
   if (st != XG_OK) return st;
-  int bad = 0;
-  st = lu->check_pivots(&bad);
+  st = lu->check_pivots<double>(lu->lu.p, &bad);
   if (st != XG_OK) return st;
   if (bad) {
     // re-pivot: analyse again steering the transversal by the values, then refactor
@@ -672,13 +795,23 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
     lu->s = std::move(s2);
     st = lu->upload();
     if (st != XG_OK) return st;
-    k_fill<<<nblk(lu->s.nlu), 256, 0, s>>>(lu->lu.p, lu->s.nlu, 0.0);
-    k_scatter_a<<<nblk(lu->s.nnz_a), 256, 0, s>>>(lu->s.nnz_a, lu->avals.p, lu->a2lu.p, lu->lu.p);
-    st = lu->factor_device();
+    HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
+    st = lu->factor_device<double>(lu->lu.p);
     if (st != XG_OK) return st;
-    st = lu->check_pivots(&bad);
+    st = lu->check_pivots<double>(lu->lu.p, &bad);
     if (st != XG_OK) return st;
-    return bad;
+    if (bad) return bad;
+  }
+  lu->have64 = true;
+  if (lu->mode == 1) {
+    // FP32 factors with the new pivot order (the refinement falls back to the
+    // FP64 factors computed above if they do not converge)
+    st = lu->factor_device<float>(lu->lu32.p);
+    if (st != XG_OK) return st;
+    int bad32 = 0;
+    st = lu->check_pivots<float>(lu->lu32.p, &bad32);
+    if (st != XG_OK) return st;
+    if (bad32) lu->mode = 2;   // this pattern keeps FP32 off until set again
   }
   return XG_OK;
 }
@@ -690,7 +823,9 @@ int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
   const int n = lu->s.n;
   HIPCHK(hipMemcpyAsync(lu->b.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
   HIPCHK(hipMemsetAsync(lu->flag.p + 1, 0, sizeof(int), s));
-  int st = lu->solve_device(lu->b.p, lu->x.p);
+  int st;
+  if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
+  else st = lu->solve_device<double>(lu->lu.p, lu->y.p, lu->b.p, lu->x.p);
   if (st != XG_OK) return st;
   HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   int nanf = 0;
@@ -699,6 +834,26 @@ int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
   bool bad = nanf != 0;
   for (int i = 0; i < n && !bad; ++i) bad = !std::isfinite(x[i]);
   if (bad) { std::fill(x, x + n, 0.0); return XG_E_NAN; }
+  return XG_OK;
+}
+
+int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max) {
+  if (!lu || precision < 0 || precision > 1 || !(ir_tol > 0.0) || ir_max < 0) return XG_E_ARG;
+  lu->mode = precision;
+  lu->ir_tol = ir_tol;
+  lu->ir_max = ir_max;
+  if (precision == 1) {
+    HIPCHK(lu->lu32.alloc(lu->s.nlu));
+    HIPCHK(lu->y32.alloc(lu->s.n));
+  }
+  lu->have64 = false;
+  return XG_OK;
+}
+
+int xg_lu_precision_stats(const xg_lu* lu, double* stats) {
+  if (!lu || !stats) return XG_E_ARG;
+  for (int i = 0; i < 6; ++i) stats[i] = lu->stats[i];
+  stats[6] = lu->mode;
   return XG_OK;
 }
 
