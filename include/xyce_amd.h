@@ -31,6 +31,9 @@ extern "C" {
 #define XG_E_NAN (-99)       /* NaN in the solution (N_LAS_AmesosSolver.C:415-424) */
 #define XG_E_NOGPU (-10)     /* no HIP device */
 
+#define XG_PRED_WIN 256      /* learned predictor: unknowns per window */
+#define XG_PRED_FEAT 1344    /* learned predictor: features per window */
+
 /* Device types in netlist order (dev_type[]). */
 #define XG_DEV_MOS1 0
 #define XG_DEV_RES 1
@@ -81,6 +84,7 @@ typedef struct xg_solver_state {
 typedef struct xg_circuit xg_circuit;
 typedef struct xg_lu xg_lu;
 typedef struct xg_batch xg_batch;
+typedef struct xg_mlp xg_mlp;
 
 /* ---------------- circuit (device side) ---------------- */
 
@@ -187,6 +191,55 @@ int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
 /* Autotuner: times candidate block sizes / instance orders on the real batch
  * and keeps the fastest; result[0..2] = block size, order mode, us/launch. */
 int xg_batch_autotune(xg_batch* b, double* result);
+
+/* Learned predictor in a batch (call before the first xg_batch_advance):
+ * mode 0 off; 1 collect training pairs, up to pair_capacity accepted steps
+ * per instance; 2 serve `net` (in_dim XG_PRED_FEAT, out_dim XG_PRED_WIN):
+ * after each launch the network proposes the Newton initial guess of every
+ * instance that starts a step, x = x_poly + s * y per window of
+ * XG_PRED_WIN unknowns (features: see xg_batch_pairs).  A proposal with a
+ * non-finite or |y| > 8 entry is not used, and a step whose Newton solve
+ * fails from a proposal is retried from the polynomial predictor, so the
+ * step control and tolerances are those of the run without it. */
+int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity);
+/* Collected pairs (mode 1), one record per window of an accepted step:
+ * features x [XG_PRED_FEAT] bf16 = (x_n - x_{n-1}, x_{n-1} - x_{n-2},
+ * x_{n-2} - x_{n-3}, x_poly - x_n, x_cubic - x_poly) / s (XG_PRED_WIN each;
+ * s = the window's largest |entry| of the first four + 1e-9), then
+ * h_{n+1}/h_n, h_n/h_{n-1}, h_{n-1}/h_{n-2}, order, log10(s)/10, 1, zeros;
+ * target t [XG_PRED_WIN] = (x_{n+1} - x_poly) / s; mask m = 1 where the
+ * window row exists.  Copies up to max_records into the device buffers
+ * (NULL: count only); *count = records copied (or available). */
+int xg_batch_pairs(xg_batch* b, long long max_records, unsigned short* x_dev, float* t_dev, float* m_dev,
+                   long long* count);
+/* c[0..4] = pairs collected, steps started from a network proposal, such
+ * steps whose Newton solve failed (retried from the polynomial predictor),
+ * windows, mode. */
+int xg_batch_pred_counters(xg_batch* b, double* c);
+
+/* ---------------- learned Newton predictor network ---------------- */
+/* No counterpart in the reference.  An MLP (hidden_layers GELU layers of
+ * `width`, linear output) with BF16 weights/activations and FP32
+ * accumulation on the matrix cores, trained by its own backward and Adam
+ * kernels.  It proposes the Newton initial guess that
+ * OneStep::obtainPredictor (N_TIA_OneStep.C:135) computes by polynomial
+ * extrapolation; the polynomial predictor stays the fallback.  All
+ * dimensions and row counts are multiples of 64.  Pointers named *_dev are
+ * device pointers; bf16 values are passed as their uint16 bit patterns. */
+int xg_mlp_create(int in_dim, int width, int hidden_layers, int out_dim, int max_rows, unsigned long long seed,
+                  xg_mlp** out);
+int xg_mlp_destroy(xg_mlp* net);
+/* FP32 master parameters, per layer W [out][in] then b [out]; set = 1
+ * uploads (and resets the Adam moments), 0 downloads; *count = total. */
+int xg_mlp_params(xg_mlp* net, int set, float* params, long long* count);
+/* y_dev [rows][out_dim] = net(x_dev [rows][in_dim]) */
+int xg_mlp_forward(xg_mlp* net, int rows, const unsigned short* x_dev, float* y_dev);
+/* Forward and backward of loss = sum(mask * (y - t)^2) / count; leaves the
+ * gradients for xg_mlp_adam; copies them to grads (host, may be NULL). */
+int xg_mlp_loss_backward(xg_mlp* net, int rows, const unsigned short* x_dev, const float* t_dev,
+                         const float* mask_dev, int count, double* loss, float* grads);
+int xg_mlp_adam(xg_mlp* net, float lr, float beta1, float beta2, float eps);
+int xg_mlp_sync(xg_mlp* net);
 
 /* ---------------- host-only introspection (no GPU needed) ---------------- */
 

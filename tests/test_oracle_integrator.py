@@ -1,0 +1,132 @@
+"""Pins the oracle's Newton loop, DC operating point, trapezoid integrator,
+step control and sparse LU against closed forms and library routines.
+
+* resistor divider and current-source-into-resistor DC points (exact);
+* RC charging from a PWL ramp: the node voltage against the analytic
+  response of a first-order system to a ramp-then-hold input, within the
+  local-error tolerance (RELTOL 1e-3 of the swing);
+* RC discharge from an initial DC point: exp(-t/RC);
+* a linear circuit converges in one Newton solve per step once the
+  predictor is exact to the tolerance, and never needs more than two;
+* the sparse LU (min-degree + threshold partial pivoting) against numpy on
+  ring Jacobians and random diagonally weak matrices.
+"""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle, lu_solve
+from xyce_amd import circuits as C
+
+
+def _node_col(o, c, node):
+    return int(o.pattern()["node_gid"][node])
+
+
+def test_resistor_divider_dc():
+    c = C.Circuit()
+    a, b = c.node("a"), c.node("b")
+    c.add_vsrc(a, 0, c.add_wave(C.W_DC, [3.0]))
+    c.add_res(a, b, 1000.0)
+    c.add_res(b, 0, 2000.0)
+    o = Oracle(c)
+    st, x, *_ = o.dcop()
+    assert st > 0
+    assert abs(x[_node_col(o, c, b)] - 2.0) <= 1e-12
+    assert abs(x[_node_col(o, c, a)] - 3.0) <= 1e-12
+
+
+def test_current_source_into_resistor_dc():
+    c = C.Circuit()
+    a = c.node("a")
+    c.add_isrc(0, a, c.add_wave(C.W_DC, [2e-3]))   # 2 mA from ground into a
+    c.add_res(a, 0, 500.0)
+    o = Oracle(c)
+    st, x, *_ = o.dcop()
+    assert st > 0
+    assert abs(x[_node_col(o, c, a)] - 1.0) <= 1e-12
+
+
+def _rc(wave_params, r=1e3, cap=1e-12):
+    c = C.Circuit()
+    a, b = c.node("a"), c.node("b")
+    c.add_vsrc(a, 0, c.add_wave(C.W_PWL, wave_params))
+    c.add_res(a, b, r)
+    c.add_cap(b, 0, cap)
+    return c, a, b
+
+
+def _ramp_response(t, t0, tr, v, tau):
+    """First-order response y' = (u - y)/tau, y(0)=0, to u = ramp 0->v over
+    [t0, t0+tr], held at v afterwards."""
+    y = np.zeros_like(t)
+    k = v / tr
+    m1 = (t > t0) & (t <= t0 + tr)
+    s = t[m1] - t0
+    y[m1] = k * (s - tau * (1 - np.exp(-s / tau)))
+    m2 = t > t0 + tr
+    y1 = k * (tr - tau * (1 - np.exp(-tr / tau)))
+    y[m2] = v + (y1 - v) * np.exp(-(t[m2] - t0 - tr) / tau)
+    return y
+
+
+def test_rc_ramp_response_matches_closed_form():
+    t0, tr, v = 0.2e-9, 50e-12, 1.0
+    c, a, b = _rc([0.0, 0.0, t0, 0.0, t0 + tr, v])
+    o = Oracle(c)
+    r = o.tran(5e-9, 1e-12, 0.0)
+    assert r["status"] == 0
+    col = _node_col(o, c, b)
+    ref = _ramp_response(r["t"], t0, tr, v, 1e-9)
+    err = np.max(np.abs(r["x"][:, col] - ref))
+    assert err <= 2e-3 * v, err
+    # the source node follows the PWL exactly at every accepted point
+    ua = np.interp(r["t"], [0, t0, t0 + tr, 1], [0, 0, v, v])
+    assert np.max(np.abs(r["x"][:, _node_col(o, c, a)] - ua)) <= 1e-12
+    # breakpoints are hit exactly
+    for bp in (t0, t0 + tr):
+        assert np.min(np.abs(r["t"] - bp)) <= 1e-21
+
+
+def test_rc_discharge_is_exponential():
+    # source steps from 1 V to 0 in 1 ps at t=0.1 ns; the cap then discharges
+    t0, tr = 0.1e-9, 1e-12
+    c, a, b = _rc([0.0, 1.0, t0, 1.0, t0 + tr, 0.0], r=2e3, cap=0.5e-12)
+    o = Oracle(c)
+    r = o.tran(4e-9, 1e-12, 0.0)
+    assert r["status"] == 0
+    col = _node_col(o, c, b)
+    t = r["t"]
+    m = t >= t0 + 5 * tr
+    ref = np.exp(-(t[m] - t0 - tr / 2) / 1e-9)
+    err = np.max(np.abs(r["x"][m, col] - ref))
+    assert err <= 3e-3, err
+    assert abs(r["x"][0, col] - 1.0) <= 1e-12      # DC point
+
+
+def test_linear_circuit_needs_at_most_two_solves_per_step():
+    c = C.power_grid(6, 6, seed=2, pad_pitch=3, n_sinks=4, tstop=1e-9)
+    o = Oracle(c)
+    r = o.tran(1e-9, 1e-12, 0.0)
+    assert r["status"] == 0 and r["rejects"] == 0
+    assert r["steps"] <= r["newton_iters"] <= 2 * r["steps"] + 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sparse_lu_matches_numpy(seed):
+    rng = np.random.default_rng(seed)
+    n = 60
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, i] = rng.uniform(1e-6, 1.0) * (1 if rng.random() < 0.7 else 1e-6)
+        for j in rng.choice(n, 4, replace=False):
+            A[i, j] += rng.normal()
+    rp = [0]
+    ci, v = [], []
+    for i in range(n):
+        nz = np.nonzero(A[i])[0]
+        ci += list(nz); v += list(A[i, nz]); rp.append(len(ci))
+    b = rng.normal(size=n)
+    st, x = lu_solve(n, np.array(rp), np.array(ci), np.array(v), b)
+    xn = np.linalg.solve(A, b)
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b) * np.linalg.cond(A) ** 0.5
+    np.testing.assert_allclose(x, xn, rtol=1e-8, atol=1e-10 * np.max(np.abs(xn)))

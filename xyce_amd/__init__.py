@@ -42,7 +42,12 @@ EXPORTS = [
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
     "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse",
+    "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
+    "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
 ]
+
+PRED_WIN = 256      # learned predictor: unknowns per window (XG_PRED_WIN)
+PRED_FEAT = 1344    # features per window (XG_PRED_FEAT)
 
 
 class XgError(RuntimeError):
@@ -339,6 +344,32 @@ class Batch:
                 break
         return self.counters()
 
+    def set_predictor(self, mode, net=None, pair_capacity=0):
+        """Learned predictor: mode 0 off, 1 collect training pairs, 2 serve `net` (an MLP)."""
+        self._net = net
+        _check(lib().xg_batch_predictor(self.h, int(mode), net.h if net is not None else None, int(pair_capacity)),
+               "xg_batch_predictor")
+
+    def pairs(self, max_records=None):
+        """Collected training pairs as CUDA tensors (x bf16 [r][PRED_FEAT], t, m float32 [r][PRED_WIN])."""
+        import torch
+        cnt = C.c_longlong()
+        _check(lib().xg_batch_pairs(self.h, 0, None, None, None, C.byref(cnt)), "xg_batch_pairs")
+        n = cnt.value if max_records is None else min(cnt.value, int(max_records))
+        x = torch.zeros((n, PRED_FEAT), dtype=torch.bfloat16, device="cuda")
+        t = torch.zeros((n, PRED_WIN), dtype=torch.float32, device="cuda")
+        m = torch.zeros((n, PRED_WIN), dtype=torch.float32, device="cuda")
+        if n:
+            _check(lib().xg_batch_pairs(self.h, C.c_longlong(n), C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()),
+                                        C.c_void_p(m.data_ptr()), C.byref(cnt)), "xg_batch_pairs")
+        return x, t, m
+
+    def pred_counters(self):
+        c = np.zeros(5)
+        _check(lib().xg_batch_pred_counters(self.h, _p(c)), "xg_batch_pred_counters")
+        return dict(pairs=int(c[0]), learned_steps=int(c[1]), learned_newton_failures=int(c[2]),
+                    windows=int(c[3]), mode=int(c[4]))
+
     def config(self, block=0, iters_per_launch=0):
         _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch)), "xg_batch_config")
 
@@ -359,3 +390,67 @@ class Batch:
         r = np.zeros(3)
         _check(lib().xg_batch_autotune(self.h, _p(r)), "xg_batch_autotune")
         return dict(block=int(r[0]), order_mode=int(r[1]), us_per_launch=float(r[2]))
+
+
+class MLP:
+    """Learned Newton-predictor network on the GPU: BF16 weights and
+    activations, FP32 accumulation on the matrix cores, hand-written forward,
+    backward and Adam kernels (xg_mlp_*).  Device buffers are passed as torch
+    CUDA tensors (bf16 inputs, float32 targets/masks/outputs)."""
+
+    def __init__(self, in_dim, width=512, hidden_layers=3, out_dim=256, max_rows=4096, seed=5):
+        self.in_dim, self.width, self.hidden, self.out_dim = in_dim, width, hidden_layers, out_dim
+        self.max_rows = max_rows
+        h = C.c_void_p()
+        _check(lib().xg_mlp_create(in_dim, width, hidden_layers, out_dim, max_rows, C.c_ulonglong(seed),
+                                   C.byref(h)), "xg_mlp_create")
+        self.h = h
+        cnt = C.c_longlong()
+        _check(lib().xg_mlp_params(self.h, 0, None, C.byref(cnt)), "xg_mlp_params")
+        self.n_params = cnt.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().xg_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+    def dims(self):
+        return [(self.in_dim if l == 0 else self.width, self.width if l < self.hidden else self.out_dim)
+                for l in range(self.hidden + 1)]
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(lib().xg_mlp_params(self.h, 0, _p(p), None), "xg_mlp_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(lib().xg_mlp_params(self.h, 1, _p(p), None), "xg_mlp_params")
+
+    @staticmethod
+    def _dp(t):
+        assert t.is_cuda and t.is_contiguous()
+        return C.c_void_p(t.data_ptr())
+
+    def forward(self, x, y):
+        """y[rows][out] (float32, cuda) = net(x[rows][in] (bfloat16, cuda))."""
+        rows = x.shape[0]
+        _check(lib().xg_mlp_forward(self.h, rows, self._dp(x), self._dp(y)), "xg_mlp_forward")
+
+    def loss_backward(self, x, t, mask, count, want_grads=False):
+        rows = x.shape[0]
+        loss = C.c_double()
+        g = np.zeros(self.n_params, np.float32) if want_grads else None
+        _check(lib().xg_mlp_loss_backward(self.h, rows, self._dp(x), self._dp(t), self._dp(mask), int(count),
+                                          C.byref(loss), _p(g)), "xg_mlp_loss_backward")
+        return (loss.value, g) if want_grads else loss.value
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(lib().xg_mlp_adam(self.h, C.c_float(lr), C.c_float(beta1), C.c_float(beta2), C.c_float(eps)),
+               "xg_mlp_adam")
+
+    def sync(self):
+        _check(lib().xg_mlp_sync(self.h), "xg_mlp_sync")

@@ -71,13 +71,21 @@ struct InstState {
   double t, h, psi0, psi1, psi2, last_h, h_sum, saved_h, est, ck, beta1, beta2, errwt;
   double norm_init, norm_old, tmp_rate, wt_dx;
   double steps_taken;
-  double pad[31];
+  // learned predictor: step sizes of the last 4 accepted steps (newest
+  // first), accepted points in the history, flags and counters
+  double hP[4];
+  int n_acc, feat_valid, pred_skip, used_learned;
+  long long n_pairs, n_learned, n_learned_rejects;
+  double pad[27];
 };
 
 struct InstDesc {
   int cls, dev_off, vec_off, nnz_off, src_off, rec_off;
   long long dense_off;   // n*n scratch for the partial-pivoting fallback
   long long term_off;    // this instance's slot-ordered term array
+  int bp_off, n_bp;      // this instance's breakpoints (sorted, ending at tstop)
+  int win_off;           // first predictor window (rows of 256 unknowns)
+  long long pair_off;    // first training-pair record (capacity * windows)
 };
 
 struct BatchArgs {
@@ -100,8 +108,8 @@ struct BatchArgs {
   // linear part (offset nnz_off) and sources (offset src_off)
   const double* linF; const double* linQ;
   const int* src_wave; const int* wave_type; const int* wave_off; const double* wave_data;
-  // breakpoints (common to all instances)
-  int n_bp; const double* bp;
+  // breakpoints: per instance, concatenated (InstDesc::bp_off / n_bp)
+  const double* bp;
   // transient parameters
   double tstop, tstep, hmax;
   // recording
@@ -111,7 +119,17 @@ struct BatchArgs {
   int iters;                   // Newton iterations per launch
   unsigned long long* prof;    // per-phase wall-clock ticks of one instance (null = off)
   double* dense;           // per-instance n*n scratch
-  int* ipiv_scratch;       // unused (row swaps are applied in place)
+  // learned predictor (mode 0 off, 1 collect training pairs, 2 serve)
+  int pred_mode;
+  int pair_cap;            // training pairs per instance
+  double* xP;              // last 4 accepted solutions, planes of total_n
+  long long total_n;
+  unsigned short* feat;    // [windows][XG_PRED_FEAT] bf16 features of the current step
+  double* wscale;          // [windows] feature scale
+  const float* pred_out;   // [windows][XG_PRED_WIN] network output
+  unsigned short* pair_x;  // [records][XG_PRED_FEAT]
+  float* pair_t;           // [records][XG_PRED_WIN] target (x_new - x_poly) / scale
+  float* pair_m;           // [records][XG_PRED_WIN] 1 for rows that exist
 };
 
 constexpr double kRelTol = 1e-3, kAbsTol = 1e-6;                       // TIA defaults
@@ -172,9 +190,117 @@ __device__ __forceinline__ double residual_combine(bool dc, int order, double h,
 }
 #pragma clang fp contract(fast)
 
-__device__ double next_bp(const BatchArgs& A, InstState& S) {
-  while (S.bpi < A.n_bp && A.bp[S.bpi] <= S.t * (1 + 1e-13) + 1e-30) ++S.bpi;
-  return S.bpi < A.n_bp ? A.bp[S.bpi] : A.tstop;
+__device__ double next_bp(const BatchArgs& A, const InstDesc& I, InstState& S) {
+  const double* bp = A.bp + I.bp_off;
+  while (S.bpi < I.n_bp && bp[S.bpi] <= S.t * (1 + 1e-13) + 1e-30) ++S.bpi;
+  return S.bpi < I.n_bp ? bp[S.bpi] : A.tstop;
+}
+
+__device__ __forceinline__ unsigned short d_f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// Learned-predictor features of the step about to be taken, per window of
+// XG_PRED_WIN unknowns: with x_n..x_{n-3} the last accepted solutions and
+// x_poly the polynomial predictor, the scaled vectors
+//   (x_n - x_{n-1}, x_{n-1} - x_{n-2}, x_{n-2} - x_{n-3}, x_poly - x_n,
+//    x_cubic - x_poly) / s,
+// x_cubic the cubic through the 4 points at t_{n+1}, s the window's largest
+// |entry| of the first four (+1e-9), then the step-size ratios.  The
+// network's output y gives x = x_poly + s y.
+__device__ void pred_features(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S, double* red) {
+  const int n = C.n, off = I.vec_off, tid = threadIdx.x, nth = blockDim.x;
+  const long long TN = A.total_n;
+  const int nw = (n + XG_PRED_WIN - 1) / XG_PRED_WIN;
+  // Lagrange weights of the cubic at t_{n+1} (times relative to t_n)
+  const double h1 = S.h, ha = S.hP[0], hb = S.hP[1], hc = S.hP[2];
+  const double tau[4] = {0.0, -ha, -ha - hb, -ha - hb - hc};
+  double L[4];
+  for (int j = 0; j < 4; ++j) {
+    double v = 1.0;
+    for (int k = 0; k < 4; ++k)
+      if (k != j) v *= (h1 - tau[k]) / (tau[j] - tau[k]);
+    L[j] = v;
+  }
+  for (int w = 0; w < nw; ++w) {
+    const int r0 = w * XG_PRED_WIN, r1 = min(n, r0 + XG_PRED_WIN);
+    double mx = 0.0;
+    for (int r = r0 + tid; r < r1; r += nth) {
+      const double x0 = A.xP[off + r], x1 = A.xP[TN + off + r], x2 = A.xP[2 * TN + off + r],
+                   x3 = A.xP[3 * TN + off + r];
+      const double xp = A.xn0[off + r];
+      mx = fmax(mx, fmax(fmax(fabs(x0 - x1), fabs(x1 - x2)), fmax(fabs(x2 - x3), fabs(xp - x0))));
+    }
+    mx = block_max(mx, red);
+    const double sc = mx + 1e-9;
+    unsigned short* f = A.feat + (size_t)(I.win_off + w) * XG_PRED_FEAT;
+    for (int j = tid; j < XG_PRED_WIN; j += nth) {
+      const int r = r0 + j;
+      float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      if (r < r1) {
+        const double x0 = A.xP[off + r], x1 = A.xP[TN + off + r], x2 = A.xP[2 * TN + off + r],
+                     x3 = A.xP[3 * TN + off + r];
+        const double xp = A.xn0[off + r];
+        const double xc = L[0] * x0 + L[1] * x1 + L[2] * x2 + L[3] * x3;
+        v[0] = (float)((x0 - x1) / sc); v[1] = (float)((x1 - x2) / sc); v[2] = (float)((x2 - x3) / sc);
+        v[3] = (float)((xp - x0) / sc); v[4] = (float)((xc - xp) / sc);
+      }
+      for (int q = 0; q < 5; ++q) f[q * XG_PRED_WIN + j] = d_f2bf(v[q]);
+    }
+    for (int j = 5 * XG_PRED_WIN + tid; j < XG_PRED_FEAT; j += nth) {
+      const int q = j - 5 * XG_PRED_WIN;
+      float v = 0.f;
+      if (q == 0) v = (float)(h1 / ha);
+      else if (q == 1) v = (float)(ha / hb);
+      else if (q == 2) v = (float)(hb / hc);
+      else if (q == 3) v = (float)S.order;
+      else if (q == 4) v = (float)(log10(sc) / 10.0);
+      else if (q == 5) v = 1.0f;
+      f[j] = d_f2bf(v);
+    }
+    if (tid == 0) A.wscale[I.win_off + w] = sc;
+  }
+}
+
+// Accepted step: the training pair of this step (collect mode) and the
+// shift of the solution history.
+__device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S,
+                            const double* xs) {
+  const int n = C.n, off = I.vec_off, tid = threadIdx.x, nth = blockDim.x;
+  const long long TN = A.total_n;
+  const int nw = (n + XG_PRED_WIN - 1) / XG_PRED_WIN;
+  if (A.pred_mode == 1 && S.feat_valid && S.n_pairs < A.pair_cap) {
+    for (int w = 0; w < nw; ++w) {
+      const size_t rec = (size_t)I.pair_off + (size_t)S.n_pairs * nw + w;
+      const unsigned short* f = A.feat + (size_t)(I.win_off + w) * XG_PRED_FEAT;
+      for (int j = tid; j < XG_PRED_FEAT; j += nth) A.pair_x[rec * XG_PRED_FEAT + j] = f[j];
+      const double sc = A.wscale[I.win_off + w];
+      for (int j = tid; j < XG_PRED_WIN; j += nth) {
+        const int r = w * XG_PRED_WIN + j;
+        const bool ok = r < n;
+        A.pair_t[rec * XG_PRED_WIN + j] = ok ? (float)((xs[r] - A.xn0[off + r]) / sc) : 0.f;
+        A.pair_m[rec * XG_PRED_WIN + j] = ok ? 1.f : 0.f;
+      }
+    }
+  }
+  for (int r = tid; r < n; r += nth) {
+    A.xP[3 * TN + off + r] = A.xP[2 * TN + off + r];
+    A.xP[2 * TN + off + r] = A.xP[TN + off + r];
+    A.xP[TN + off + r] = A.xP[off + r];
+    A.xP[off + r] = xs[r];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    if (A.pred_mode == 1 && S.feat_valid && S.n_pairs < A.pair_cap) S.n_pairs++;
+    S.hP[3] = S.hP[2]; S.hP[2] = S.hP[1]; S.hP[1] = S.hP[0]; S.hP[0] = S.h;
+    S.n_acc++;
+    S.feat_valid = 0;
+    S.pred_skip = 0;
+    S.used_learned = 0;
+  }
 }
 
 // Top of a time step: clamp to the next breakpoint, OneStep::updateCoeffs
@@ -182,7 +308,7 @@ __device__ double next_bp(const BatchArgs& A, InstState& S) {
 __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C, const InstDesc& I, double* red) {
   __shared__ double sh[4];
   if (threadIdx.x == 0) {
-    const double stop = next_bp(A, S);
+    const double stop = next_bp(A, I, S);
     if (S.t + S.h > stop) { S.saved_h = S.h; S.h = stop - S.t; S.h_sum = S.h + S.last_h; }
     const double t1 = S.h;
     if (S.order == 2) S.psi2 = S.psi1;
@@ -216,6 +342,17 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
   mx = block_max(mx, red);
   if (threadIdx.x == 0) S.errwt = kRelTol * mx + kAbsTol;
   __syncthreads();
+  if (A.pred_mode != 0 && S.n_acc >= 4) {
+    pred_features(A, I, C, S, red);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      S.feat_valid = 1;
+      // serve: wait for the network's proposal (the apply kernel resumes the
+      // instance); after a Newton failure with it, retry with the polynomial one
+      if (A.pred_mode == 2 && !S.pred_skip) S.mode = 3;
+    }
+    __syncthreads();
+  }
 }
 
 // Dense LU with partial pivoting of one instance's Jacobian, in the
@@ -613,11 +750,13 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       A.xH1[off + i] = 0.0;
       A.xH2[off + i] = 0.0;
       A.qH2[off + i] = 0.0;
+      if (A.pred_mode != 0) A.xP[off + i] = xs[i];   // the operating point starts the history
     }
+    if (tid == 0 && A.pred_mode != 0) S.n_acc = 1;
     if (tid == 0) {
       S.mode = 1;
       S.t = 0.0;
-      const double stop = next_bp(A, S);
+      const double stop = next_bp(A, I, S);
       double hh = 0.005 * fabs(stop - S.t);
       if (A.tstep > 0.0) hh = fmin(A.tstep, hh);
       if (A.hmax > 0 && hh > A.hmax) hh = A.hmax;
@@ -690,6 +829,7 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
       for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
     }
+    if (A.pred_mode != 0) pred_accept(A, I, C, S, xs);
     if (tid == 0) S.t = S.t + S.h;
     __syncthreads();
     if (A.rec_max > 0 && S.n_rec < A.rec_max) {
@@ -705,11 +845,11 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       sh_int[5] = 0;
       if (S.t >= A.tstop * (1 - 1e-12)) {
         S.mode = 2;
-      } else if (S.bpi < A.n_bp && fabs(S.t - A.bp[S.bpi]) <= 1e-13 * fmax(1.0, fabs(S.t)) + 1e-24) {
+      } else if (S.bpi < I.n_bp && fabs(S.t - A.bp[I.bp_off + S.bpi]) <= 1e-13 * fmax(1.0, fabs(S.t)) + 1e-24) {
         // breakpoint: restart at order 1 (OneStep::initialize, not at t = 0)
         S.saved_h = newh;
         ++S.bpi;
-        const double stop = next_bp(A, S);
+        const double stop = next_bp(A, I, S);
         double hh = 0.1 * fmin(S.saved_h, fabs(stop - S.t));
         if (A.hmax > 0 && hh > A.hmax) hh = A.hmax;
         S.h = fmax(hh, kHmin);
@@ -739,6 +879,12 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
   if (tid == 0) {
     S.n_rejects++;
     S.nef++;
+    if (status <= 0 && S.used_learned) {
+      // Newton failed from the network's proposal: retry from the polynomial one
+      S.pred_skip = 1;
+      S.n_learned_rejects++;
+    }
+    S.used_learned = 0;
     const int ord = S.order;
     if (ord >= 1) S.psi0 = S.psi1;
     if (ord >= 2) S.psi1 = S.psi2;
@@ -805,6 +951,54 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   if (tid == 0) A.st[inst] = S;
 }
 
+// Resumes the instances waiting for the network's proposal (mode 3):
+// x = x_poly + s * y per window when every |y| <= kPredTrust and finite,
+// else x = x_poly (the polynomial predictor is the fallback).
+constexpr float kPredTrust = 8.0f;
+__global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
+  const int inst = blockIdx.x;
+  if (inst >= A.n_inst) return;
+  __shared__ int mode, bad;
+  if (threadIdx.x == 0) { mode = A.st[inst].mode; bad = 0; }
+  __syncthreads();
+  if (mode != 3) return;
+  const InstDesc I = A.inst[inst];
+  const ClassDesc C = A.cls[I.cls];
+  const int n = C.n, off = I.vec_off;
+  int b = 0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const float y = A.pred_out[(size_t)(I.win_off + r / XG_PRED_WIN) * XG_PRED_WIN + r % XG_PRED_WIN];
+    if (!(fabsf(y) <= kPredTrust)) b = 1;
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  const bool use = bad == 0;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) {
+    const int w = r / XG_PRED_WIN;
+    const double y = A.pred_out[(size_t)(I.win_off + w) * XG_PRED_WIN + r % XG_PRED_WIN];
+    A.x[off + r] = A.xn0[off + r] + (use ? A.wscale[I.win_off + w] * y : 0.0);
+  }
+  if (threadIdx.x == 0) {
+    InstState& S = A.st[inst];
+    S.mode = 1;
+    S.used_learned = use ? 1 : 0;
+    if (use) S.n_learned++;
+  }
+}
+
+// Gathers collected training-pair records (src[i] -> row i).
+__global__ void k_pair_gather(long long n, const long long* src, const unsigned short* px, const float* pt,
+                              const float* pm, unsigned short* ox, float* ot, float* om) {
+  const long long i = blockIdx.x;
+  if (i >= n) return;
+  const long long s = src[i];
+  for (int j = threadIdx.x; j < XG_PRED_FEAT; j += blockDim.x) ox[i * XG_PRED_FEAT + j] = px[s * XG_PRED_FEAT + j];
+  for (int j = threadIdx.x; j < XG_PRED_WIN; j += blockDim.x) {
+    ot[i * XG_PRED_WIN + j] = pt[s * XG_PRED_WIN + j];
+    om[i * XG_PRED_WIN + j] = pm[s * XG_PRED_WIN + j];
+  }
+}
+
 __global__ void k_init_state(int n_inst, InstState* st) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_inst) return;
@@ -854,6 +1048,13 @@ struct xg_batch {
   DArr<double> d_x, d_xn0, d_xH0, d_xH1, d_xH2, d_qH0, d_qH1, d_qH2, d_F, d_Q, d_B, d_F0, d_B0, d_wt;
   DArr<double> d_linF, d_linQ, d_wave_data, d_bp, d_rec_t, d_rec_x, d_dense;
   DArr<unsigned long long> d_prof;
+  // learned predictor
+  int pred_mode = 0, pred_rows = 0, total_windows = 0, pair_cap = 0;
+  long long total_n = 0, total_records = 0;
+  xg_mlp* net = nullptr;
+  DArr<double> d_xP, d_wscale;
+  DArr<unsigned short> d_feat, d_pair_x;
+  DArr<float> d_pred_out, d_pair_t, d_pair_m;
   BatchArgs args{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -866,7 +1067,6 @@ struct xg_batch {
   double last_seconds = 0;
   double tune_us = 0;
   int order_mode = 1;
-  std::vector<double> bps;
   ~xg_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -887,8 +1087,14 @@ struct xg_batch {
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     graph_len = 0;
   }
-  void launch() {
+  int launch() {
     hipLaunchKernelGGL(k_batch_step, dim3(n_inst), dim3(block), shmem, stream, args);
+    if (pred_mode == 2) {
+      const int st = mlp_forward_on_stream(net, pred_rows, d_feat.p, d_pred_out.p, stream);
+      if (st != XG_OK) return st;
+      hipLaunchKernelGGL(k_pred_apply, dim3(n_inst), dim3(256), 0, stream, args);
+    }
+    return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
   }
 };
 
@@ -1112,14 +1318,19 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       wave_off.push_back((int)wave_data.size());
     }
     for (auto& s : hc.srcs) src_wave.push_back(wbase + s.wave);
-    for (size_t si = 0; si < hc.srcs.size(); ++si) wave_breakpoints(hc, hc.srcs[si].wave, tstep, tstop, all_bp);
+    {
+      // this instance's own breakpoints (a Xyce run of it alone stops at these)
+      std::vector<double> bp;
+      for (size_t si = 0; si < hc.srcs.size(); ++si) wave_breakpoints(hc, hc.srcs[si].wave, tstep, tstop, bp);
+      bp.push_back(tstop);
+      std::sort(bp.begin(), bp.end());
+      bp.erase(std::unique(bp.begin(), bp.end()), bp.end());
+      b->insts[i].bp_off = (int)all_bp.size();
+      b->insts[i].n_bp = (int)bp.size();
+      all_bp.insert(all_bp.end(), bp.begin(), bp.end());
+    }
     dev_off += hc.n_mos; vec_off += hc.n; nnz_off += hc.nnz; src_off += (int)hc.srcs.size();
   }
-  // common breakpoint list (the union over instances; every instance stops at all of them)
-  all_bp.push_back(tstop);
-  std::sort(all_bp.begin(), all_bp.end());
-  all_bp.erase(std::unique(all_bp.begin(), all_bp.end()), all_bp.end());
-  b->bps = all_bp;
   const size_t NT = dev_off;
   if (hipStreamCreate(&b->stream) != hipSuccess) return XG_E_HIP;
   HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1));
@@ -1155,11 +1366,12 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.qH1 = b->d_qH1.p; A.qH2 = b->d_qH2.p; A.F = b->d_F.p; A.Q = b->d_Q.p; A.B = b->d_B.p; A.F0 = b->d_F0.p;
   A.B0 = b->d_B0.p; A.wt = b->d_wt.p; A.linF = b->d_linF.p; A.linQ = b->d_linQ.p; A.src_wave = b->d_src_wave.p;
   A.wave_type = b->d_wave_type.p; A.wave_off = b->d_wave_off.p; A.wave_data = b->d_wave_data.p;
-  A.n_bp = (int)all_bp.size(); A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
+  A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
   A.rec_max = std::max(record_max, 0); A.rec_t = b->d_rec_t.p; A.rec_x = b->d_rec_x.p;
   A.max_n = max_n; A.max_nlu = max_nlu; A.max_blob = max_blob;
   A.n_kcls = (int)(kcls.size() / KSTRIDE); A.iters = 1; A.prof = nullptr;
-  A.dense = b->d_dense.p; A.ipiv_scratch = nullptr;
+  A.dense = b->d_dense.p;
+  A.pred_mode = 0;
   b->shmem = sizeof(double) * (((size_t)3 * (max_n + 1) + max_nlu + 1) & ~(size_t)1) +
              sizeof(unsigned short) * (size_t)max_blob;
   if (b->shmem > 150 * 1024) return XG_E_ARG;
@@ -1182,7 +1394,15 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
     if (!b->gexec || b->graph_len != graph_chunk) {
       b->drop_graph();
       HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < graph_chunk; ++i) b->launch();
+      for (int i = 0; i < graph_chunk; ++i) {
+        const int st = b->launch();
+        if (st != XG_OK) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(b->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          return st;
+        }
+      }
       HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
       HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
       b->graph_len = graph_chunk;
@@ -1192,7 +1412,10 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
       left -= graph_chunk;
     }
   }
-  for (; left > 0; --left) b->launch();
+  for (; left > 0; --left) {
+    const int st = b->launch();
+    if (st != XG_OK) return st;
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(b->ev1, b->stream));
   HIPCHK(hipEventSynchronize(b->ev1));
@@ -1200,6 +1423,90 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_seconds = ms * 1e-3;
   b->launches += launches;
+  return XG_OK;
+}
+
+int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity) {
+  if (!b || mode < 0 || mode > 2 || pair_capacity < 0) return XG_E_ARG;
+  if (b->launches != 0) return XG_E_ARG;            // the history must start at the operating point
+  if (mode == 2) {
+    int in = 0, out = 0, max_rows = 0;
+    if (!net || mlp_dims(net, &in, &out, &max_rows) != XG_OK) return XG_E_ARG;
+    if (in != XG_PRED_FEAT || out != XG_PRED_WIN) return XG_E_ARG;
+  }
+  if (mode == 1 && pair_capacity == 0) return XG_E_ARG;
+  // windows and training-pair records per instance
+  long long win = 0, rec = 0, tn = 0;
+  for (int i = 0; i < b->n_inst; ++i) {
+    const int n = b->classes[b->insts[i].cls].n;
+    const int nw = (n + XG_PRED_WIN - 1) / XG_PRED_WIN;
+    b->insts[i].win_off = (int)win;
+    b->insts[i].pair_off = rec;
+    win += nw;
+    rec += (long long)nw * (mode == 1 ? pair_capacity : 0);
+    tn += n;
+  }
+  const int rows = (int)((win + 63) / 64 * 64);
+  if (mode == 2) {
+    int in = 0, out = 0, max_rows = 0;
+    mlp_dims(net, &in, &out, &max_rows);
+    if (rows > max_rows) return XG_E_ARG;
+  }
+  b->pred_mode = mode; b->net = net; b->pred_rows = rows; b->total_windows = (int)win;
+  b->pair_cap = mode == 1 ? pair_capacity : 0; b->total_n = tn; b->total_records = rec;
+  if (mode != 0) {
+    HIPCHK(b->d_xP.zeros((size_t)4 * tn));
+    HIPCHK(b->d_wscale.zeros(rows));
+    HIPCHK(b->d_feat.zeros((size_t)rows * XG_PRED_FEAT));
+    HIPCHK(b->d_pred_out.zeros((size_t)rows * XG_PRED_WIN));
+  }
+  if (mode == 1) {
+    HIPCHK(b->d_pair_x.zeros((size_t)rec * XG_PRED_FEAT));
+    HIPCHK(b->d_pair_t.zeros((size_t)rec * XG_PRED_WIN));
+    HIPCHK(b->d_pair_m.zeros((size_t)rec * XG_PRED_WIN));
+  }
+  HIPCHK(hipMemcpy(b->d_inst.p, b->insts.data(), sizeof(InstDesc) * b->n_inst, hipMemcpyHostToDevice));
+  BatchArgs& A = b->args;
+  A.pred_mode = mode; A.pair_cap = b->pair_cap; A.xP = b->d_xP.p; A.total_n = tn; A.feat = b->d_feat.p;
+  A.wscale = b->d_wscale.p; A.pred_out = b->d_pred_out.p; A.pair_x = b->d_pair_x.p; A.pair_t = b->d_pair_t.p;
+  A.pair_m = b->d_pair_m.p;
+  b->drop_graph();
+  return XG_OK;
+}
+
+int xg_batch_pairs(xg_batch* b, long long max_records, unsigned short* x_dev, float* t_dev, float* m_dev,
+                   long long* count) {
+  if (!b || !count || b->pred_mode != 1) return XG_E_ARG;
+  std::vector<InstState> st(b->n_inst);
+  HIPCHK(hipMemcpy(st.data(), b->d_st.p, sizeof(InstState) * b->n_inst, hipMemcpyDeviceToHost));
+  std::vector<long long> src;
+  for (int i = 0; i < b->n_inst; ++i) {
+    const int nw = (b->classes[b->insts[i].cls].n + XG_PRED_WIN - 1) / XG_PRED_WIN;
+    for (long long k = 0; k < st[i].n_pairs * nw; ++k) src.push_back(b->insts[i].pair_off + k);
+  }
+  *count = (long long)src.size();
+  if (!x_dev || !t_dev || !m_dev || max_records <= 0) return XG_OK;
+  const long long n = std::min<long long>(max_records, (long long)src.size());
+  for (long long k = 0; k < n; ++k)
+    if (src[k] < 0 || src[k] >= b->total_records) return XG_E_ARG;
+  DArr<long long> d_src;
+  src.resize(n);
+  HIPCHK(d_src.up(src));
+  if (n) hipLaunchKernelGGL(k_pair_gather, dim3((unsigned)n), dim3(256), 0, b->stream, n, d_src.p, b->d_pair_x.p,
+                            b->d_pair_t.p, b->d_pair_m.p, x_dev, t_dev, m_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  *count = n;
+  return XG_OK;
+}
+
+int xg_batch_pred_counters(xg_batch* b, double* c) {
+  if (!b || !c) return XG_E_ARG;
+  std::vector<InstState> st(b->n_inst);
+  HIPCHK(hipMemcpy(st.data(), b->d_st.p, sizeof(InstState) * b->n_inst, hipMemcpyDeviceToHost));
+  double pairs = 0, learned = 0, rej = 0;
+  for (auto& s : st) { pairs += s.n_pairs; learned += s.n_learned; rej += s.n_learned_rejects; }
+  c[0] = pairs; c[1] = learned; c[2] = rej; c[3] = b->total_windows; c[4] = b->pred_mode;
   return XG_OK;
 }
 

@@ -98,4 +98,9 @@ struct LuSymbolic {
 int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s,
                bool want_schedule = true);
 
+// learned-predictor network (mlp.hip): forward on a caller stream (a
+// hipStream_t), and its dimensions
+int mlp_forward_on_stream(xg_mlp* net, int rows, const unsigned short* x, float* y, void* stream);
+int mlp_dims(const xg_mlp* net, int* in_dim, int* out_dim, int* max_rows);
+
 }  // namespace xg
