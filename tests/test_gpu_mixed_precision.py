@@ -23,12 +23,20 @@ from xyce_amd import circuits as C
 pytestmark = pytest.mark.gpu
 
 
-def _tran_system(c, seed):
+def _tran_system(c, seed, rails):
+    """Transient Jacobian (trapezoid, h = 2 ps) at a chosen state: every node
+    near mid-rail (all MOSFETs conducting: a well-conditioned system) or
+    near the rails (stacks with both devices off leave mid nodes held only by
+    gmin: an ill-conditioned one)."""
     o = Oracle(c)
-    st, x0, sta0, sto0, Q, F, B = o.dcop()
-    assert st > 0
     rng = np.random.default_rng(seed)
-    x = x0 + rng.normal(0.0, 0.05, size=o.n)
+    gid = o.pattern()["node_gid"]
+    x = np.zeros(o.n)
+    for node in range(1, c.n_nodes):
+        v = rng.choice([0.05, 1.75]) if rails else 0.9
+        x[gid[node]] = v + rng.normal(0.0, 0.02)
+    x[gid[1]] = 1.8                       # supply node
+    sta0, sto0 = np.zeros(max(o.nsta, 1)), np.zeros(max(o.nsto, 1))
     ref = o.load(x, sta0, sto0, h=2e-12, order=2)
     rhs, J = o.combine(ref, ref["Q"], ref["F"] - ref["B"])
     return o, J, rhs
@@ -37,10 +45,8 @@ def _tran_system(c, seed):
 @pytest.mark.parametrize("gates", [300, 3000])
 def test_fp32_refined_solve_matches_fp64(gpu, gates):
     c = C.gate_dag(gates, seed=4)
-    o, J, rhs = _tran_system(c, gates)
+    o, J, rhs = _tran_system(c, gates, rails=False)
     p = o.pattern()
-    A = sp.csr_matrix((J, p["colind"], p["rowptr"]), shape=(o.n, o.n))
-    x64 = spla.spsolve(A.tocsc(), rhs)
     s = X.DirectSolver(o.n, p["rowptr"], p["colind"])
     s.set_precision(1, ir_tol=1e-12, ir_max=10)
     for rep in range(2):       # refactor + solve twice (the Newton loop's pattern)
@@ -48,8 +54,9 @@ def test_fp32_refined_solve_matches_fp64(gpu, gates):
         assert st == 0
     ps = s.precision_stats()
     assert ps["precision"] == 1 and ps["fp64_fallbacks"] == 0 and ps["fp32_pivot_failures"] == 0, ps
-    x64 = spla.spsolve((A * 1.01).tocsc(), rhs)
-    res = np.max(np.abs(A * 1.01 @ x - rhs)) / np.max(np.abs(rhs))
+    A = sp.csr_matrix((J * 1.01, p["colind"], p["rowptr"]), shape=(o.n, o.n))
+    x64 = spla.spsolve(A.tocsc(), rhs)
+    res = np.max(np.abs(A @ x - rhs)) / np.max(np.abs(rhs))
     err = np.abs(x - x64) / (1e-3 * np.abs(x64) + 1e-6)
     print(f"n={o.n}: corrections {ps['last_corrections']}, rel residual {res:.2e}, "
           f"max per-node error {err.max():.2e} of RELTOL|x|+ABSTOL")
@@ -60,6 +67,29 @@ def test_fp32_refined_solve_matches_fp64(gpu, gates):
     st, xd = s64.doSolve(J * 1.01, rhs)
     assert st == 0
     assert np.max(np.abs(xd - x) / (1e-3 * np.abs(x64) + 1e-6)) <= 1e-9
+
+
+def test_fp32_refined_solve_on_ill_conditioned_state(gpu):
+    """Near-rail state: whatever mix of FP32 refinement and FP64 fallback the
+    solver picks, the answer matches the FP64 path's."""
+    c = C.gate_dag(2000, seed=6)
+    o, J, rhs = _tran_system(c, 9, rails=True)
+    p = o.pattern()
+    s64 = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    st, xd = s64.doSolve(J, rhs)
+    assert st == 0
+    s = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    s.set_precision(1, ir_tol=1e-12, ir_max=10)
+    st, x = s.doSolve(J, rhs)
+    assert st == 0
+    ps = s.precision_stats()
+    A = sp.csr_matrix((J, p["colind"], p["rowptr"]), shape=(o.n, o.n))
+    r32 = np.max(np.abs(A @ x - rhs)) / np.max(np.abs(rhs))
+    r64 = np.max(np.abs(A @ xd - rhs)) / np.max(np.abs(rhs))
+    err = np.abs(x - xd) / (1e-3 * np.abs(xd) + 1e-6)
+    print(f"n={o.n}: {ps}; residual fp32+IR {r32:.2e} fp64 {r64:.2e}; max per-node diff {err.max():.2e}")
+    assert r32 <= max(1e-12, 10 * r64)
+    assert err.max() <= 1e-6
 
 
 def _two_by_two(delta):

@@ -55,6 +55,12 @@ struct ClassDesc {
   int blob_off, blob_len;
   int b_flp, b_ftg, b_fop, b_fdiv, b_fops, b_lurp, b_luci, b_diag, b_fsp, b_fsr, b_bsp, b_bsr, b_prow, b_pcol,
       b_rp, b_cc, b_a2lu;
+  // long LU targets / solve rows (>= kLongOps terms, one wavefront each) come
+  // last in their level: first long index per level
+  int b_fls, b_fss, b_bss;
+  // term ranges, linear entries (CSR slots with a nonzero linear value, per
+  // row) and per-row source lists, all uint16
+  int b_vptr, b_mptr, b_lrp, b_lk, b_bgp, b_bge;
 };
 
 // ---- per instance state (one 512-byte record in HBM)
@@ -140,6 +146,8 @@ constexpr int kDcMaxNewton = 200;
 constexpr double kHmin = 1e-18;
 constexpr int kLongSeg = 32;     // term groups this long are reduced by a wavefront
 constexpr int kMaxLong = 16;     // per class
+constexpr int kLongOps = 16;     // LU targets / solve rows this long are reduced by a wavefront
+constexpr int kMaxSrc = 64;      // sources per instance (their values are staged in LDS)
 
 // Block reductions: 64-lane wave shuffles, then one LDS exchange across waves.
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -367,7 +375,7 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
   const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
   double* D = A.dense + I.dense_off;
   const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
-  const int* mptr = ci + C.o_mptr;
+  const unsigned short* mptr = prog + C.b_mptr;
   const double* T = A.terms + I.term_off;
   const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
   for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
@@ -437,7 +445,8 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
 // runs A.iters of them back to back with the instance state kept in LDS).
 __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S,
                                 double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
-                                double* y, double* lu, const unsigned short* prog, const double* kcls) {
+                                double* y, double* lu, const unsigned short* prog, const double* kcls,
+                                double* srcv) {
   const int tid = threadIdx.x, nth = blockDim.x;
   const int n = C.n, off = I.vec_off;
   const int* ci = A.ci;
@@ -460,6 +469,16 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
   __syncthreads();
   for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
   if (tid == 0) xs[n] = 0.0;
+  {
+    // source values at this iteration's time, staged in LDS (read after the
+    // barrier that ends the device evaluation)
+    const double ts = dc ? 0.0 : S.t + S.h;
+    for (int k = tid; k < C.n_src; k += nth) {
+      const int w = A.src_wave[I.src_off + k];
+      const int wo = A.wave_off[w];
+      srcv[k] = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, ts, A.tstep, A.tstop);
+    }
+  }
   __syncthreads();
   XG_PHASE(0);
   // ---- 2. device evaluation (one thread per MOSFET); each term goes to its
@@ -494,8 +513,8 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
   {
     const int nl = C.n_vlong + C.n_mlong;
     const int* lg = ci + C.o_long;
-    const int* vptr = ci + C.o_vptr;
-    const int* mptr = ci + C.o_mptr;
+    const unsigned short* vptr = prog + C.b_vptr;
+    const unsigned short* mptr = prog + C.b_mptr;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
     for (int j = wave; j < nl; j += nwaves) {
       const int g = lg[j];
@@ -508,14 +527,14 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
     }
     if (nl) __syncthreads();
   }
-  const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
+  const unsigned short* cc = prog + C.b_cc;
   // ---- 3. vectors per row (contiguous term ranges) and the residual
   const double h = S.h;
-  const double tsrc = dc ? 0.0 : S.t + S.h;
   double n2 = 0.0, ninf = 0.0;
   {
-    const int* vptr = ci + C.o_vptr;
-    const int* bgp = ci + C.o_bgp; const int* bge = ci + C.o_bge;
+    const unsigned short* vptr = prog + C.b_vptr;
+    const unsigned short* bgp = prog + C.b_bgp; const unsigned short* bge = prog + C.b_bge;
+    const unsigned short* lrp = prog + C.b_lrp; const unsigned short* lk = prog + C.b_lk;
     const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
     for (int r = tid; r < n; r += nth) {
       double acc[4];
@@ -533,18 +552,17 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       double f = acc[0], q = acc[1];
       const double fl = acc[2], ql = acc[3];
       double sf = 0.0, sq = 0.0;
-      for (int k = rp[r]; k < rp[r + 1]; ++k) {
-        const double xv = xs[cc[k]];
-        if (linF[k] != 0.0) sf += linF[k] * xv;
-        if (linQ[k] != 0.0) sq += linQ[k] * xv;
+      for (int k = lrp[r]; k < lrp[r + 1]; ++k) {
+        const int sl = lk[k];
+        const double xv = xs[cc[sl]];
+        sf += linF[sl] * xv;
+        sq += linQ[sl] * xv;
       }
       f += sf; q += sq;
       double b = 0.0;
       for (int e = bgp[r]; e < bgp[r + 1]; ++e) {
         const int ent = bge[e];
-        const int w = A.src_wave[I.src_off + (ent >> 1)];
-        const int wo = A.wave_off[w];
-        const double v = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, tsrc, A.tstep, A.tstop);
+        const double v = srcv[ent >> 1];
         if (ent & 1) b -= v; else b += v;
       }
       A.F[off + r] = f; A.Q[off + r] = q; A.B[off + r] = b;
@@ -614,7 +632,7 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       for (int p = tid; p < nlu; p += nth) lu[p] = 0.0;
       __syncthreads();
       {
-        const int* mptr = ci + C.o_mptr;
+        const unsigned short* mptr = prog + C.b_mptr;
         const unsigned short* a2lu = prog + C.b_a2lu;
         const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
         const int* lg = ci + C.o_long;
@@ -639,31 +657,51 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       __syncthreads();
       XG_PHASE(5);
       {
+        // level-scheduled refactor; KLU's threshold test (|l_ij| <= 1/0.001,
+        // finite nonzero pivots) is evaluated as the entries are produced
         const unsigned short* flp = prog + C.b_flp; const unsigned short* ftg = prog + C.b_ftg;
         const unsigned short* fop = prog + C.b_fop; const unsigned short* fops = prog + C.b_fops;
-        const unsigned short* fdiv = prog + C.b_fdiv;
+        const unsigned short* fdiv = prog + C.b_fdiv; const unsigned short* fls = prog + C.b_fls;
+        const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
+        int bad = 0;
         for (int l = 0; l < C.nflev; ++l) {
-          for (int t = flp[l] + tid; t < flp[l + 1]; t += nth) {
+          const int t0 = flp[l], ts = fls[l], t1 = flp[l + 1];
+          for (int t = t0 + tid; t < ts; t += nth) {
             const int p = ftg[t];
             double v = lu[p];
             for (int o = fop[t]; o < fop[t + 1]; ++o) v -= lu[fops[2 * o]] * lu[fops[2 * o + 1]];
             const int dv = fdiv[t];
-            if (dv != 0xFFFF) v /= lu[dv];
+            if (dv != 0xFFFF) {
+              const double d = lu[dv];
+              v /= d;
+              if (!(fabs(v) <= 1.0e3) || !(fabs(d) > 0.0) || !isfinite(d)) bad = 1;
+            }
             lu[p] = v;
+          }
+          for (int j = ts + wave; j < t1; j += nwaves) {
+            double acc = 0.0;
+            for (int o = fop[j] + lane; o < fop[j + 1]; o += 64) acc += lu[fops[2 * o]] * lu[fops[2 * o + 1]];
+            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
+            if (lane == 0) {
+              const int p = ftg[j];
+              double v = lu[p] - acc;
+              const int dv = fdiv[j];
+              if (dv != 0xFFFF) {
+                const double d = lu[dv];
+                v /= d;
+                if (!(fabs(v) <= 1.0e3) || !(fabs(d) > 0.0) || !isfinite(d)) bad = 1;
+              }
+              lu[p] = v;
+            }
           }
           __syncthreads();
         }
-      }
-      XG_PHASE(6);
-      // KLU's threshold test on the static pivots: |l_ij| <= 1/0.001, finite, nonzero pivot
-      {
-        const unsigned short* lurp = prog + C.b_lurp; const unsigned short* dg = prog + C.b_diag;
-        int bad = 0;
+        XG_PHASE(6);
+        // the last pivot is never a divisor: check every diagonal
+        const unsigned short* dg = prog + C.b_diag;
         for (int i = tid; i < n; i += nth) {
-          const double dgv = lu[dg[i]];
-          if (!(fabs(dgv) > 0.0) || !isfinite(dgv)) bad = 1;
-          for (int p = lurp[i]; p < dg[i]; ++p)
-            if (!(fabs(lu[p]) <= 1.0e3)) bad = 1;
+          const double d = lu[dg[i]];
+          if (!(fabs(d) > 0.0) || !isfinite(d)) bad = 1;
         }
         if (bad) atomicOr(&sh_int[6], 1);
         __syncthreads();
@@ -678,21 +716,37 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
         const unsigned short* bsp = prog + C.b_bsp; const unsigned short* bsr = prog + C.b_bsr;
         for (int i = tid; i < n; i += nth) y[i] = rhs[prow[i]];
         __syncthreads();
+        const unsigned short* fss = prog + C.b_fss; const unsigned short* bss = prog + C.b_bss;
+        const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
         for (int l = 0; l < C.nfs; ++l) {
-          for (int t = fsp[l] + tid; t < fsp[l + 1]; t += nth) {
+          for (int t = fsp[l] + tid; t < fss[l]; t += nth) {
             const int i = fsr[t];
             double v = y[i];
             for (int p = lurp[i]; p < dg[i]; ++p) v -= lu[p] * y[luci[p]];
             y[i] = v;
           }
+          for (int j = fss[l] + wave; j < fsp[l + 1]; j += nwaves) {
+            const int i = fsr[j];
+            double acc = 0.0;
+            for (int p = lurp[i] + lane; p < dg[i]; p += 64) acc += lu[p] * y[luci[p]];
+            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
+            if (lane == 0) y[i] -= acc;
+          }
           __syncthreads();
         }
         for (int l = 0; l < C.nbs; ++l) {
-          for (int t = bsp[l] + tid; t < bsp[l + 1]; t += nth) {
+          for (int t = bsp[l] + tid; t < bss[l]; t += nth) {
             const int i = bsr[t];
             double v = y[i];
             for (int p = dg[i] + 1; p < lurp[i + 1]; ++p) v -= lu[p] * y[luci[p]];
             y[i] = v / lu[dg[i]];
+          }
+          for (int j = bss[l] + wave; j < bsp[l + 1]; j += nwaves) {
+            const int i = bsr[j];
+            double acc = 0.0;
+            for (int p = dg[i] + 1 + lane; p < lurp[i + 1]; p += 64) acc += lu[p] * y[luci[p]];
+            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
+            if (lane == 0) y[i] = (y[i] - acc) / lu[dg[i]];
           }
           __syncthreads();
         }
@@ -919,6 +973,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   __shared__ int sh_int[8];
   __shared__ double sh_d[8];
   __shared__ double kc_s[16 * KSTRIDE];
+  __shared__ double srcv[kMaxSrc];
   extern __shared__ double dyn[];
   double* xs = dyn;                         // n + 1
   double* rhs = xs + (A.max_n + 1);         // n
@@ -944,7 +999,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   }
   __syncthreads();
   for (int it = 0; it < A.iters; ++it) {
-    batch_iteration(A, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls);
+    batch_iteration(A, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv);
     __syncthreads();
     if (S.mode >= 2 || S.mode < 0) break;
   }
@@ -1153,6 +1208,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
     key.insert(key.end(), hc.mg_ent.begin(), hc.mg_ent.end());
     key.insert(key.end(), hc.mos_lid.begin(), hc.mos_lid.end());
     for (auto& s : hc.srcs) { key.push_back(s.kind); key.push_back(s.li_p); key.push_back(s.li_n); }
+    for (int k = 0; k < hc.nnz; ++k) key.push_back((hc.linF[k] != 0.0 ? 1 : 0) | (hc.linQ[k] != 0.0 ? 2 : 0));
     int cid;
     auto it = cls_of.find(key);
     if (it == cls_of.end()) {
@@ -1219,6 +1275,8 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         C.o_long = push(lg);
       }
       // per-row source lists (local source index)
+      std::vector<int> bgp_h, bge_h;
+      bool blob_ok = false;
       {
         std::vector<int> cnt(n + 1, 0);
         for (auto& e : hc.srcs) {
@@ -1237,6 +1295,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           }
         }
         C.o_bgp = push(ptr); C.o_bge = push(ent);
+        bgp_h = ptr; bge_h = ent;
       }
       C.o_lid = push(hc.mos_lid);
       C.nops = (int)sym.f_ops.size() / 2;
@@ -1250,11 +1309,60 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           for (int x : v) blob.push_back((unsigned short)x);
           return o;
         };
-        C.b_flp = put(sym.f_lvl_ptr); C.b_ftg = put(sym.f_tgt); C.b_fop = put(sym.f_op_ptr); C.b_fdiv = put(fdiv16);
-        C.b_fops = put(sym.f_ops); C.b_lurp = put(sym.lu_rowptr); C.b_luci = put(sym.lu_colind);
-        C.b_diag = put(sym.lu_diag); C.b_fsp = put(sym.fs_lvl_ptr); C.b_fsr = put(sym.fs_rows);
-        C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(sym.bs_rows); C.b_prow = put(sym.perm_row);
+        // LU targets: within each level the ones with >= kLongOps operations
+        // go last (a wavefront each); fls[l] = first of them
+        std::vector<int> ftg, fop{0}, fops, fdv, fls;
+        for (size_t l = 0; l + 1 < sym.f_lvl_ptr.size(); ++l) {
+          std::vector<int> shrt, lng;
+          for (int t = sym.f_lvl_ptr[l]; t < sym.f_lvl_ptr[l + 1]; ++t)
+            (sym.f_op_ptr[t + 1] - sym.f_op_ptr[t] >= kLongOps ? lng : shrt).push_back(t);
+          auto add = [&](int t) {
+            ftg.push_back(sym.f_tgt[t]);
+            for (int o = sym.f_op_ptr[t]; o < sym.f_op_ptr[t + 1]; ++o) {
+              fops.push_back(sym.f_ops[2 * o]); fops.push_back(sym.f_ops[2 * o + 1]);
+            }
+            fop.push_back((int)fops.size() / 2);
+            fdv.push_back(fdiv16[t]);
+          };
+          for (int t : shrt) add(t);
+          fls.push_back((int)ftg.size());
+          for (int t : lng) add(t);
+        }
+        // solve rows: long L (forward) / U (backward) parts last in their level
+        auto split_rows = [&](const std::vector<int>& lp, const std::vector<int>& rows, bool fwd,
+                              std::vector<int>& out, std::vector<int>& split) {
+          for (size_t l = 0; l + 1 < lp.size(); ++l) {
+            std::vector<int> lng;
+            for (int t = lp[l]; t < lp[l + 1]; ++t) {
+              const int i = rows[t];
+              const int len = fwd ? sym.lu_diag[i] - sym.lu_rowptr[i] : sym.lu_rowptr[i + 1] - sym.lu_diag[i] - 1;
+              if (len >= kLongOps) lng.push_back(i); else out.push_back(i);
+            }
+            split.push_back((int)out.size());
+            out.insert(out.end(), lng.begin(), lng.end());
+          }
+        };
+        std::vector<int> fsr, fss, bsr, bss;
+        split_rows(sym.fs_lvl_ptr, sym.fs_rows, true, fsr, fss);
+        split_rows(sym.bs_lvl_ptr, sym.bs_rows, false, bsr, bss);
+        // linear entries per row: CSR slots with a nonzero linear value
+        std::vector<int> lrp{0}, lk;
+        for (int r = 0; r < n; ++r) {
+          for (int k = hc.rowptr[r]; k < hc.rowptr[r + 1]; ++k)
+            if (hc.linF[k] != 0.0 || hc.linQ[k] != 0.0) lk.push_back(k);
+          lrp.push_back((int)lk.size());
+        }
+        C.b_flp = put(sym.f_lvl_ptr); C.b_ftg = put(ftg); C.b_fop = put(fop); C.b_fdiv = put(fdv);
+        C.b_fops = put(fops); C.b_fls = put(fls);
+        C.b_lurp = put(sym.lu_rowptr); C.b_luci = put(sym.lu_colind);
+        C.b_diag = put(sym.lu_diag); C.b_fsp = put(sym.fs_lvl_ptr); C.b_fsr = put(fsr); C.b_fss = put(fss);
+        C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(bsr); C.b_bss = put(bss); C.b_prow = put(sym.perm_row);
         C.b_pcol = put(sym.perm_col); C.b_rp = put(hc.rowptr); C.b_cc = put(hc.colind); C.b_a2lu = put(sym.a2lu);
+        C.b_vptr = put(vptr); C.b_mptr = put(mptr); C.b_lrp = put(lrp); C.b_lk = put(lk);
+        C.b_bgp = put(bgp_h); C.b_bge = put(bge_h);
+        blob_ok = (int)ftg.size() == C.nlu && (int)fls.size() == C.nflev && (int)fsr.size() == n &&
+                  (int)bsr.size() == n && (int)fss.size() == C.nfs && (int)bss.size() == C.nbs &&
+                  fop.back() == C.nops && (int)fops.size() == 2 * C.nops;
         while ((blob.size() - b0) % 8) blob.push_back(0);
         C.blob_off = (int)b0;
         C.blob_len = (int)(blob.size() - b0);
@@ -1272,6 +1380,8 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
                   in(tpos, -1, C.nterm);
         // every value of the uint16 program must fit (0xFFFF is the "no division" sentinel)
         ok = ok && C.nlu < 0xFFFF && (int)sym.f_ops.size() / 2 < 0xFFFF && C.nnz < 0xFFFF && C.n < 0xFFFF;
+        // term ranges are uint16 in the program; source values are staged in LDS
+        ok = ok && blob_ok && C.nterm < 0xFFFF && C.n_src <= kMaxSrc && 2 * C.nnz < 0xFFFF;
         ok = ok && (int)sym.f_op_ptr.size() == C.nlu + 1 && sym.f_op_ptr[C.nlu] * 2 == (int)sym.f_ops.size();
         ok = ok && sym.f_lvl_ptr.back() == C.nlu && sym.fs_lvl_ptr.back() == C.n && sym.bs_lvl_ptr.back() == C.n;
         ok = ok && (int)hc.rowptr.size() == C.n + 1 && hc.rowptr[C.n] == C.nnz;
