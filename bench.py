@@ -74,7 +74,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512)
     ap.add_argument("--cpu-tstop", type=float, default=3e-9)
-    ap.add_argument("--iters", type=int, default=1, help="Newton iterations per launch")
+    ap.add_argument("--form", choices=("auto", "split", "fused"), default="auto",
+                    help="launch form: autotuned, separate evaluation kernel, or fused kernel")
+    ap.add_argument("--iters", type=int, default=8, help="Newton iterations per launch of the fused form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the batched engine shards "
@@ -95,10 +97,12 @@ def main():
     t0 = time.time()
     b = X.Batch(ckts, tstop, tstep, hmax)
     log(f"batch setup {time.time() - t0:.1f}s")
+    b.config(iters_per_launch=args.iters, split_eval=0)
     tune = b.autotune()
-    b.config(iters_per_launch=args.iters)
-    tune["iters_per_launch"] = args.iters
-    log(f"autotune: {tune}")
+    if args.form != "auto":
+        b.config(iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
+    log(f"autotune: {tune} (form {args.form})")
+    k_iter = 1 if (tune["split_eval"] if args.form == "auto" else args.form == "split") else args.iters
     b.advance(args.warmup, graph_chunk=50)
     c0 = b.counters()
     torch.cuda.synchronize()
@@ -119,7 +123,9 @@ def main():
     # single-instance FP64 GPU path: the same engine on the 101-stage ring
     r1 = C.ring(101, seed=1)
     b1 = X.Batch([r1], 50e-9, tstep, hmax)
-    b1.config(iters_per_launch=args.iters)
+    b1.config(iters_per_launch=args.iters, split_eval=0)
+    tune1 = b1.autotune()
+    log(f"single-instance autotune: {tune1}")
     b1.advance(args.warmup, graph_chunk=50)
     s0 = b1.counters()
     torch.cuda.synchronize()
@@ -131,12 +137,14 @@ def main():
     single = (s1["newton_iters"] - s0["newton_iters"]) / wall1
     log(f"single instance: {single:.1f} Newton steps/s ({wall1 / args.steps * 1e6:.1f} us/launch)")
 
-    byts = algorithmic_bytes(ckts) * args.iters
+    byts = algorithmic_bytes(ckts) * k_iter
     us_launch = dev_s / args.steps * 1e6
     achieved = byts / (dev_s / args.steps) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_batch_step", "bytes_per_launch": int(byts), "us_per_launch": round(us_launch, 2)}
+                "kernel": "k_batch_eval + k_batch_step" if k_iter == 1 and tune["split_eval"] else "k_batch_step",
+                "bytes_per_launch": int(byts), "us_per_launch": round(us_launch, 2),
+                "iterations_per_launch": k_iter}
 
     cpu = None
     if not args.no_cpu:

@@ -151,9 +151,10 @@ int xg_lu_info(const xg_lu* lu, double* info);
  * factors, so a solve is never less accurate than the FP64 path. */
 int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max);
 /* stats[0..6] = corrections of the last solve, its relative residual,
- * solves, corrections in total, solves finished by FP64 factors, refactors
- * whose FP32 pivots failed the threshold test, current precision
- * (2 = FP32 switched off for this pattern after a failure). */
+ * solves refined, corrections in total, solves finished by FP64 factors
+ * after the refinement stopped, refactors whose FP32 pivots failed the
+ * threshold test where the FP64 ones passed (those solve with FP64
+ * factors), current precision. */
 int xg_lu_precision_stats(const xg_lu* lu, double* stats);
 
 /* ---------------- batched Monte Carlo engine ---------------- */
@@ -176,10 +177,14 @@ int xg_batch_counters(xg_batch* b, double* counters);
 int xg_batch_instance(xg_batch* b, int inst, double* tstate, double* x);
 /* Recorded accepted points of one instance: t[max], x[max*n]; returns count. */
 int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x);
-/* Workgroup size (64/128/256/512, 0 = keep) and Newton iterations per
- * launch (1 = one launch per batched Newton iteration; K > 1 keeps each
- * instance's state and program in LDS across K iterations, 0 = keep). */
-int xg_batch_config(xg_batch* b, int block, int iters_per_launch);
+/* Launch form.  block: workgroup size (64/128/256/512, 0 = keep).
+ * split_eval 1 (default): each launch runs a device-evaluation kernel over
+ * all MOSFETs of the iterating instances (one thread per device) and then
+ * the per-instance kernel, one Newton iteration per launch; 0: the
+ * evaluation is fused into the per-instance kernel, which then runs
+ * iters_per_launch iterations back to back with the instance's state and
+ * program kept in LDS (0 = keep); -1 = keep. */
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval);
 /* Per-phase device timers of the first-scheduled instance (the counterpart of
  * the reference's Residual / Jacobian / Linear-solve StatTop timers,
  * src/UtilityPKG/N_UTL_Stats.h:481-511): enable = 1 resets and starts,
@@ -188,8 +193,10 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch);
  * Jacobian gather, LU refactor, pivot threshold test, triangular solves,
  * update, step control; phase_us[15] = iterations timed. */
 int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
-/* Autotuner: times candidate block sizes / instance orders on the real batch
- * and keeps the fastest; result[0..2] = block size, order mode, us/launch. */
+/* Autotuner: times the launch forms (split / fused with several iterations
+ * per launch) x workgroup sizes x instance orders on the real batch and keeps
+ * the fastest; result[0..4] = block size, order mode (1 = largest instances
+ * first), us per batched iteration, split_eval, iterations per launch. */
 int xg_batch_autotune(xg_batch* b, double* result);
 
 /* Learned predictor in a batch (call before the first xg_batch_advance):
@@ -205,7 +212,8 @@ int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity);
 /* Collected pairs (mode 1), one record per window of an accepted step:
  * features x [XG_PRED_FEAT] bf16 = (x_n - x_{n-1}, x_{n-1} - x_{n-2},
  * x_{n-2} - x_{n-3}, x_poly - x_n, x_cubic - x_poly) / s (XG_PRED_WIN each;
- * s = the window's largest |entry| of the first four + 1e-9), then
+ * s = the window's largest |entry| of the first four + 1e-3 max|x_n| +
+ * 1e-6), then
  * h_{n+1}/h_n, h_n/h_{n-1}, h_{n-1}/h_{n-2}, order, log10(s)/10, 1, zeros;
  * target t [XG_PRED_WIN] = (x_{n+1} - x_poly) / s; mask m = 1 where the
  * window row exists.  Copies up to max_records into the device buffers

@@ -167,6 +167,10 @@ struct Work {
   std::vector<double> staCur, staNext, stoCur, stoNext;
   int limited = 0;
   double time = 0, tstep = 0, tstop = 0, bptol = 1e-18;
+  // transient tolerances: local error (TIA RELTOL/ABSTOL, N_TIA_TIAParams.C)
+  // and Newton weights (NONLIN-TRAN RELTOL/ABSTOL, N_NLS_NLParams.C); the
+  // defaults are the reference's
+  double tia_reltol = 1e-3, tia_abstol = 1e-6, nl_reltol = 1e-2, nl_abstol = 1e-6;
   SparseLU slu;
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
@@ -467,7 +471,7 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
   std::vector<double> xH0(w.x.begin(), w.x.begin() + n), xH1(n, 0.0), xH2(n, 0.0);
   std::vector<double> qH0(w.Q.begin(), w.Q.begin() + n), qH1(n), qH2(n, 0.0);
   std::vector<double> F0(w.F.begin(), w.F.begin() + n), B0(w.B.begin(), w.B.begin() + n);
-  const double reltol = 1e-3, abstol = 1e-6;
+  const double reltol = w.tia_reltol, abstol = w.tia_abstol;
   const double r_min = 0.25, r_max = 0.9, r_hincr = 2.0, r_hincr_test = 2.0, tolAimFac = 0.5;
   const double h0_safety = 2.0, h0_max_factor = 0.005;
   const double hmin = 1e-18;
@@ -508,7 +512,7 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
   };
   initialize(true);
   SolverState ss; ss.dcop = 0; ss.tran = 1;
-  NLP p{1e-2, 1e-6, 0.33, 1e-2, 1e-6, 20, true};
+  NLP p{w.nl_reltol, w.nl_abstol, 0.33, 1e-2, 1e-6, 20, true};
   std::vector<double> xn0(n), xcurr(n), corr(n), errwt(n);
   if (record) { R.t.push_back(0.0); R.x.insert(R.x.end(), xH0.begin(), xH0.end()); }
   while (t < tstop * (1 - 1e-12)) {
@@ -807,6 +811,14 @@ int orc_dcop(OrcHandle* h, double* x, double* sta, double* sto, double* Q, doubl
 }
 
 // Full transient.  stats: [steps, rejects, newton_iters, loads, seconds, status, npoints]
+// Transient tolerances (a tighter run is the accurate reference that the
+// tests compare runs with different Newton paths against).
+int orc_set_tolerances(OrcHandle* h, double tia_reltol, double tia_abstol, double nl_reltol, double nl_abstol) {
+  h->w->tia_reltol = tia_reltol; h->w->tia_abstol = tia_abstol;
+  h->w->nl_reltol = nl_reltol; h->w->nl_abstol = nl_abstol;
+  return 0;
+}
+
 int orc_tran(OrcHandle* h, double tstop, double tstep, double hmax, int record, double t_record_stop, double* stats) {
   Work& w = *h->w;
   h->last = tran(w, tstop, tstep, hmax, record != 0, t_record_stop);

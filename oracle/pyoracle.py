@@ -131,6 +131,10 @@ class Oracle:
         st = lib().orc_dcop(self.h, _p(x), _p(sta), _p(sto), _p(Q), _p(F), _p(B))
         return st, x, sta[: self.nsta], sto[: self.nsto], Q, F, B
 
+    def set_tolerances(self, tia_reltol=1e-3, tia_abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
+        lib().orc_set_tolerances(self.h, C.c_double(tia_reltol), C.c_double(tia_abstol), C.c_double(nl_reltol),
+                                 C.c_double(nl_abstol))
+
     def tran(self, tstop, tstep, hmax, record=True, t_record_stop=None):
         stats = np.zeros(7)
         st = lib().orc_tran(self.h, tstop, tstep, hmax, 1 if record else 0,

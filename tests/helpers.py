@@ -29,3 +29,20 @@ def waveform_error(t_a, x_a, t_b, x_b):
         xb = np.interp(t_a[m], t_b, x_b[:, j])
         err = max(err, float(np.max(np.abs(x_a[m, j] - xb))))
     return err
+
+
+def waveform_error_tube(t_a, x_a, t_b, x_b, dt):
+    """Like waveform_error, but each point of run a may match run b anywhere
+    within +-dt in time (the comparison SPICE regression tools make for
+    oscillators, whose edges shift by a fraction of a step when the Newton
+    path differs within its tolerances)."""
+    t0 = max(t_a[0], t_b[0]) + dt; t1 = min(t_a[-1], t_b[-1]) - dt
+    m = (t_a >= t0) & (t_a <= t1)
+    err = 0.0
+    shifts = np.linspace(-dt, dt, 17)
+    for j in range(x_a.shape[1]):
+        d = np.full(int(m.sum()), np.inf)
+        for s in shifts:
+            d = np.minimum(d, np.abs(x_a[m, j] - np.interp(t_a[m] + s, t_b, x_b[:, j])))
+        err = max(err, float(np.max(d)) if d.size else 0.0)
+    return err

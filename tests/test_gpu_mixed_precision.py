@@ -47,7 +47,7 @@ def test_fp32_refined_solve_matches_fp64(gpu, gates):
     c = C.gate_dag(gates, seed=4)
     o, J, rhs = _tran_system(c, gates, rails=False)
     p = o.pattern()
-    s = X.DirectSolver(o.n, p["rowptr"], p["colind"])
+    s = X.DirectSolver(o.n, p["rowptr"], p["colind"], values=J)   # pivot order steered by the values
     s.set_precision(1, ir_tol=1e-12, ir_max=10)
     for rep in range(2):       # refactor + solve twice (the Newton loop's pattern)
         st, x = s.doSolve(J * (1.0 + 0.01 * rep), rhs)

@@ -204,16 +204,53 @@ def test_batch_uneven_instances_match_individual_runs(gpu):
         assert np.array_equal(t_a, t_b) and np.array_equal(x_a, x_b)
 
 
-def test_iterations_per_launch_do_not_change_results(gpu):
-    """K Newton iterations per launch (state kept in LDS) give bit-identical
-    trajectories to one iteration per launch."""
+def test_launch_forms_do_not_change_results(gpu):
+    """The separate evaluation kernel (one iteration per launch) and the
+    fused kernel with 1 or K Newton iterations per launch (state kept in LDS)
+    give bit-identical trajectories."""
     cs = [C.ring(s, seed=7 + s, mc_seed=7 + s) for s in (9, 23)]
     out = []
-    for k in (1, 4):
+    for split, k in ((1, 1), (0, 1), (0, 4)):
         b = X.Batch(cs, 0.5e-9, 1e-12, 1e-11, record_max=2000)
-        b.config(iters_per_launch=k)
+        b.config(iters_per_launch=k, split_eval=split)
         b.run_to_end(chunk=64)
         n0 = X.host_pattern(cs[0])["n"]; n1 = X.host_pattern(cs[1])["n"]
         out.append((b.points(0, n0), b.points(1, n1)))
-    for a, c in zip(out[0], out[1]):
+    for other in out[1:]:
+        for a, c in zip(out[0], other):
+            assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_graph_replays_match_eager_launches(gpu):
+    """Launches captured in a hipGraph and replayed (with counter reads and
+    synchronisation between replays) give the same trajectory as eager
+    launches."""
+    cs = [C.ring(s, seed=3 + s, mc_seed=3 + s) for s in (13, 29)]
+    res = []
+    for chunk in (0, 16):
+        b = X.Batch(cs, 0.3e-9, 1e-12, 1e-11, record_max=2000)
+        for _ in range(40):
+            b.advance(16, graph_chunk=chunk)
+            b.counters()
+        n0 = X.host_pattern(cs[0])["n"]; n1 = X.host_pattern(cs[1])["n"]
+        res.append((b.points(0, n0), b.points(1, n1), b.counters()["newton_iters"]))
+    assert res[0][2] == res[1][2]
+    for a, c in zip(res[0][:2], res[1][:2]):
         assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
+def test_batch_power_grids_match_oracle(gpu):
+    """Linear RC meshes with PWL sinks at different times: each instance
+    stops at its own source breakpoints and follows the oracle."""
+    cs = [C.power_grid(6, 6, seed=20 + k, pad_pitch=3, n_sinks=4, tstop=1e-9) for k in range(3)]
+    b = X.Batch(cs, 1e-9, 1e-12, 0.0, record_max=4000)
+    cnt = b.run_to_end(chunk=256)
+    assert cnt["done"] == 3 and cnt["failed"] == 0, cnt
+    for i, c in enumerate(cs):
+        o = Oracle(c)
+        r = o.tran(1e-9, 1e-12, 0.0)
+        assert r["status"] == 0
+        t_g, x_g = b.points(i, o.n)
+        assert abs(len(t_g) - len(r["t"])) <= 0.02 * len(r["t"]) + 2, (len(t_g), len(r["t"]))
+        err = waveform_error(t_g, x_g, r["t"], r["x"])
+        assert err <= 1e-3 * 1.8 + 1e-6, err

@@ -1,0 +1,111 @@
+"""Learned Newton predictor: trained on the GPU by the library's own kernels
+on step pairs the batched engine records, served in-batch.
+
+Checks:
+* pair collection: one record per window of every accepted step after the
+  history fills, features and targets finite, the mask covering exactly
+  the instance's unknowns;
+* serving leaves step control and tolerances unchanged: waveforms within
+  RELTOL of the FP64 oracle, accepted step counts within 2 % of the run
+  without the network;
+* what the network gains: Newton iterations per accepted step with and
+  without it (printed; the trained network must not need more iterations
+  than the polynomial predictor).
+"""
+import numpy as np
+import pytest
+import torch
+
+import xyce_amd as X
+from oracle.pyoracle import Oracle
+from tests.helpers import waveform_error, waveform_error_tube
+from xyce_amd import circuits as C
+from xyce_amd import predictor as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _rings(n, seed0):
+    rng = np.random.default_rng(seed0)
+    return [C.ring(int(s), seed=seed0 + i, mc_seed=seed0 + i) for i, s in enumerate(rng.choice([11, 13, 15, 17, 19, 21], n))]
+
+
+def test_pairs_are_collected(gpu):
+    cs = _rings(8, 1)
+    x, t, m, c = P.collect_pairs(cs, 0.5e-9, 1e-12, 1e-11, capacity=1000)
+    assert c["done"] == 8 and c["failed"] == 0
+    # one record per accepted step once 4 points (the operating point and 3 steps) exist
+    assert x.shape[0] == c["pairs"]
+    assert c["pairs"] >= c["steps"] - 8 * 4
+    assert torch.isfinite(x.float()).all() and torch.isfinite(t).all()
+    assert float(m.sum(1).min()) >= 13 and float(m.sum(1).max()) <= 23
+
+
+def _grids(n, seed0):
+    """The power-grid family: 10x10 RC meshes, 4 supply pads, 16 PWL sinks
+    at seeded nodes and times."""
+    return [C.power_grid(10, 10, seed=seed0 + 3 * i, pad_pitch=5, n_sinks=16, tstop=2e-9) for i in range(n)]
+
+
+def _evaluate(net, test_set, tstop, hmax, oscillator=False, check=(0, 3)):
+    """Runs the test set with and without the network; prints iterations per
+    step and waveform errors.  Non-oscillating circuits: both runs within
+    RELTOL*|V| + ABSTOL of the FP64 oracle at the same tolerances.
+    Oscillators: edges move by a fraction of a step whenever the Newton path
+    changes (the oracle at the reference's tolerances is itself ~10 mV from
+    an accurate solution after 2 ns), so both runs are measured against an
+    accurate oracle run (RELTOL 1e-6) in a time tube of one tstep, and the
+    network's run may not be worse than the polynomial one's (+ RELTOL)."""
+    base, b0 = P.run(test_set, tstop, 1e-12, hmax, record_max=40000)
+    learned, b1 = P.run(test_set, tstop, 1e-12, hmax, net=net, record_max=40000)
+    n = len(test_set)
+    assert base["done"] == learned["done"] == n
+    ipb = base["newton_iters"] / base["steps"]
+    ipl = learned["newton_iters"] / learned["steps"]
+    print(f"Newton iterations per accepted step: polynomial {ipb:.4f}, learned {ipl:.4f} "
+          f"({learned['learned_steps']} steps from the network, {learned['learned_newton_failures']} retried); "
+          f"accepted steps {base['steps']} / {learned['steps']}")
+    tol = 1e-3 * 1.8 + 1e-6
+    for i in check:
+        o = Oracle(test_set[i])
+        if oscillator:
+            o.set_tolerances(1e-6, 1e-9, 1e-5, 1e-9)
+        r = o.tran(tstop, 1e-12, hmax)
+        assert r["status"] == 0
+        errs = {}
+        for name, b in (("polynomial", b0), ("learned", b1)):
+            tg, xg = b.points(i, o.n)
+            errs[name] = waveform_error_tube(tg, xg, r["t"], r["x"], 1e-12) if oscillator else \
+                waveform_error(tg, xg, r["t"], r["x"])
+        print(f"instance {i}: waveform error vs {'accurate ' if oscillator else ''}FP64 oracle: "
+              f"polynomial {errs['polynomial']:.2e} V, learned {errs['learned']:.2e} V")
+        if oscillator:
+            assert errs["learned"] <= 1.5 * errs["polynomial"] + tol
+        else:
+            assert errs["polynomial"] <= tol and errs["learned"] <= tol
+    return base, learned, ipb, ipl
+
+
+def test_trained_predictor_on_power_grids(gpu):
+    """The power-grid family: linear, so a proposal within the Newton
+    weights saves the second solve of a step."""
+    x, t, m, c = P.collect_pairs(_grids(192, 100), 2e-9, 1e-12, 0.0, capacity=4000)
+    net = P.new_net(max_rows=2048, seed=5)
+    hist = P.train(net, x, t, m, epochs=4, batch=1024, lr=1e-3)
+    print(f"{x.shape[0]} pairs; loss {hist['loss']} (polynomial predictor {hist['poly_loss']:.3e})")
+    assert hist["loss"][-1] < hist["poly_loss"]
+    base, learned, ipb, ipl = _evaluate(net, _grids(16, 5000), 2e-9, 0.0)
+    assert abs(learned["steps"] - base["steps"]) <= 0.02 * base["steps"] + 16
+    assert ipl <= ipb * 1.01
+
+
+def test_trained_predictor_on_rings(gpu):
+    """Ring oscillators (nonlinear)."""
+    x, t, m, c = P.collect_pairs(_rings(256, 100), 2e-9, 1e-12, 1e-11, capacity=4000)
+    net = P.new_net(max_rows=2048, seed=5)
+    hist = P.train(net, x, t, m, epochs=3, batch=1024, lr=1e-3)
+    print(f"{x.shape[0]} pairs; loss {hist['loss']} (polynomial predictor {hist['poly_loss']:.3e})")
+    assert hist["loss"][-1] < hist["poly_loss"]
+    base, learned, ipb, ipl = _evaluate(net, _rings(16, 900), 2e-9, 1e-11, oscillator=True, check=(0, 7))
+    assert abs(learned["steps"] - base["steps"]) <= 0.02 * base["steps"] + 16
+    assert ipl <= ipb * 1.01

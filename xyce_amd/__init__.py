@@ -370,8 +370,11 @@ class Batch:
         return dict(pairs=int(c[0]), learned_steps=int(c[1]), learned_newton_failures=int(c[2]),
                     windows=int(c[3]), mode=int(c[4]))
 
-    def config(self, block=0, iters_per_launch=0):
-        _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch)), "xg_batch_config")
+    def config(self, block=0, iters_per_launch=0, split_eval=-1):
+        """block: workgroup size (0 keep); split_eval 1: separate evaluation kernel, one iteration per
+        launch; 0: evaluation fused, iters_per_launch iterations per launch; -1 keep."""
+        _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch), C.c_int(split_eval)),
+               "xg_batch_config")
 
     PHASES = ("x_load", "device_eval", "vector_gather_residual", "norms", "newton_decision", "jacobian_gather",
               "lu_refactor", "pivot_check", "triangular_solves", "update", "step_control")
@@ -387,9 +390,10 @@ class Batch:
         return out
 
     def autotune(self):
-        r = np.zeros(3)
+        r = np.zeros(5)
         _check(lib().xg_batch_autotune(self.h, _p(r)), "xg_batch_autotune")
-        return dict(block=int(r[0]), order_mode=int(r[1]), us_per_launch=float(r[2]))
+        return dict(block=int(r[0]), order_mode=int(r[1]), us_per_iteration=float(r[2]), split_eval=int(r[3]),
+                    iters_per_launch=int(r[4]))
 
 
 class MLP:

@@ -82,7 +82,11 @@ struct InstState {
   double hP[4];
   int n_acc, feat_valid, pred_skip, used_learned;
   long long n_pairs, n_learned, n_learned_rejects;
-  double pad[27];
+  // Newton iterations per step started from each predictor (decaying
+  // averages): the selector that lets Newton convergence judge the network
+  double it_learned, st_learned, it_poly, st_poly;
+  long long n_choice;
+  double pad[22];
 };
 
 struct InstDesc {
@@ -120,11 +124,16 @@ struct BatchArgs {
   double tstop, tstep, hmax;
   // recording
   int rec_max; double* rec_t; double* rec_x;
-  int max_n, max_nlu, max_blob;
+  int max_n, max_nlu, max_blob;   // over all instances (max_n: stride of the recorded points)
+  int lds_n, lds_nlu;              // LDS layout of this launch's instances (their largest class)
   int n_kcls;                  // entries of the MOSFET class table
   int iters;                   // Newton iterations per launch
   unsigned long long* prof;    // per-phase wall-clock ticks of one instance (null = off)
   double* dense;           // per-instance n*n scratch
+  // separate device-evaluation kernel: owning instance of every MOSFET and
+  // the per-instance "some device limited its voltages" flag it raises
+  const int* dev_inst;
+  int* lim;
   // learned predictor (mode 0 off, 1 collect training pairs, 2 serve)
   int pred_mode;
   int pair_cap;            // training pairs per instance
@@ -217,7 +226,8 @@ __device__ __forceinline__ unsigned short d_f2bf(float f) {
 //   (x_n - x_{n-1}, x_{n-1} - x_{n-2}, x_{n-2} - x_{n-3}, x_poly - x_n,
 //    x_cubic - x_poly) / s,
 // x_cubic the cubic through the 4 points at t_{n+1}, s the window's largest
-// |entry| of the first four (+1e-9), then the step-size ratios.  The
+// |entry| of the first four plus RELTOL max|x_n| + ABSTOL, then the
+// step-size ratios.  The
 // network's output y gives x = x_poly + s y.
 __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S, double* red) {
   const int n = C.n, off = I.vec_off, tid = threadIdx.x, nth = blockDim.x;
@@ -235,15 +245,19 @@ __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const Class
   }
   for (int w = 0; w < nw; ++w) {
     const int r0 = w * XG_PRED_WIN, r1 = min(n, r0 + XG_PRED_WIN);
-    double mx = 0.0;
+    double mx = 0.0, mv = 0.0;
     for (int r = r0 + tid; r < r1; r += nth) {
       const double x0 = A.xP[off + r], x1 = A.xP[TN + off + r], x2 = A.xP[2 * TN + off + r],
                    x3 = A.xP[3 * TN + off + r];
       const double xp = A.xn0[off + r];
       mx = fmax(mx, fmax(fmax(fabs(x0 - x1), fabs(x1 - x2)), fmax(fabs(x2 - x3), fabs(xp - x0))));
+      mv = fmax(mv, fabs(x0));
     }
     mx = block_max(mx, red);
-    const double sc = mx + 1e-9;
+    mv = block_max(mv, red);
+    // scale: the window's largest change, floored at the local-error weight
+    // (RELTOL |x| + ABSTOL), so quiet windows are not blown up
+    const double sc = mx + kRelTol * mv + kAbsTol;
     unsigned short* f = A.feat + (size_t)(I.win_off + w) * XG_PRED_FEAT;
     for (int j = tid; j < XG_PRED_WIN; j += nth) {
       const int r = r0 + j;
@@ -271,6 +285,26 @@ __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const Class
     }
     if (tid == 0) A.wscale[I.win_off + w] = sc;
   }
+}
+
+// Per-instance predictor selector: decaying averages of the Newton
+// iterations of steps started from the network's proposal and from the
+// polynomial predictor.
+__device__ __forceinline__ void pred_score(InstState& S, bool learned, int iters) {
+  const double k = 0.97;
+  if (learned) { S.it_learned = k * S.it_learned + iters; S.st_learned = k * S.st_learned + 1.0; }
+  else { S.it_poly = k * S.it_poly + iters; S.st_poly = k * S.st_poly + 1.0; }
+}
+// Use the network for this step?  Both predictors are tried while their
+// averages are young and every 16th choice goes to the one not preferred;
+// otherwise the one with fewer iterations per step wins.
+__device__ __forceinline__ bool pred_choose(InstState& S) {
+  const long long c = S.n_choice++;
+  bool pick;
+  if (S.st_learned < 4.0 || S.st_poly < 4.0) pick = (c & 1) == 0;
+  else pick = S.it_learned / S.st_learned <= S.it_poly / S.st_poly;
+  if (c % 16 == 15) pick = !pick;
+  return pick;
 }
 
 // Accepted step: the training pair of this step (collect mode) and the
@@ -302,6 +336,7 @@ __device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDe
   }
   __syncthreads();
   if (tid == 0) {
+    if (A.pred_mode == 2 && S.feat_valid) pred_score(S, S.used_learned != 0, S.newton_iter);
     if (A.pred_mode == 1 && S.feat_valid && S.n_pairs < A.pair_cap) S.n_pairs++;
     S.hP[3] = S.hP[2]; S.hP[2] = S.hP[1]; S.hP[1] = S.hP[0]; S.hP[0] = S.h;
     S.n_acc++;
@@ -357,7 +392,7 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
       S.feat_valid = 1;
       // serve: wait for the network's proposal (the apply kernel resumes the
       // instance); after a Newton failure with it, retry with the polynomial one
-      if (A.pred_mode == 2 && !S.pred_skip) S.mode = 3;
+      if (A.pred_mode == 2 && !S.pred_skip && pred_choose(S)) S.mode = 3;
     }
     __syncthreads();
   }
@@ -398,17 +433,17 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
       const double v = fabs(D[(size_t)i * n + k]);
       if (v > best) { best = v; bi = i; }
     }
-    red[tid] = best; sh_i[tid] = bi;
-    __syncthreads();
-    for (int st = nth / 2; st > 0; st >>= 1) {
-      if (tid < st) {
-        const double o = red[tid + st]; const int oi = sh_i[tid + st];
-        if (o > red[tid] || (o == red[tid] && oi < sh_i[tid])) { red[tid] = o; sh_i[tid] = oi; }
-      }
-      __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(best, o, 64);
+      const int oi = __shfl_xor(bi, o, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
     }
-    const int p = sh_i[0];
-    const double pmax = red[0];
+    if ((tid & 63) == 0) { red[tid >> 6] = best; sh_i[tid >> 6] = bi; }
+    __syncthreads();
+    double pmax = red[0];
+    int p = sh_i[0];
+    for (int w = 1; w < (nth >> 6); ++w)
+      if (red[w] > pmax || (red[w] == pmax && sh_i[w] < p)) { pmax = red[w]; p = sh_i[w]; }
     __syncthreads();
     if (!(pmax > 0.0) || !isfinite(pmax)) { singular = 1; break; }
     if (p != k) {
@@ -443,7 +478,8 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
 
 // One Newton iteration of one instance (the body of every launch; a launch
 // runs A.iters of them back to back with the instance state kept in LDS).
-__device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S,
+template <bool FUSED>
+__device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I, const ClassDesc& C, InstState& S,
                                 double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
                                 double* y, double* lu, const unsigned short* prog, const double* kcls,
                                 double* srcv) {
@@ -469,6 +505,11 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
   __syncthreads();
   for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
   if (tid == 0) xs[n] = 0.0;
+  if (!FUSED && tid == 0) {
+    // the evaluation kernel ran just before this launch
+    sh_int[0] = A.lim[inst];
+    A.lim[inst] = 0;
+  }
   {
     // source values at this iteration's time, staged in LDS (read after the
     // barrier that ends the device evaluation)
@@ -483,7 +524,7 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
   XG_PHASE(0);
   // ---- 2. device evaluation (one thread per MOSFET); each term goes to its
   //         slot-ordered position in the instance's term array
-  {
+  if (FUSED) {
     MosFlags fl;
     fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0; fl.newton_iter = S.newton_iter;
     const int* lid = ci + C.o_lid;
@@ -938,6 +979,7 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
       S.pred_skip = 1;
       S.n_learned_rejects++;
     }
+    if (A.pred_mode == 2 && S.feat_valid) pred_score(S, S.used_learned != 0, S.newton_iter);
     S.used_learned = 0;
     const int ord = S.order;
     if (ord >= 1) S.psi0 = S.psi1;
@@ -962,25 +1004,26 @@ __device__ void batch_iteration(const BatchArgs& A, const InstDesc& I, const Cla
 #undef XG_PHASE
 }
 
+template <bool FUSED>
 __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   if ((int)blockIdx.x >= A.n_inst) return;
   const int inst = A.order[blockIdx.x];
   const InstDesc I = A.inst[inst];
   const ClassDesc C = A.cls[I.cls];
   __shared__ InstState S;
-  __shared__ double red[512];
-  __shared__ int red_i[512];
+  __shared__ double red[16];
+  __shared__ int red_i[16];
   __shared__ int sh_int[8];
   __shared__ double sh_d[8];
-  __shared__ double kc_s[16 * KSTRIDE];
+  __shared__ double kc_s[FUSED ? 16 * KSTRIDE : 1];
   __shared__ double srcv[kMaxSrc];
   extern __shared__ double dyn[];
   double* xs = dyn;                         // n + 1
-  double* rhs = xs + (A.max_n + 1);         // n
-  double* y = rhs + A.max_n;                // n
-  double* lu = y + A.max_n;                 // nlu
+  double* rhs = xs + (A.lds_n + 1);         // n
+  double* y = rhs + A.lds_n;                // n
+  double* lu = y + A.lds_n;                 // nlu
   // class program after the doubles, on a 16-byte boundary (uint4 copies)
-  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.max_n + 1) + A.max_nlu + 1) & ~1));
+  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.lds_n + 1) + A.lds_nlu + 1) & ~1));
   const int tid = threadIdx.x, nth = blockDim.x;
   if (tid == 0) S = A.st[inst];
   __syncthreads();
@@ -993,17 +1036,53 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
   }
   const double* kcls = A.kcls;
-  if (A.n_kcls <= 16) {
+  if (FUSED && A.n_kcls <= 16) {
     for (int i = tid; i < A.n_kcls * KSTRIDE; i += nth) kc_s[i] = A.kcls[i];
     kcls = kc_s;
   }
   __syncthreads();
-  for (int it = 0; it < A.iters; ++it) {
-    batch_iteration(A, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv);
+  const int iters = FUSED ? A.iters : 1;
+  for (int it = 0; it < iters; ++it) {
+    batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv);
     __syncthreads();
     if (S.mode >= 2 || S.mode < 0) break;
   }
   if (tid == 0) A.st[inst] = S;
+}
+
+// Device evaluation of every MOSFET of every instance that is iterating
+// (DC or transient), one thread per device over the structure-of-arrays
+// state: MOSFET1 updateIntermediateVars + the vector and matrix terms,
+// written to their slots of the instance's term array.  Runs before the
+// per-instance kernel of the same launch.
+__global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= A.nt_mos) return;
+  const int inst = A.dev_inst[g];
+  const int mode = A.st[inst].mode;
+  if (mode != 0 && mode != 1) return;
+  const int newton_iter = A.st[inst].newton_iter;
+  const InstDesc& I = A.inst[inst];
+  const ClassDesc& C = A.cls[I.cls];
+  const int d = g - I.dev_off;
+  const int n = C.n, off = I.vec_off;
+  const int NT = A.nt_mos;
+  const bool dc = mode == 0;
+  MosFlags fl;
+  fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0; fl.newton_iter = newton_iter;
+  const int* lid = A.ci + C.o_lid + 6 * d;
+  const int* tp = A.ci + C.o_tpos + (size_t)d * R_COUNT;
+  double* T = A.terms + I.term_off;
+  const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
+  double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+  for (int j = 0; j < 6; ++j) volt[j] = lid[j] < n ? A.x[off + lid[j]] : 0.0;
+  for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
+  for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
+  auto emit = [&](int plane, double v) { const int p = tp[plane]; if (p >= 0) T[p] = v; };
+  const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
+  for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
+  for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
+  if (lim) atomicOr(&A.lim[inst], 1);
 }
 
 // Resumes the instances waiting for the network's proposal (mode 3):
@@ -1114,7 +1193,15 @@ struct xg_batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 256;
+  int split = 1;                  // 1: separate device-evaluation kernel + per-instance kernel
   size_t shmem = 0;
+  // launch buckets: instances grouped by the LDS their class needs, one
+  // per-instance launch each with its own dynamic LDS size
+  struct Bucket { int start, count, n, nlu; size_t shmem; };
+  std::vector<Bucket> buckets;
+  std::vector<size_t> cls_lds;    // per class: dynamic LDS bytes
+  std::vector<int> inst_cls;
+  DArr<int> d_dev_inst, d_lim;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   int graph_len = 0;
@@ -1129,11 +1216,36 @@ struct xg_batch {
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
   }
+  static size_t lds_bytes(int n, int nlu, int blob) {
+    return sizeof(double) * (((size_t)3 * (n + 1) + nlu + 1) & ~(size_t)1) + sizeof(unsigned short) * (size_t)blob;
+  }
+  // mode 0: given order, one launch sized for the largest class; mode 1:
+  // instances by LDS need (largest first), up to 4 launches of equal size,
+  // each sized for its own largest class
   void set_order(int mode) {
     std::vector<int> ord(n_inst);
     std::iota(ord.begin(), ord.end(), 0);
-    if (mode == 1)  // largest instances first (longest-processing-time order)
-      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cls_n[a] > cls_n[b]; });
+    buckets.clear();
+    if (mode == 1) {
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+        return cls_lds[inst_cls[a]] > cls_lds[inst_cls[b]];
+      });
+      const int nb = std::min(4, n_inst);
+      for (int k = 0; k < nb; ++k) {
+        const int s0 = (int)((long long)n_inst * k / nb), s1 = (int)((long long)n_inst * (k + 1) / nb);
+        if (s1 <= s0) continue;
+        Bucket bk{s0, s1 - s0, 0, 0, 0};
+        int blob = 0;
+        for (int j = s0; j < s1; ++j) {
+          const ClassDesc& C = classes[inst_cls[ord[j]]];
+          bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu); blob = std::max(blob, C.blob_len);
+        }
+        bk.shmem = lds_bytes(bk.n, bk.nlu, blob);
+        buckets.push_back(bk);
+      }
+    } else {
+      buckets.push_back(Bucket{0, n_inst, args.max_n, args.max_nlu, shmem});
+    }
     (void)hipMemcpy(d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice);
     order_mode = mode;
   }
@@ -1143,7 +1255,16 @@ struct xg_batch {
     graph_len = 0;
   }
   int launch() {
-    hipLaunchKernelGGL(k_batch_step, dim3(n_inst), dim3(block), shmem, stream, args);
+    if (split) {
+      if (args.nt_mos > 0)
+        hipLaunchKernelGGL(k_batch_eval, dim3((args.nt_mos + 255) / 256), dim3(256), 0, stream, args);
+    }
+    for (const Bucket& bk : buckets) {
+      BatchArgs a = args;
+      a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
+      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(block), bk.shmem, stream, a);
+      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(block), bk.shmem, stream, a);
+    }
     if (pred_mode == 2) {
       const int st = mlp_forward_on_stream(net, pred_rows, d_feat.p, d_pred_out.p, stream);
       if (st != XG_OK) return st;
@@ -1165,7 +1286,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   std::vector<int> ci;                              // concatenated class arrays
   std::map<std::vector<int>, int> cls_of;           // structure key -> class
   std::vector<double> kcls, tvbi, beta, tvto, linF, linQ, wave_data;
-  std::vector<int> mcls, src_wave, wave_type, wave_off{0};
+  std::vector<int> mcls, dev_inst, src_wave, wave_type, wave_off{0};
   std::vector<unsigned long long> mask;
   std::map<std::vector<double>, int> kindex;
   int dev_off = 0, vec_off = 0, nnz_off = 0, src_off = 0;
@@ -1413,6 +1534,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       if (kt == kindex.end()) { kid = (int)kindex.size(); kindex.emplace(kk, kid); kcls.insert(kcls.end(), kk.begin(), kk.end()); }
       else kid = kt->second;
       mcls.push_back(kid);
+      dev_inst.push_back(i);
     }
     tvbi.insert(tvbi.end(), hc.mos_tvbi.begin(), hc.mos_tvbi.end());
     beta.insert(beta.end(), hc.mos_beta.begin(), hc.mos_beta.end());
@@ -1445,6 +1567,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   if (hipStreamCreate(&b->stream) != hipSuccess) return XG_E_HIP;
   HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1));
   HIPCHK(b->d_ci.up(ci)); HIPCHK(b->d_cls.up(b->classes)); HIPCHK(b->d_inst.up(b->insts));
+  HIPCHK(b->d_dev_inst.up(dev_inst)); HIPCHK(b->d_lim.zeros(n_inst));
   HIPCHK(b->d_kcls.up(kcls)); HIPCHK(b->d_mcls.up(mcls)); HIPCHK(b->d_tvbi.up(tvbi)); HIPCHK(b->d_beta.up(beta));
   HIPCHK(b->d_tvto.up(tvto)); HIPCHK(b->d_mask.up(mask));
   HIPCHK(b->d_sto_cur.zeros(6 * NT)); HIPCHK(b->d_sto_nxt.zeros(6 * NT));
@@ -1479,11 +1602,14 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.bp = b->d_bp.p; A.tstop = tstop; A.tstep = tstep; A.hmax = hmax;
   A.rec_max = std::max(record_max, 0); A.rec_t = b->d_rec_t.p; A.rec_x = b->d_rec_x.p;
   A.max_n = max_n; A.max_nlu = max_nlu; A.max_blob = max_blob;
+  A.lds_n = max_n; A.lds_nlu = max_nlu;
   A.n_kcls = (int)(kcls.size() / KSTRIDE); A.iters = 1; A.prof = nullptr;
   A.dense = b->d_dense.p;
   A.pred_mode = 0;
-  b->shmem = sizeof(double) * (((size_t)3 * (max_n + 1) + max_nlu + 1) & ~(size_t)1) +
-             sizeof(unsigned short) * (size_t)max_blob;
+  A.dev_inst = b->d_dev_inst.p; A.lim = b->d_lim.p;
+  b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
+  for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu, C.blob_len));
+  for (const InstDesc& D : b->insts) b->inst_cls.push_back(D.cls);
   if (b->shmem > 150 * 1024) return XG_E_ARG;
   b->set_order(1);
   *out = b.release();
@@ -1666,40 +1792,51 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
 
 int xg_batch_autotune(xg_batch* b, double* result) {
   if (!b) return XG_E_ARG;
-  // Candidates: workgroup size x instance order.  Each candidate runs a few
-  // real launches (they advance the simulation; every launch is a valid
-  // Newton iteration) and the fastest per-launch time wins.
+  // Candidates: kernel form (separate evaluation kernel + one iteration per
+  // launch, or fused with K iterations per launch) x workgroup size x
+  // instance order.  Each candidate runs a few real launches (they advance
+  // the simulation; every launch does valid Newton iterations) and the
+  // fastest time per batched iteration wins.
   const int blocks[3] = {128, 256, 512};
+  const int fused_iters = b->args.iters > 1 ? b->args.iters : 8;
   double best = 1e30;
-  int best_b = b->block, best_o = b->order_mode;
-  for (int o = 0; o < 2; ++o) {
-    b->set_order(o);
-    for (int bs : blocks) {
-      b->block = bs;
-      b->drop_graph();
-      int st = xg_batch_advance(b, 2, 0);
-      if (st != XG_OK) return st;
-      st = xg_batch_advance(b, 8, 0);
-      if (st != XG_OK) return st;
-      const double us = b->last_seconds / 8 * 1e6;
-      if (us < best) { best = us; best_b = bs; best_o = o; }
+  int best_b = b->block, best_o = b->order_mode, best_s = b->split, best_k = b->args.iters;
+  for (int sp = 1; sp >= 0; --sp) {
+    for (int o = 0; o < 2; ++o) {
+      b->set_order(o);
+      for (int bs : blocks) {
+        b->block = bs;
+        b->split = sp;
+        b->args.iters = sp ? 1 : fused_iters;
+        b->drop_graph();
+        int st = xg_batch_advance(b, 2, 0);
+        if (st != XG_OK) return st;
+        st = xg_batch_advance(b, 8, 0);
+        if (st != XG_OK) return st;
+        const double us = b->last_seconds / 8 * 1e6 / b->args.iters;
+        if (us < best) { best = us; best_b = bs; best_o = o; best_s = sp; best_k = b->args.iters; }
+      }
     }
   }
   b->block = best_b;
+  b->split = best_s;
+  b->args.iters = best_k;
   b->set_order(best_o);
   b->drop_graph();
   b->tune_us = best;
-  if (result) { result[0] = best_b; result[1] = best_o; result[2] = best; }
+  if (result) { result[0] = best_b; result[1] = best_o; result[2] = best; result[3] = best_s; result[4] = best_k; }
   return XG_OK;
 }
 
-int xg_batch_config(xg_batch* b, int block, int iters_per_launch) {
-  if (!b || iters_per_launch < 0) return XG_E_ARG;
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval) {
+  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 1) return XG_E_ARG;
   if (block) {
     if (block != 64 && block != 128 && block != 256 && block != 512) return XG_E_ARG;
     b->block = block;
   }
+  if (split_eval >= 0) b->split = split_eval;
   if (iters_per_launch) b->args.iters = iters_per_launch;
+  if (b->split) b->args.iters = 1;
   b->drop_graph();
   return XG_OK;
 }

@@ -772,47 +772,51 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
   lu->have64 = false;
   int bad = 0, st;
+  bool fp32_failed = false;
   if (lu->mode == 1) {
     st = lu->factor_device<float>(lu->lu32.p);
     if (st != XG_OK) return st;
     st = lu->check_pivots<float>(lu->lu32.p, &bad);
     if (st != XG_OK) return st;
     if (!bad) return XG_OK;
-    lu->stats[5] += 1;    // FP32 pivots fail the test: use the FP64 path for these values
+    fp32_failed = true;
   }
   st = lu->factor_device<double>(lu->lu.p);
   if (st != XG_OK) return st;
   st = lu->check_pivots<double>(lu->lu.p, &bad);
   if (st != XG_OK) return st;
-  if (bad) {
-    // re-pivot: analyse again steering the transversal by the values, then refactor
-    LuSymbolic s2;
-    try {
-      if (lu_analyse(lu->s.n, lu->a_rowptr.data(), lu->a_colind.data(), vals, s2, true) != XG_OK) return bad;
-    } catch (...) {
-      return bad;
-    }
-    lu->s = std::move(s2);
-    st = lu->upload();
-    if (st != XG_OK) return st;
-    HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
-    st = lu->factor_device<double>(lu->lu.p);
-    if (st != XG_OK) return st;
-    st = lu->check_pivots<double>(lu->lu.p, &bad);
-    if (st != XG_OK) return st;
-    if (bad) return bad;
+  if (!bad) {
+    // the FP32 pivots failed where the FP64 ones pass: solve with FP64 factors
+    if (fp32_failed) lu->stats[5] += 1;
+    lu->have64 = true;
+    return XG_OK;
   }
-  lu->have64 = true;
+  // the pivot order itself fails the threshold test (KLU would re-pivot):
+  // analyse again steering the transversal by the values, then refactor
+  LuSymbolic s2;
+  try {
+    if (lu_analyse(lu->s.n, lu->a_rowptr.data(), lu->a_colind.data(), vals, s2, true) != XG_OK) return bad;
+  } catch (...) {
+    return bad;
+  }
+  lu->s = std::move(s2);
+  st = lu->upload();
+  if (st != XG_OK) return st;
+  HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
   if (lu->mode == 1) {
-    // FP32 factors with the new pivot order (the refinement falls back to the
-    // FP64 factors computed above if they do not converge)
     st = lu->factor_device<float>(lu->lu32.p);
     if (st != XG_OK) return st;
-    int bad32 = 0;
-    st = lu->check_pivots<float>(lu->lu32.p, &bad32);
+    st = lu->check_pivots<float>(lu->lu32.p, &bad);
     if (st != XG_OK) return st;
-    if (bad32) lu->mode = 2;   // this pattern keeps FP32 off until set again
+    if (!bad) return XG_OK;
+    lu->stats[5] += 1;
   }
+  st = lu->factor_device<double>(lu->lu.p);
+  if (st != XG_OK) return st;
+  st = lu->check_pivots<double>(lu->lu.p, &bad);
+  if (st != XG_OK) return st;
+  if (bad) return bad;
+  lu->have64 = true;
   return XG_OK;
 }
 
