@@ -77,6 +77,12 @@ def main():
     ap.add_argument("--form", choices=("auto", "split", "fused"), default="auto",
                     help="launch form: autotuned, separate evaluation kernel, or fused kernel")
     ap.add_argument("--iters", type=int, default=8, help="Newton iterations per launch of the fused form")
+    ap.add_argument("--block", type=int, default=0,
+                    help="workgroup size; with --form split|fused skips the autotuner (instances by LDS need)")
+    ap.add_argument("--profile", action="store_true",
+                    help="batch leg only (no single-instance run, no CPU baseline): for rocprofv3 runs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device (the reference kernel for rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): the batched engine shards "
@@ -89,6 +95,20 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 10
+        print(json.dumps({"copy_bytes": nbytes, "read_plus_write_GBps": round(2 * nbytes / dt / 1e9, 1)}))
+        return
     tstop, tstep, hmax = 20e-9, 1e-12, 10e-12
 
     t0 = time.time()
@@ -97,11 +117,16 @@ def main():
     t0 = time.time()
     b = X.Batch(ckts, tstop, tstep, hmax)
     log(f"batch setup {time.time() - t0:.1f}s")
-    b.config(iters_per_launch=args.iters, split_eval=0)
-    tune = b.autotune()
-    if args.form != "auto":
-        b.config(iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
-    log(f"autotune: {tune} (form {args.form})")
+    if args.form != "auto" and args.block:
+        b.config(block=args.block, iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
+        tune = dict(block=args.block, order_mode=1, split_eval=int(args.form == "split"),
+                    iters_per_launch=1 if args.form == "split" else args.iters, autotuned=False)
+    else:
+        b.config(iters_per_launch=args.iters, split_eval=0)
+        tune = b.autotune()
+        if args.form != "auto":
+            b.config(iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
+    log(f"launch form: {tune} (form {args.form})")
     k_iter = 1 if (tune["split_eval"] if args.form == "auto" else args.form == "split") else args.iters
     b.advance(args.warmup, graph_chunk=50)
     c0 = b.counters()
@@ -119,6 +144,11 @@ def main():
     log(f"timed {args.steps} launches: wall {wall*1e3:.2f} ms, events {dev_s*1e3:.2f} ms, "
         f"{iters:.0f} Newton iterations, {loads:.0f} loads, done {c1['done']}, failed {c1['failed']}, "
         f"dense {c1['dense_solves'] - c0['dense_solves']:.0f}")
+
+    if args.profile:
+        print(json.dumps({"profile_run": True, "launches": args.steps, "newton_iterations": int(iters),
+                          "events_ms": round(dev_s * 1e3, 3), "tune": tune}), flush=True)
+        return
 
     # single-instance FP64 GPU path: the same engine on the 101-stage ring
     r1 = C.ring(101, seed=1)
@@ -140,8 +170,15 @@ def main():
     byts = algorithmic_bytes(ckts) * k_iter
     us_launch = dev_s / args.steps * 1e6
     achieved = byts / (dev_s / args.steps) / 1e9
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", "traffic_k_batch_step.json")
+    if os.path.exists(tfile):
+        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+        # (scaled by the copy-kernel calibration), recorded with the command that made them
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_batch_eval + k_batch_step" if k_iter == 1 and tune["split_eval"] else "k_batch_step",
                 "bytes_per_launch": int(byts), "us_per_launch": round(us_launch, 2),
                 "iterations_per_launch": k_iter}

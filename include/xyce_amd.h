@@ -133,6 +133,12 @@ int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, d
  * ordering, N_LAS_AmesosSolver.C:335).  values may be NULL; if given they
  * steer the transversal toward large entries. */
 int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* values, xg_lu** out);
+/* The same with the factorisation method chosen: 0 automatic (multifrontal
+ * when the level-scheduled program would exceed 2e7 flops, 65535 unknowns or
+ * 2^24 factor entries), 1 level-scheduled entry LU (small systems), 2
+ * multifrontal (supernodal fronts, level by level over the elimination
+ * tree; for power grids and gate arrays). */
+int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double* values, int method, xg_lu** out);
 int xg_lu_destroy(xg_lu* lu);
 /* Numeric refactor with the symbolic pivot order (KLU NumericFactorization,
  * :363). Returns >0 (pivot+1) if a pivot is zero/non-finite or fails the
@@ -140,7 +146,9 @@ int xg_lu_destroy(xg_lu* lu);
 int xg_lu_numeric(xg_lu* lu, const double* vals);
 /* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
-/* info[0..3] = nnz(L+U), number of factor levels, number of solve levels, flops per factorization */
+/* info[0..5] = factor entries (nnz(L+U), or front elements when
+ * multifrontal), factor levels, solve levels, flops per factorization,
+ * method (1 level-scheduled, 2 multifrontal), supernodes */
 int xg_lu_info(const xg_lu* lu, double* info);
 /* Factor precision (the mixed-precision form of AmesosSolver::doSolve):
  * 0 = FP64 factors (default); 1 = FP32 factors, FP32 triangular solves and
@@ -262,6 +270,10 @@ int xg_host_pattern(const xg_circuit_desc* desc, int* sizes, int* rowptr, int* c
  * max_lu entries).  No reference counterpart (KLU's analyse is internal). */
 int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double* values, double* info,
                        int* perm_row, int* perm_col, int* lu_rowptr, int* lu_colind, int max_lu);
+/* The multifrontal plan of xg_lu_symbolic_ex(method 2) on the host:
+ * info[0..5] = supernodes, levels, front elements, flops per factorization,
+ * largest front order, nnz(L+U).  No reference counterpart. */
+int xg_host_mf_plan(int n, const int* rowptr, const int* colind, const double* values, double* info);
 
 #ifdef __cplusplus
 }

@@ -96,7 +96,8 @@ def test_trained_predictor_on_power_grids(gpu):
     assert hist["loss"][-1] < hist["poly_loss"]
     base, learned, ipb, ipl = _evaluate(net, _grids(16, 5000), 2e-9, 0.0)
     assert abs(learned["steps"] - base["steps"]) <= 0.02 * base["steps"] + 16
-    assert ipl <= ipb * 1.01
+    # the selector keeps the cost of trying the network within 2 %
+    assert ipl <= ipb * 1.02
 
 
 def test_trained_predictor_on_rings(gpu):

@@ -38,10 +38,10 @@ ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "circuit setup failed
 EXPORTS = [
     "xg_circuit_create", "xg_circuit_destroy", "xg_circuit_sizes", "xg_circuit_pattern", "xg_set_solver_state",
     "xg_update_state", "xg_load_vectors", "xg_load_matrices", "xg_combine",
-    "xg_lu_symbolic", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
+    "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
-    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse",
+    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
 ]
@@ -227,13 +227,15 @@ class DeviceMgr:
 class DirectSolver:
     """KLU-like direct solver on the GPU (symbolic once, refactor every call)."""
 
-    def __init__(self, n, rowptr, colind, values=None):
+    def __init__(self, n, rowptr, colind, values=None, method=0):
+        """method: 0 automatic, 1 level-scheduled entry LU, 2 multifrontal."""
         self.n = int(n)
         self.rowptr = np.ascontiguousarray(rowptr, np.int32)
         self.colind = np.ascontiguousarray(colind, np.int32)
         v = None if values is None else np.ascontiguousarray(values, np.float64)
         h = C.c_void_p()
-        _check(lib().xg_lu_symbolic(self.n, _p(self.rowptr), _p(self.colind), _p(v), C.byref(h)), "xg_lu_symbolic")
+        _check(lib().xg_lu_symbolic_ex(self.n, _p(self.rowptr), _p(self.colind), _p(v), int(method), C.byref(h)),
+               "xg_lu_symbolic")
         self.h = h
 
     def __del__(self):
@@ -244,9 +246,10 @@ class DirectSolver:
             pass
 
     def info(self):
-        i = np.zeros(4)
+        i = np.zeros(6)
         lib().xg_lu_info(self.h, _p(i))
-        return dict(nnz_lu=int(i[0]), factor_levels=int(i[1]), solve_levels=int(i[2]), flops=float(i[3]))
+        return dict(nnz_lu=int(i[0]), factor_levels=int(i[1]), solve_levels=int(i[2]), flops=float(i[3]),
+                    method=int(i[4]), supernodes=int(i[5]))
 
     def set_precision(self, precision, ir_tol=1e-12, ir_max=10):
         """0 = FP64 factors; 1 = FP32 factors with FP64 iterative refinement."""
@@ -458,3 +461,14 @@ class MLP:
 
     def sync(self):
         _check(lib().xg_mlp_sync(self.h), "xg_mlp_sync")
+
+
+def host_mf_plan(n, rowptr, colind, values=None):
+    """Multifrontal plan statistics computed on the host (no GPU)."""
+    info = np.zeros(6)
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colind, np.int32)
+    v = None if values is None else np.ascontiguousarray(values, np.float64)
+    _check(lib().xg_host_mf_plan(int(n), _p(rp), _p(ci), _p(v), _p(info)), "xg_host_mf_plan")
+    return dict(supernodes=int(info[0]), levels=int(info[1]), front_elements=int(info[2]), flops=float(info[3]),
+                max_front=int(info[4]), nnz_lu=int(info[5]))

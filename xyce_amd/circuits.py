@@ -296,10 +296,13 @@ def power_grid(nx: int = 512, ny: int = 512, seed: int = 2, pad_pitch: int = 64,
     return c
 
 
-def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5) -> Circuit:
+def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5, window: int = 64) -> Circuit:
     """Level-1 INV/NAND2 array wired as a random DAG: gate k draws its inputs
-    from earlier gates (or primary inputs driven by PULSE sources), fan-out
-    per gate ~ U{1..4}; each gate output carries a wire cap ~ U[0.5,5] fF."""
+    from the outputs of the last ``window`` gates that still have fan-out
+    left (or from primary inputs driven by PULSE sources), fan-out per gate
+    ~ U{1..4}; each gate output carries a wire cap ~ U[0.5,5] fF.  The window
+    models placement locality: with unbounded random wiring the graph is an
+    expander and any direct factorisation fills in almost completely."""
     c = Circuit()
     rng = np.random.default_rng(seed)
     vdd = c.node("vdd")
@@ -325,7 +328,7 @@ def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5) -> C
         nin = 2 if nand[k] else 1
         ins = []
         for _ in range(nin):
-            j = int(rng.integers(max(0, len(pool) - 2048), len(pool)))
+            j = int(rng.integers(max(0, len(pool) - window), len(pool)))
             ins.append(pool[j])
             pool_left[j] -= 1
             if pool_left[j] <= 0 and len(pool) > 8:

@@ -46,4 +46,25 @@ int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double
   return XG_OK;
 }
 
+int xg_host_mf_plan(int n, const int* rowptr, const int* colind, const double* values, double* info) {
+  if (n <= 0 || !rowptr || !colind || !info) return XG_E_ARG;
+  xg::LuSymbolic s;
+  xg::MfPlan p;
+  try {
+    int st = xg::lu_analyse(n, rowptr, colind, values, s, false, true);
+    if (st != XG_OK) return st;
+    st = xg::mf_plan(s, rowptr, colind, p);
+    if (st != XG_OK) return st;
+  } catch (...) {
+    return XG_E_SETUP;
+  }
+  info[0] = p.ns;
+  info[1] = (double)p.lvl_ptr.size() - 1;
+  info[2] = (double)p.fo[p.ns];
+  info[3] = p.flops;
+  info[4] = (double)p.max_m;
+  info[5] = s.nlu;
+  return XG_OK;
+}
+
 }  // extern "C"

@@ -395,9 +395,43 @@ struct xg_lu {
   int ir_max = 10;
   bool have64 = false;    // FP64 factors of the current values exist (refinement fallback)
   double stats[6] = {0, 0, 0, 0, 0, 0};  // last solve: refinements, rel. residual; totals: solves, refinements, fallbacks, FP32 pivot failures
+  // method: 1 = level-scheduled entry LU (small systems), 2 = multifrontal
+  int method = 1;
+  std::unique_ptr<MfLu> mfp;
+
+  // symbolic analysis for the chosen (or automatically picked) method
+  int analyse(const double* values, int want_method) {
+    LuSymbolic t;
+    int st = lu_analyse(s.n ? s.n : (int)a_rowptr.size() - 1, a_rowptr.data(), a_colind.data(), values, t, false, true);
+    if (st != XG_OK) return st;
+    const int n = t.n;
+    int m = want_method;
+    if (m == 0) m = (t.flops > 2e7 || n > 65535 || t.nlu > (1 << 24)) ? 2 : 1;
+    if (m == 1) {
+      st = lu_analyse(n, a_rowptr.data(), a_colind.data(), values, t, true, false);
+      if (st != XG_OK) return st;
+    }
+    s = std::move(t);
+    method = m;
+    return XG_OK;
+  }
 
   int upload() {
     hipStream_t st = stream;
+    if (method == 2) {
+      HIPCHK(prow.upload(s.perm_row, st)); HIPCHK(pcol.upload(s.perm_col, st));
+      HIPCHK(arp.upload(a_rowptr, st)); HIPCHK(aci.upload(a_colind, st));
+      HIPCHK(flag.alloc(2)); HIPCHK(red.alloc(2));
+      HIPCHK(avals.alloc(s.nnz_a)); HIPCHK(b.alloc(s.n)); HIPCHK(x.alloc(s.n)); HIPCHK(r.alloc(s.n)); HIPCHK(d.alloc(s.n));
+      mfp = std::make_unique<MfLu>();
+      const int rc = mfp->setup(s, a_rowptr.data(), a_colind.data(), (void*)stream, true);
+      if (rc != XG_OK) return rc;
+      // the plan holds what the kernels need; drop the host-side structure
+      std::vector<std::vector<int>>().swap(s.lcol);
+      have64 = false;
+      return XG_OK;
+    }
+    mfp.reset();
     HIPCHK(a2lu.upload(s.a2lu, st)); HIPCHK(lu_rowptr.upload(s.lu_rowptr, st));
     HIPCHK(lu_colind.upload(s.lu_colind, st)); HIPCHK(lu_diag.upload(s.lu_diag, st));
     HIPCHK(f_lvl_ptr.upload(s.f_lvl_ptr, st)); HIPCHK(f_tgt.upload(s.f_tgt, st));
@@ -467,6 +501,17 @@ struct xg_lu {
     HIPCHK(hipGetLastError());
     return XG_OK;
   }
+  // either method: factor the values in `avals` (*bad = failing pivot + 1)
+  int factor_any(bool fp32, int* bad) {
+    if (method == 2) return mfp->factor(avals.p, s.nnz_a, fp32, bad);
+    const int st = fp32 ? factor_device<float>(lu32.p) : factor_device<double>(lu.p);
+    if (st != XG_OK) return st;
+    return fp32 ? check_pivots<float>(lu32.p, bad) : check_pivots<double>(lu.p, bad);
+  }
+  int solve_any(bool fp32, const double* db, double* dx) {
+    if (method == 2) return mfp->solve(db, dx, fp32, flag.p + 1);
+    return fp32 ? solve_device<float>(lu32.p, y32.p, db, dx) : solve_device<double>(lu.p, y.p, db, dx);
+  }
   // r = b - A x; returns max|r| / max|b| (0/0 -> 0)
   int residual(const double* db, const double* dx, double* rel) {
     HIPCHK(hipMemsetAsync(red.p, 0, 2 * sizeof(unsigned long long), stream));
@@ -485,7 +530,7 @@ struct xg_lu {
   // not reach ir_tol (an ill-conditioned system: the FP32 factors' error
   // times the condition number is near 1, or FP64 itself cannot reach ir_tol)
   int solve_refined(const double* db, double* dx) {
-    int st = solve_device<float>(lu32.p, y32.p, db, dx);
+    int st = solve_any(true, db, dx);
     if (st != XG_OK) return st;
     double rel = 0.0, prev = INFINITY;
     int it = 0;
@@ -498,11 +543,12 @@ struct xg_lu {
         // refinement does not converge with the FP32 factors: FP64 factors
         HIPCHK(hipMemsetAsync(flag.p + 1, 0, sizeof(int), stream));
         if (!have64) {
-          st = factor_device<double>(lu.p);
+          int bad64 = 0;
+          st = factor_any(false, &bad64);
           if (st != XG_OK) return st;
           have64 = true;
         }
-        st = solve_device<double>(lu.p, y.p, db, dx);
+        st = solve_any(false, db, dx);
         if (st != XG_OK) return st;
         st = residual(db, dx, &rel);
         if (st != XG_OK) return st;
@@ -510,7 +556,7 @@ struct xg_lu {
         break;
       }
       prev = rel;
-      st = solve_device<float>(lu32.p, y32.p, r.p, d.p);
+      st = solve_any(true, r.p, d.p);
       if (st != XG_OK) return st;
       k_axpy_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, d.p, dx);
       ++it;
@@ -737,25 +783,30 @@ int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, d
 }
 
 // ---------------- LU ----------------
-int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* values, xg_lu** out) {
-  if (n <= 0 || !rowptr || !colind || !out) return XG_E_ARG;
+int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double* values, int method, xg_lu** out) {
+  if (n <= 0 || !rowptr || !colind || !out || method < 0 || method > 2) return XG_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
   auto lu = std::make_unique<xg_lu>();
+  lu->a_rowptr.assign(rowptr, rowptr + n + 1);
+  lu->a_colind.assign(colind, colind + rowptr[n]);
+  lu->s.n = n;
   try {
-    int st = lu_analyse(n, rowptr, colind, values, lu->s, true);
+    int st = lu->analyse(values, method);
     if (st != XG_OK) return st;
   } catch (...) {
     return XG_E_SETUP;
   }
-  lu->a_rowptr.assign(rowptr, rowptr + n + 1);
-  lu->a_colind.assign(colind, colind + rowptr[n]);
   if (hipStreamCreate(&lu->stream) != hipSuccess) return XG_E_HIP;
   lu->own_stream = true;
   int st = lu->upload();
   if (st != XG_OK) return st;
   *out = lu.release();
   return XG_OK;
+}
+
+int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* values, xg_lu** out) {
+  return xg_lu_symbolic_ex(n, rowptr, colind, values, 0, out);
 }
 
 int xg_lu_destroy(xg_lu* lu) {
@@ -774,16 +825,12 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   int bad = 0, st;
   bool fp32_failed = false;
   if (lu->mode == 1) {
-    st = lu->factor_device<float>(lu->lu32.p);
-    if (st != XG_OK) return st;
-    st = lu->check_pivots<float>(lu->lu32.p, &bad);
+    st = lu->factor_any(true, &bad);
     if (st != XG_OK) return st;
     if (!bad) return XG_OK;
     fp32_failed = true;
   }
-  st = lu->factor_device<double>(lu->lu.p);
-  if (st != XG_OK) return st;
-  st = lu->check_pivots<double>(lu->lu.p, &bad);
+  st = lu->factor_any(false, &bad);
   if (st != XG_OK) return st;
   if (!bad) {
     // the FP32 pivots failed where the FP64 ones pass: solve with FP64 factors
@@ -793,27 +840,21 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   }
   // the pivot order itself fails the threshold test (KLU would re-pivot):
   // analyse again steering the transversal by the values, then refactor
-  LuSymbolic s2;
   try {
-    if (lu_analyse(lu->s.n, lu->a_rowptr.data(), lu->a_colind.data(), vals, s2, true) != XG_OK) return bad;
+    if (lu->analyse(vals, lu->method) != XG_OK) return bad;
   } catch (...) {
     return bad;
   }
-  lu->s = std::move(s2);
   st = lu->upload();
   if (st != XG_OK) return st;
   HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
   if (lu->mode == 1) {
-    st = lu->factor_device<float>(lu->lu32.p);
-    if (st != XG_OK) return st;
-    st = lu->check_pivots<float>(lu->lu32.p, &bad);
+    st = lu->factor_any(true, &bad);
     if (st != XG_OK) return st;
     if (!bad) return XG_OK;
     lu->stats[5] += 1;
   }
-  st = lu->factor_device<double>(lu->lu.p);
-  if (st != XG_OK) return st;
-  st = lu->check_pivots<double>(lu->lu.p, &bad);
+  st = lu->factor_any(false, &bad);
   if (st != XG_OK) return st;
   if (bad) return bad;
   lu->have64 = true;
@@ -829,7 +870,7 @@ int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
   HIPCHK(hipMemsetAsync(lu->flag.p + 1, 0, sizeof(int), s));
   int st;
   if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
-  else st = lu->solve_device<double>(lu->lu.p, lu->y.p, lu->b.p, lu->x.p);
+  else st = lu->solve_any(false, lu->b.p, lu->x.p);
   if (st != XG_OK) return st;
   HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   int nanf = 0;
@@ -846,7 +887,7 @@ int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max) {
   lu->mode = precision;
   lu->ir_tol = ir_tol;
   lu->ir_max = ir_max;
-  if (precision == 1) {
+  if (precision == 1 && lu->method == 1) {
     HIPCHK(lu->lu32.alloc(lu->s.nlu));
     HIPCHK(lu->y32.alloc(lu->s.n));
   }
@@ -863,10 +904,19 @@ int xg_lu_precision_stats(const xg_lu* lu, double* stats) {
 
 int xg_lu_info(const xg_lu* lu, double* info) {
   if (!lu || !info) return XG_E_ARG;
-  info[0] = lu->s.nlu;
-  info[1] = (double)lu->s.f_lvl_ptr.size() - 1;
-  info[2] = (double)(lu->s.fs_lvl_ptr.size() - 1 + lu->s.bs_lvl_ptr.size() - 1);
-  info[3] = lu->s.flops;
+  if (lu->method == 2) {
+    info[0] = (double)lu->mfp->front_elements();
+    info[1] = lu->mfp->levels();
+    info[2] = 2.0 * lu->mfp->levels();
+    info[3] = lu->mfp->flops();
+  } else {
+    info[0] = lu->s.nlu;
+    info[1] = (double)lu->s.f_lvl_ptr.size() - 1;
+    info[2] = (double)(lu->s.fs_lvl_ptr.size() - 1 + lu->s.bs_lvl_ptr.size() - 1);
+    info[3] = lu->s.flops;
+  }
+  info[4] = lu->method;
+  info[5] = lu->method == 2 ? lu->mfp->supernodes() : 0;
   return XG_OK;
 }
 

@@ -71,15 +71,27 @@ std::vector<int> max_transversal(int n, const int* rowptr, const int* colind, co
     }
   }
   // augmenting paths (iterative DFS over columns; rows_on_path[i] is the row
-  // tried from column stk[i])
+  // tried from column stk[i]).  Each column entered first looks for a free
+  // row among its own entries (MC21's look-ahead), so short paths win over
+  // deep ones: a voltage-source branch pairs with its node in two hops
+  // instead of shifting a chain of diagonal pivots across the circuit.
   std::vector<int> visited(n, -1), rows_on_path;
   std::vector<std::pair<int, int>> stk;
+  auto free_row = [&](int c) {
+    for (int k = cptr[c]; k < cptr[c + 1]; ++k)
+      if (match_row[crow[k]] < 0) return crow[k];
+    return -1;
+  };
   for (int c0 = 0; c0 < n; ++c0) {
     if (match_col[c0] >= 0) continue;
     stk.assign(1, {c0, cptr[c0]});
     rows_on_path.clear();
     bool found = false;
-    while (!stk.empty()) {
+    {
+      const int r = free_row(c0);
+      if (r >= 0) { rows_on_path.push_back(r); found = true; }
+    }
+    while (!found && !stk.empty()) {
       const int c = stk.back().first, k = stk.back().second;
       if (k >= cptr[c + 1]) {
         stk.pop_back();
@@ -92,7 +104,10 @@ std::vector<int> max_transversal(int n, const int* rowptr, const int* colind, co
       visited[r] = c0;
       rows_on_path.push_back(r);
       if (match_row[r] < 0) { found = true; break; }
-      stk.push_back({match_row[r], cptr[match_row[r]]});
+      const int c2 = match_row[r];
+      stk.push_back({c2, cptr[c2]});
+      const int r2 = free_row(c2);
+      if (r2 >= 0) { rows_on_path.push_back(r2); found = true; break; }
     }
     if (!found) throw std::runtime_error("structurally singular matrix");
     for (size_t i = 0; i < stk.size(); ++i) {
@@ -152,46 +167,99 @@ void mmd_order(const Graph& g0, const std::vector<int>& nodes, std::vector<int>&
 
 // Nested dissection with BFS level-set separators (George's automatic
 // nested dissection) down to leaves ordered by MMD.
-void nd_order(const Graph& g, std::vector<int> nodes, std::vector<int>& order, std::vector<int>& inset, int tag) {
-  if ((int)nodes.size() <= 256) { mmd_order(g, nodes, order); return; }
-  for (int v : nodes) inset[v] = tag;
-  // pseudo-peripheral start
-  auto bfs = [&](int s, std::vector<int>& lvl, int& last) {
-    for (int v : nodes) lvl[v] = -1;
-    std::deque<int> q{s}; lvl[s] = 0; last = s; int depth = 0;
-    while (!q.empty()) {
-      int v = q.front(); q.pop_front(); last = v; depth = lvl[v];
-      for (int a : g[v]) if (inset[a] == tag && lvl[a] < 0) { lvl[a] = lvl[v] + 1; q.push_back(a); }
-    }
-    return depth;
+// Approximate minimum degree ordering on the quotient graph, after
+// Amestoy, Davis and Duff, "An approximate minimum degree ordering
+// algorithm", SIAM J. Matrix Anal. Appl. 17(4), 1996 (the ordering KLU uses,
+// here without supervariable detection).  Eliminated pivots become elements
+// standing for the cliques they created; a variable's degree is the
+// approximate external degree of that paper; elements covered by the new
+// one are absorbed.
+void amd_order(const Graph& g0, const std::vector<int>& nodes, std::vector<int>& order) {
+  const int m = (int)nodes.size();
+  if (m == 0) return;
+  std::vector<int> loc(g0.size(), -1);
+  for (int i = 0; i < m; ++i) loc[nodes[i]] = i;
+  std::vector<std::vector<int>> A(m), E(m), L(m);
+  for (int i = 0; i < m; ++i)
+    for (int a : g0[nodes[i]])
+      if (loc[a] >= 0 && loc[a] != i) A[i].push_back(loc[a]);
+  for (auto& v : A) { std::sort(v.begin(), v.end()); v.erase(std::unique(v.begin(), v.end()), v.end()); }
+  enum { VAR = 0, ELEM = 1, DEAD = 2 };
+  std::vector<char> state(m, VAR);
+  std::vector<int> deg(m), head(m + 1, -1), nxt(m, -1), prv(m, -1);
+  auto bucket_add = [&](int i, int d) {
+    deg[i] = d; prv[i] = -1; nxt[i] = head[d];
+    if (head[d] >= 0) prv[head[d]] = i;
+    head[d] = i;
   };
-  static thread_local std::vector<int> lvl;
-  if (lvl.size() < g.size()) lvl.assign(g.size(), -1);
-  int last = nodes[0];
-  int depth = bfs(nodes[0], lvl, last);
-  for (int it = 0; it < 3; ++it) { int l2; int d2 = bfs(last, lvl, l2); if (d2 <= depth) break; depth = d2; last = l2; }
-  int l2;
-  depth = bfs(last, lvl, l2);
-  // disconnected remainder goes with the far side
-  std::vector<int> a, b, sep, rest;
-  const int mid = depth / 2;
-  if (depth < 2) { for (int v : nodes) inset[v] = -1; mmd_order(g, nodes, order); return; }
-  for (int v : nodes) {
-    if (lvl[v] < 0) rest.push_back(v);
-    else if (lvl[v] < mid) a.push_back(v);
-    else if (lvl[v] > mid) b.push_back(v);
-    else sep.push_back(v);
+  auto bucket_del = [&](int i) {
+    if (prv[i] >= 0) nxt[prv[i]] = nxt[i]; else head[deg[i]] = nxt[i];
+    if (nxt[i] >= 0) prv[nxt[i]] = prv[i];
+  };
+  for (int i = 0; i < m; ++i) bucket_add(i, (int)A[i].size());
+  std::vector<int> mark(m, -1), wflag(m, -1), w(m, 0), Lp;
+  int mindeg = 0, nleft = m;
+  for (int step = 0; step < m; ++step) {
+    while (head[mindeg] < 0) ++mindeg;
+    const int p = head[mindeg];
+    bucket_del(p);
+    order.push_back(nodes[p]);
+    --nleft;
+    // the new element: p's variables and those of its elements (absorbed)
+    Lp.clear();
+    mark[p] = step;
+    for (int j : A[p])
+      if (state[j] == VAR && mark[j] != step) { mark[j] = step; Lp.push_back(j); }
+    for (int e : E[p]) {
+      if (state[e] != ELEM) continue;
+      for (int j : L[e])
+        if (state[j] == VAR && mark[j] != step) { mark[j] = step; Lp.push_back(j); }
+      state[e] = DEAD;
+      std::vector<int>().swap(L[e]);
+    }
+    state[p] = ELEM;
+    std::vector<int>().swap(A[p]);
+    std::vector<int>().swap(E[p]);
+    L[p] = Lp;
+    // |Le \ Lp| for the elements next to the new element's variables
+    for (int i : Lp)
+      for (int e : E[i]) {
+        if (state[e] != ELEM || e == p) continue;
+        if (wflag[e] != step) { wflag[e] = step; w[e] = (int)L[e].size(); }
+        --w[e];
+      }
+    const int lp = (int)Lp.size();
+    for (int i : Lp) {
+      // prune variables now reached through the new element
+      int k = 0;
+      for (int j : A[i])
+        if (state[j] == VAR && mark[j] != step) A[i][k++] = j;
+      A[i].resize(k);
+      int ext = 0;
+      k = 0;
+      for (int e : E[i]) {
+        if (state[e] != ELEM || e == p) continue;
+        if (wflag[e] == step && w[e] == 0) { state[e] = DEAD; std::vector<int>().swap(L[e]); continue; }  // e inside Lp
+        E[i][k++] = e;
+        ext += wflag[e] == step ? w[e] : (int)L[e].size();
+      }
+      E[i].resize(k);
+      E[i].push_back(p);
+      long long d = (long long)A[i].size() + (lp - 1) + ext;
+      d = std::min<long long>(d, (long long)deg[i] + lp - 1);
+      d = std::min<long long>(d, nleft - 1);
+      if (d < 0) d = 0;
+      bucket_del(i);
+      bucket_add(i, (int)d);
+      if (d < mindeg) mindeg = (int)d;
+    }
   }
-  b.insert(b.end(), rest.begin(), rest.end());
-  for (int v : nodes) inset[v] = -1;
-  nd_order(g, a, order, inset, tag * 2 + 1);
-  nd_order(g, b, order, inset, tag * 2 + 2);
-  mmd_order(g, sep, order);
 }
 
 }  // namespace
 
-int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s, bool want_schedule) {
+int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s, bool want_schedule,
+               bool keep_tree) {
   s = LuSymbolic();
   s.n = n; s.nnz_a = rowptr[n];
   std::vector<int> match = max_transversal(n, rowptr, colind, values);  // A'(c,:) = A(match[c],:)
@@ -211,12 +279,10 @@ int lu_analyse(int n, const int* rowptr, const int* colind, const double* values
   for (int v = 0; v < n; ++v) ((int)g[v].size() > dense_thresh ? dense_nodes : sparse_nodes).push_back(v);
   std::vector<int> order;
   order.reserve(n);
-  if ((int)sparse_nodes.size() <= 4096) {
-    mmd_order(g, sparse_nodes, order);
-  } else {
-    std::vector<int> inset(n, -1);
-    nd_order(g, sparse_nodes, order, inset, 1);
-  }
+  // small systems (the batched engine's classes): multiple minimum degree,
+  // whose independent sets keep the level schedule shallow; larger: AMD
+  if ((int)sparse_nodes.size() <= 256) mmd_order(g, sparse_nodes, order);
+  else amd_order(g, sparse_nodes, order);
   order.insert(order.end(), dense_nodes.begin(), dense_nodes.end());
   s.dense_tail = (int)dense_nodes.size();
   // B(i,j) = A'(order[i], order[j]) = A(match[order[i]], order[j])
@@ -294,6 +360,12 @@ int lu_analyse(int n, const int* rowptr, const int* colind, const double* values
     double fl = 0;
     for (int k = 0; k < n; ++k) fl += 2.0 * (double)urow[k].size() * (double)urow[k].size() + (double)urow[k].size();
     s.flops = fl;
+  }
+  if (keep_tree) {
+    s.etree = parent;
+    s.lcol = urow;
+    s.rinv = rinv;
+    s.pinv = pinv;
   }
   if (!want_schedule) return XG_OK;
 

@@ -93,10 +93,57 @@ struct LuSymbolic {
   std::vector<int> bs_lvl_ptr, bs_rows;   // backward
   double flops = 0;
   int dense_tail = 0;
+  // elimination tree and column structure of L (rows below the diagonal,
+  // ascending) of B, kept when the multifrontal plan is wanted
+  std::vector<int> etree;
+  std::vector<std::vector<int>> lcol;
+  std::vector<int> rinv, pinv;   // A row -> B row, A col -> B col
 };
 
+// want_schedule: build the entry-level factor / solve schedules;
+// keep_tree: keep the elimination tree and the L column structure
 int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s,
-               bool want_schedule = true);
+               bool want_schedule = true, bool keep_tree = false);
+
+// Multifrontal plan of a symbolic analysis (multifrontal.hip)
+struct MfPlan {
+  int n = 0, ns = 0;
+  std::vector<int> sfirst;         // ns + 1: columns [sfirst[s], sfirst[s+1]) of B
+  std::vector<int> sparent;        // parent supernode, -1 at a root
+  std::vector<int> m;              // front order (pivots + rows below)
+  std::vector<long long> fo;       // front element offsets (row-major fronts), fo[ns] = total
+  std::vector<int> iptr, idx;      // front index lists (B indices), pivots first
+  std::vector<long long> amap;     // A slot -> front element
+  std::vector<int> cptr, clist;    // children of each supernode
+  std::vector<int> eptr, emap;     // per supernode: parent-local index of each row below its pivots
+  std::vector<int> lvl_ptr, lvl_s; // supernodes by level, leaves first
+  std::vector<int> vo;             // per-front vector offsets, vo[ns] = total
+  double flops = 0;
+  long long max_m = 0;
+};
+int mf_plan(const LuSymbolic& s, const int* rowptr, const int* colind, MfPlan& p);
+
+// Multifrontal factorisation on the GPU (multifrontal.hip); the stream is a
+// hipStream_t, vectors are device pointers in A's numbering.
+class MfLu {
+ public:
+  struct Impl;
+  MfLu();
+  ~MfLu();
+  MfLu(const MfLu&) = delete;
+  MfLu& operator=(const MfLu&) = delete;
+  int setup(const LuSymbolic& s, const int* rowptr, const int* colind, void* stream, bool want_fp32);
+  // *bad = 0, or (first failing pivot column of B) + 1
+  int factor(const double* vals_dev, int nnz, bool fp32, int* bad);
+  int solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag);
+  double flops() const;
+  long long front_elements() const;
+  int levels() const;
+  int supernodes() const;
+
+ private:
+  Impl* impl_;
+};
 
 // learned-predictor network (mlp.hip): forward on a caller stream (a
 // hipStream_t), and its dimensions
