@@ -8,11 +8,12 @@
 // is processed level by level (all fronts of a level in one launch):
 //
 //   assemble  F_s = A entries of s's columns + extend-add of the children's
-//             Schur complements (one workgroup per front, children in a fixed
-//             order: deterministic, no atomics)
+//             Schur complements (the j-th children of a level in launch j,
+//             rows split over workgroups: a fixed order, no atomics)
 //   factor    dense LU of the pivot block, L21, U12 and the Schur complement
 //             F22 -= L21 U12; small fronts one workgroup each, large fronts
-//             blocked (panel, triangular solve, tiled update over the GPU)
+//             blocked by 32-column panels (diagonal block in LDS, row-parallel
+//             triangular solves for L21 and U12, tiled update over the GPU)
 //
 // Static pivots, as in the level-scheduled path, with KLU's threshold test
 // (|l_ij| <= 1/0.001, finite nonzero pivots) on every column.
@@ -178,22 +179,27 @@ __global__ void k_mf_scatter(int nnz, const double* vals, const long long* amap,
   if (k < nnz) F[amap[k]] += (T)vals[k];
 }
 
-// extend-add of the children's Schur complements into each front of a level
+// extend-add of one child's Schur complement into its parent: block.x picks
+// the child from `list`, block.y a chunk of 16 of its rows.  Children of one
+// parent go in separate launches (their j-th child in launch j), so every
+// parent element is added to in a fixed order and no two blocks of a launch
+// write the same element.
 template <class T>
-__global__ void k_mf_assemble(MfDev d, const int* list, T* F) {
-  const int s = list[blockIdx.x];
-  const int ms = d.m[s];
-  T* Fs = F + d.fo[s];
-  for (int c = d.cptr[s]; c < d.cptr[s + 1]; ++c) {
-    const int ch = d.clist[c];
-    const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
-    const T* Fc = F + d.fo[ch];
-    const int* em = d.emap + d.eptr[ch];
-    for (long long e = threadIdx.x; e < (long long)nu * nu; e += blockDim.x) {
-      const int a = (int)(e / nu), b = (int)(e % nu);
-      Fs[(long long)em[a] * ms + em[b]] += Fc[(long long)(npc + a) * mc + npc + b];
-    }
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_mf_extend_add(MfDev d, const int* list, const int* sparent, T* F) {
+  const int ch = list[blockIdx.x];
+  const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
+  const int a0 = blockIdx.y * 16;
+  if (a0 >= nu) return;
+  const int pa = sparent[ch];
+  const int ms = d.m[pa];
+  T* Fs = F + d.fo[pa];
+  const T* Fc = F + d.fo[ch];
+  const int* em = d.emap + d.eptr[ch];
+  const int a1 = min(nu, a0 + 16);
+  for (int a = a0; a < a1; ++a) {
+    const long long prow = (long long)em[a] * ms;
+    const T* crow = Fc + (long long)(npc + a) * mc + npc;
+    for (int b = threadIdx.x; b < nu; b += blockDim.x) Fs[prow + em[b]] += crow[b];
   }
 }
 
@@ -233,47 +239,95 @@ __global__ void __launch_bounds__(256) k_mf_factor_small(MfDev d, const int* lis
   if (threadIdx.x == 0 && flag) atomicMin(bad, d.sfirst[s] + 1);
 }
 
-// large fronts, blocked: (1) panel of pw columns starting at p0, rows p0..m
+// large fronts, blocked by panels of pw <= 32 columns
+// (1) one launch per panel: every workgroup (one wavefront) factors the
+// pw x pw diagonal block in LDS (workgroup 0 also stores the factors in
+// `dbuf`, which the update launch copies into the front: the other
+// workgroups may still be reading the unfactored block), then solves a 64-row block of L21 (x U11 = row) or a 64-column
+// block of U12 (L11 x = column), its tile staged through LDS so the global
+// loads and stores are coalesced.
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_panel(T* Fs, int ms, int p0, int pw, int col0, int* bad) {
+__global__ void __launch_bounds__(64) k_mf_panel_solve(T* Fs, int ms, int p0, int pw, int nrb, int col0, T* dbuf,
+                                                       int* bad) {
+  __shared__ T D[32][33];
+  __shared__ T Tl[64][33];
   __shared__ int flag;
-  if (threadIdx.x == 0) flag = 0;
+  const int t = threadIdx.x;
+  if (t == 0) flag = 0;
+  for (int e = t; e < pw * pw; e += 64) D[e / pw][e % pw] = Fs[(long long)(p0 + e / pw) * ms + p0 + e % pw];
   __syncthreads();
-  for (int k = p0; k < p0 + pw; ++k) {
-    const T piv = Fs[(long long)k * ms + k];
-    if (mf_bad_pivot(piv)) { if (threadIdx.x == 0) flag = 1; break; }
-    for (int i = k + 1 + threadIdx.x; i < ms; i += blockDim.x) {
-      const T l = Fs[(long long)i * ms + k] / piv;
-      Fs[(long long)i * ms + k] = l;
+  for (int k = 0; k < pw; ++k) {
+    const T piv = D[k][k];
+    if (mf_bad_pivot(piv)) { if (t == 0) flag = 1; break; }
+    if (t > k && t < pw) {
+      const T l = D[t][k] / piv;
+      D[t][k] = l;
       if (!(fabs((double)l) <= 1.0e3)) flag = 1;
-      for (int j = k + 1; j < p0 + pw; ++j) Fs[(long long)i * ms + j] -= l * Fs[(long long)k * ms + j];
+      for (int j = k + 1; j < pw; ++j) D[t][j] -= l * D[k][j];
     }
     __syncthreads();
   }
   __syncthreads();
-  if (threadIdx.x == 0 && flag) atomicMin(bad, col0 + p0 + 1);
-}
-
-// (2) U12 = L11^-1 F12 for the panel rows, one thread per column
-template <class T>
-__global__ void k_mf_trsm(T* Fs, int ms, int p0, int pw) {
-  const int j = p0 + pw + blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ms) return;
-  for (int r = p0 + 1; r < p0 + pw; ++r) {
-    T v = Fs[(long long)r * ms + j];
-    for (int q = p0; q < r; ++q) v -= Fs[(long long)r * ms + q] * Fs[(long long)q * ms + j];
-    Fs[(long long)r * ms + j] = v;
+  const int base = p0 + pw;
+  if (blockIdx.x == 0) {
+    for (int e = t; e < pw * pw; e += 64) dbuf[e] = D[e / pw][e % pw];
   }
+  if ((int)blockIdx.x < nrb) {
+    // L21 rows [r0, r0 + 64)
+    const int r0 = base + blockIdx.x * 64;
+    for (int e = t; e < 64 * pw; e += 64) {
+      const int rr = e / pw, q = e % pw;
+      Tl[rr][q] = r0 + rr < ms ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
+    }
+    __syncthreads();
+    if (r0 + t < ms) {
+      for (int q = 0; q < pw; ++q) {
+        T a = Tl[t][q];
+        for (int r = 0; r < q; ++r) a -= Tl[t][r] * D[r][q];
+        a /= D[q][q];
+        if (!(fabs((double)a) <= 1.0e3)) flag = 1;
+        Tl[t][q] = a;
+      }
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * pw; e += 64) {
+      const int rr = e / pw, q = e % pw;
+      if (r0 + rr < ms) Fs[(long long)(r0 + rr) * ms + p0 + q] = Tl[rr][q];
+    }
+  } else {
+    // U12 columns [c0, c0 + 64)
+    const int c0 = base + (blockIdx.x - nrb) * 64;
+    for (int e = t; e < 64 * pw; e += 64) {
+      const int q = e / 64, cc = e % 64;
+      Tl[cc][q] = c0 + cc < ms ? Fs[(long long)(p0 + q) * ms + c0 + cc] : (T)0;
+    }
+    __syncthreads();
+    for (int r = 1; r < pw; ++r) {
+      T a = Tl[t][r];
+      for (int q = 0; q < r; ++q) a -= D[r][q] * Tl[t][q];
+      Tl[t][r] = a;
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * pw; e += 64) {
+      const int q = e / 64, cc = e % 64;
+      if (c0 + cc < ms) Fs[(long long)(p0 + q) * ms + c0 + cc] = Tl[cc][q];
+    }
+  }
+  __syncthreads();
+  if (t == 0 && flag && (blockIdx.x == 0 || (int)blockIdx.x < nrb)) atomicMin(bad, col0 + p0 + 1);
 }
 
-// (3) trailing update F[i][j] -= sum_q L[i][q] U[q][j], i, j >= p0 + pw,
+// (2) trailing update F[i][j] -= sum_q L[i][q] U[q][j], i, j >= p0 + pw,
 // 64x64 tiles, the panel's L rows and U columns staged in LDS
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_update(T* Fs, int ms, int p0, int pw) {
+__global__ void __launch_bounds__(256) k_mf_update(T* Fs, int ms, int p0, int pw, const T* dbuf) {
   constexpr int TB = 64, KB = 16;
   __shared__ T Ls[TB][KB + 1];
   __shared__ T Us[KB][TB + 1];
   const int base = p0 + pw;
+  if (blockIdx.x == 0 && blockIdx.y == 0)    // the panel's diagonal block factors
+    for (int e = threadIdx.x; e < pw * pw; e += blockDim.x) Fs[(long long)(p0 + e / pw) * ms + p0 + e % pw] = dbuf[e];
+  if (base >= ms) return;
   const int i0 = base + blockIdx.y * TB, j0 = base + blockIdx.x * TB;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4 x 4 outputs each
   T acc[4][4];
@@ -409,8 +463,12 @@ struct MfLu::Impl {
   DevVec<double> F64, V64, y64, b64;
   DevVec<float> F32, V32, y32;
   std::vector<std::vector<int>> small_of_level, large_of_level;
-  DevVec<int> small_list;
+  DevVec<int> small_list, ea_list, sparent;
+  DevVec<double> dbuf;            // one panel's diagonal-block factors (FP64 or FP32)
   std::vector<int> small_off;     // per level: offset into small_list
+  // extend-add launches: per level, per child index j: [offset, count, max rows]
+  struct EaLaunch { int off, count, max_nu; };
+  std::vector<std::vector<EaLaunch>> ea_of_level;
   MfDev dev() const {
     MfDev d;
     d.sfirst = sfirst.p; d.m = m.p; d.fo = fo.p; d.iptr = iptr.p; d.idx = idx.p; d.cptr = cptr.p;
@@ -441,6 +499,7 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   MFCHK(I.emap.up(P.emap, st)); MFCHK(I.vo.up(P.vo, st)); MFCHK(I.fo.up(P.fo, st)); MFCHK(I.amap.up(P.amap, st));
   MFCHK(I.prow.up(s.perm_row, st)); MFCHK(I.pcol.up(s.perm_col, st));
   MFCHK(I.bad.alloc(2));
+  MFCHK(I.dbuf.alloc(32 * 32));
   MFCHK(I.F64.alloc(P.fo[P.ns])); MFCHK(I.V64.alloc(P.vo[P.ns])); MFCHK(I.y64.alloc(P.n)); MFCHK(I.b64.alloc(P.n));
   if (want_fp32) { MFCHK(I.F32.alloc(P.fo[P.ns])); MFCHK(I.V32.alloc(P.vo[P.ns])); MFCHK(I.y32.alloc(P.n)); }
   // per level: small fronts (one launch) and large fronts (blocked, one by one)
@@ -457,6 +516,32 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   }
   I.small_off[nl] = (int)sl.size();
   MFCHK(I.small_list.up(sl, st));
+  // extend-add launches: the j-th child of every front of a level together
+  std::vector<int> ea;
+  I.ea_of_level.assign(nl, {});
+  for (int l = 1; l < nl; ++l) {
+    int maxc = 0;
+    for (int q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
+      const int t = P.lvl_s[q];
+      maxc = std::max(maxc, P.cptr[t + 1] - P.cptr[t]);
+    }
+    for (int j = 0; j < maxc; ++j) {
+      Impl::EaLaunch L{(int)ea.size(), 0, 0};
+      for (int q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
+        const int t = P.lvl_s[q];
+        if (P.cptr[t + 1] - P.cptr[t] <= j) continue;
+        const int ch = P.clist[P.cptr[t] + j];
+        const int nu = P.m[ch] - (P.sfirst[ch + 1] - P.sfirst[ch]);
+        if (nu == 0) continue;
+        ea.push_back(ch);
+        L.count++;
+        L.max_nu = std::max(L.max_nu, nu);
+      }
+      if (L.count) I.ea_of_level[l].push_back(L);
+    }
+  }
+  MFCHK(I.ea_list.up(ea, st));
+  MFCHK(I.sparent.up(P.sparent, st));
   MFCHK(I.lvl_s.up(P.lvl_s, st));
   MFCHK(hipStreamSynchronize(st));
   return XG_OK;
@@ -480,7 +565,10 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
   const int nl = (int)P.lvl_ptr.size() - 1;
   for (int l = 0; l < nl; ++l) {
     const int cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
-    if (l > 0) hipLaunchKernelGGL(k_mf_assemble<T>, dim3(cnt), dim3(256), 0, st, d, I.lvl_s.p + P.lvl_ptr[l], F);
+    (void)cnt;
+    for (const auto& L : I.ea_of_level[l])
+      hipLaunchKernelGGL(k_mf_extend_add<T>, dim3(L.count, (L.max_nu + 15) / 16), dim3(256), 0, st, d,
+                         I.ea_list.p + L.off, I.sparent.p, F);
     const int ns_small = I.small_off[l + 1] - I.small_off[l];
     if (ns_small)
       hipLaunchKernelGGL(k_mf_factor_small<T>, dim3(ns_small), dim3(256), 0, st, d, I.small_list.p + I.small_off[l], F,
@@ -490,13 +578,13 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
       const int ms = P.m[t], np = P.sfirst[t + 1] - P.sfirst[t];
       for (int p0 = 0; p0 < np; p0 += kMfPanel) {
         const int pw = std::min(kMfPanel, np - p0);
-        hipLaunchKernelGGL(k_mf_panel<T>, dim3(1), dim3(256), 0, st, Fs, ms, p0, pw, P.sfirst[t], I.bad.p);
         const int rest = ms - p0 - pw;
-        if (rest > 0) {
-          hipLaunchKernelGGL(k_mf_trsm<T>, dim3((rest + 255) / 256), dim3(256), 0, st, Fs, ms, p0, pw);
-          const unsigned tiles = (unsigned)((rest + 63) / 64);
-          hipLaunchKernelGGL(k_mf_update<T>, dim3(tiles, tiles), dim3(256), 0, st, Fs, ms, p0, pw);
-        }
+        const int nb = (rest + 63) / 64;
+        T* dbuf = reinterpret_cast<T*>(I.dbuf.p);
+        hipLaunchKernelGGL(k_mf_panel_solve<T>, dim3(std::max(1, 2 * nb)), dim3(64), 0, st, Fs, ms, p0, pw, nb,
+                           P.sfirst[t], dbuf, I.bad.p);
+        const unsigned tiles = (unsigned)std::max(1, (rest + 63) / 64);
+        hipLaunchKernelGGL(k_mf_update<T>, dim3(tiles, tiles), dim3(256), 0, st, Fs, ms, p0, pw, (const T*)dbuf);
       }
     }
   }
