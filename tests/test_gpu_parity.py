@@ -254,3 +254,35 @@ def test_batch_power_grids_match_oracle(gpu):
         assert abs(len(t_g) - len(r["t"])) <= 0.02 * len(r["t"]) + 2, (len(t_g), len(r["t"]))
         err = waveform_error(t_g, x_g, r["t"], r["x"])
         assert err <= 1e-3 * 1.8 + 1e-6, err
+
+
+def test_netlist_transient_matches_oracle(gpu, tmp_path):
+    """A hand-written Xyce netlist read by the netlist reader, run on the GPU,
+    written as .PRN, against the oracle's transient of the same circuit."""
+    from xyce_amd.netlist import parse_netlist, read_prn, run_transient
+    text = """inverter chain
+V1 vdd 0 DC 1.8
+V2 in 0 PULSE(0 1.8 0.2n 50p 50p 1n 3n)
+M1 a in 0 0 nch W=2u L=0.5u
+M2 a in vdd vdd pch W=4u L=0.5u
+M3 out a 0 0 nch W=2u L=0.5u
+M4 out a vdd vdd pch W=4u L=0.5u
+C1 a 0 3f
+C2 out 0 5f
+.model nch NMOS LEVEL=1 VTO=0.7 KP=110u
+.model pch PMOS LEVEL=1 VTO=-0.7 KP=50u
+.TRAN 1p 4n 0 10p
+.PRINT TRAN V(a) V(out)
+.END
+"""
+    t, cols = run_transient(text, prn_path=tmp_path / "inv.prn")
+    hdr, data = read_prn(tmp_path / "inv.prn")
+    assert hdr == ["Index", "TIME", "V(A)", "V(OUT)"] and len(data) == len(t)
+    c, an = parse_netlist(text)
+    o = Oracle(c)
+    r = o.tran(4e-9, 1e-12, 10e-12)
+    gid = o.pattern()["node_gid"]
+    for nm in ("A", "OUT"):
+        col = gid[an["nodes"][nm]]
+        err = waveform_error(t, cols[f"V({nm})"][:, None], r["t"], r["x"][:, [col]])
+        assert err <= 1e-3 * 1.8 + 1e-6, (nm, err)

@@ -144,19 +144,20 @@ class Circuit:
 
     # ------------------------------------------------------------------
     def to_spice(self, title="xyce_amd circuit", tran=None) -> str:
-        """Equivalent Xyce netlist (one .model per distinct card)."""
+        """Equivalent Xyce netlist (one .model per distinct card; values with 17
+        significant digits, so reading it back gives the same doubles)."""
         nm = lambda i: "0" if i == 0 else self.names.get(i, f"n{i}")
         lines = [f"* {title}"]
         for mi, card in enumerate(self.models):
             typ = "nmos" if card[0] > 0 else "pmos"
-            kv = " ".join(f"{MODEL_PARAMS[k]}={card[k]:.12g}" for k in range(1, NMP)
+            kv = " ".join(f"{MODEL_PARAMS[k]}={card[k]:.17g}" for k in range(1, NMP)
                           if not math.isnan(card[k]))
             lines.append(f".model m{mi} {typ} level=1 {kv}")
         for t, i in self.order:
             if t == MOS1:
                 d, g, s, b = self.mos_nodes[i]
                 inst = self.mos_inst[i]
-                kv = " ".join(f"{INST_PARAMS[k]}={inst[k]:.12g}" for k in range(0, 9)
+                kv = " ".join(f"{INST_PARAMS[k]}={inst[k]:.17g}" for k in range(0, 9)
                               if not math.isnan(inst[k]))
                 mname = f"m{self.mos_model[i]}"
                 if not (math.isnan(inst[9]) and math.isnan(inst[10])):
@@ -167,28 +168,28 @@ class Circuit:
                         card[2] = inst[10]
                     typ = "nmos" if card[0] > 0 else "pmos"
                     mname = f"mdev{i}"
-                    mkv = " ".join(f"{MODEL_PARAMS[k]}={card[k]:.12g}" for k in range(1, NMP)
+                    mkv = " ".join(f"{MODEL_PARAMS[k]}={card[k]:.17g}" for k in range(1, NMP)
                                    if not math.isnan(card[k]))
                     lines.append(f".model {mname} {typ} level=1 {mkv}")
                 lines.append(f"M{i} {nm(d)} {nm(g)} {nm(s)} {nm(b)} {mname} {kv}")
             elif t == RES:
                 p, n, r = self.res[i]
-                lines.append(f"R{i} {nm(p)} {nm(n)} {r:.12g}")
+                lines.append(f"R{i} {nm(p)} {nm(n)} {r:.17g}")
             elif t == CAP:
                 p, n, c = self.cap[i]
-                lines.append(f"C{i} {nm(p)} {nm(n)} {c:.12g}")
+                lines.append(f"C{i} {nm(p)} {nm(n)} {c:.17g}")
             else:
                 p, n, w = (self.vsrc if t == VSRC else self.isrc)[i]
                 wt, wp = self.waves[w]
                 if wt == W_DC:
-                    src = f"DC {wp[0]:.12g}"
+                    src = f"DC {wp[0]:.17g}"
                 elif wt == W_PULSE:
-                    src = "PULSE(" + " ".join(f"{v:.12g}" for v in wp if not math.isnan(v)) + ")"
+                    src = "PULSE(" + " ".join(f"{v:.17g}" for v in wp if not math.isnan(v)) + ")"
                 else:
-                    src = "PWL(" + " ".join(f"{v:.12g}" for v in wp) + ")"
+                    src = "PWL(" + " ".join(f"{v:.17g}" for v in wp) + ")"
                 lines.append(f"{'V' if t == VSRC else 'I'}{i} {nm(p)} {nm(n)} {src}")
         if tran:
-            lines.append(".TRAN " + " ".join(f"{v:.12g}" for v in tran))
+            lines.append(".TRAN " + " ".join(f"{v:.17g}" for v in tran))
         lines.append(".END")
         return "\n".join(lines) + "\n"
 
