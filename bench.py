@@ -127,7 +127,10 @@ def main():
         if args.form != "auto":
             b.config(iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
     log(f"launch form: {tune} (form {args.form})")
-    k_iter = 1 if (tune["split_eval"] if args.form == "auto" else args.form == "split") else args.iters
+    if args.form == "auto":
+        k_iter = tune["iters_per_launch"]
+    else:
+        k_iter = 1 if args.form == "split" else args.iters
     b.advance(args.warmup, graph_chunk=50)
     c0 = b.counters()
     torch.cuda.synchronize()
@@ -173,10 +176,12 @@ def main():
     traffic = None
     tfile = os.path.join(HERE, "profiles", "traffic_k_batch_step.json")
     if os.path.exists(tfile):
-        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-        # (scaled by the copy-kernel calibration), recorded with the command that made them
+        # HBM bytes per batched iteration from the rocprofv3 FETCH_SIZE /
+        # WRITE_SIZE passes (scaled by the copy-kernel calibration), recorded
+        # with the command that made them; times this launch's iterations
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            per_it = json.load(f).get("hbm_bytes_per_iteration")
+        traffic = int(per_it * k_iter) if per_it else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "k_batch_eval + k_batch_step" if k_iter == 1 and tune["split_eval"] else "k_batch_step",

@@ -202,9 +202,11 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
  * update, step control; phase_us[15] = iterations timed. */
 int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
 /* Autotuner: times the launch forms (split / fused with several iterations
- * per launch) x workgroup sizes x instance orders on the real batch and keeps
- * the fastest; result[0..4] = block size, order mode (1 = largest instances
- * first), us per batched iteration, split_eval, iterations per launch. */
+ * per launch) x workgroup sizes x instance grouping on the real batch and
+ * keeps the fastest; for the fused form with LDS-sized groups it then picks
+ * each group's workgroup size and the iterations per launch (4, 8, 16).
+ * result[0..8] = block size, grouping (1 = by LDS need), us per batched
+ * iteration, split_eval, iterations per launch, workgroup size of groups 0-3. */
 int xg_batch_autotune(xg_batch* b, double* result);
 
 /* Learned predictor in a batch (call before the first xg_batch_advance):

@@ -1193,11 +1193,12 @@ struct xg_batch {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 256;
+  int only_bucket = -1;           // autotuner: launch one bucket only
   int split = 1;                  // 1: separate device-evaluation kernel + per-instance kernel
   size_t shmem = 0;
   // launch buckets: instances grouped by the LDS their class needs, one
   // per-instance launch each with its own dynamic LDS size
-  struct Bucket { int start, count, n, nlu; size_t shmem; };
+  struct Bucket { int start, count, n, nlu; size_t shmem; int block; };   // block 0: the batch's
   std::vector<Bucket> buckets;
   std::vector<size_t> cls_lds;    // per class: dynamic LDS bytes
   std::vector<int> inst_cls;
@@ -1234,7 +1235,7 @@ struct xg_batch {
       for (int k = 0; k < nb; ++k) {
         const int s0 = (int)((long long)n_inst * k / nb), s1 = (int)((long long)n_inst * (k + 1) / nb);
         if (s1 <= s0) continue;
-        Bucket bk{s0, s1 - s0, 0, 0, 0};
+        Bucket bk{s0, s1 - s0, 0, 0, 0, 0};
         int blob = 0;
         for (int j = s0; j < s1; ++j) {
           const ClassDesc& C = classes[inst_cls[ord[j]]];
@@ -1244,7 +1245,7 @@ struct xg_batch {
         buckets.push_back(bk);
       }
     } else {
-      buckets.push_back(Bucket{0, n_inst, args.max_n, args.max_nlu, shmem});
+      buckets.push_back(Bucket{0, n_inst, args.max_n, args.max_nlu, shmem, 0});
     }
     (void)hipMemcpy(d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice);
     order_mode = mode;
@@ -1259,11 +1260,14 @@ struct xg_batch {
       if (args.nt_mos > 0)
         hipLaunchKernelGGL(k_batch_eval, dim3((args.nt_mos + 255) / 256), dim3(256), 0, stream, args);
     }
-    for (const Bucket& bk : buckets) {
+    for (size_t j = 0; j < buckets.size(); ++j) {
+      if (only_bucket >= 0 && (int)j != only_bucket) continue;
+      const Bucket& bk = buckets[j];
       BatchArgs a = args;
       a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
-      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(block), bk.shmem, stream, a);
-      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(block), bk.shmem, stream, a);
+      const int bs = bk.block ? bk.block : block;
+      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(bs), bk.shmem, stream, a);
+      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), bk.shmem, stream, a);
     }
     if (pred_mode == 2) {
       const int st = mlp_forward_on_stream(net, pred_rows, d_feat.p, d_pred_out.p, stream);
@@ -1794,13 +1798,24 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   if (!b) return XG_E_ARG;
   // Candidates: kernel form (separate evaluation kernel + one iteration per
   // launch, or fused with K iterations per launch) x workgroup size x
-  // instance order.  Each candidate runs a few real launches (they advance
+  // instance grouping.  Each candidate runs a few real launches (they advance
   // the simulation; every launch does valid Newton iterations) and the
-  // fastest time per batched iteration wins.
+  // fastest time per batched iteration wins.  For the fused form with
+  // LDS-sized groups, each group then gets its own workgroup size and K is
+  // chosen from {4, 8, 16}.
   const int blocks[3] = {128, 256, 512};
   const int fused_iters = b->args.iters > 1 ? b->args.iters : 8;
   double best = 1e30;
   int best_b = b->block, best_o = b->order_mode, best_s = b->split, best_k = b->args.iters;
+  auto time_launches = [&](int warm, int timed, double* us) -> int {
+    int st = xg_batch_advance(b, warm, 0);
+    if (st != XG_OK) return st;
+    st = xg_batch_advance(b, timed, 0);
+    if (st != XG_OK) return st;
+    *us = b->last_seconds / timed * 1e6;
+    return XG_OK;
+  };
+  for (auto& bk : b->buckets) bk.block = 0;
   for (int sp = 1; sp >= 0; --sp) {
     for (int o = 0; o < 2; ++o) {
       b->set_order(o);
@@ -1809,11 +1824,10 @@ int xg_batch_autotune(xg_batch* b, double* result) {
         b->split = sp;
         b->args.iters = sp ? 1 : fused_iters;
         b->drop_graph();
-        int st = xg_batch_advance(b, 2, 0);
+        double us = 0;
+        const int st = time_launches(2, 8, &us);
         if (st != XG_OK) return st;
-        st = xg_batch_advance(b, 8, 0);
-        if (st != XG_OK) return st;
-        const double us = b->last_seconds / 8 * 1e6 / b->args.iters;
+        us /= b->args.iters;
         if (us < best) { best = us; best_b = bs; best_o = o; best_s = sp; best_k = b->args.iters; }
       }
     }
@@ -1822,9 +1836,44 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   b->split = best_s;
   b->args.iters = best_k;
   b->set_order(best_o);
+  if (!best_s && best_o == 1) {
+    // fused, grouped: per-group workgroup size, then iterations per launch
+    double best_total = 1e30;
+    int best_kk = best_k;
+    std::vector<int> best_blocks(b->buckets.size(), best_b);
+    for (int kk : {4, 8, 16}) {
+      b->args.iters = kk;
+      double total = 0;
+      std::vector<int> bb(b->buckets.size(), best_b);
+      for (size_t j = 0; j < b->buckets.size(); ++j) {
+        double bj = 1e30;
+        for (int bs : {128, 256}) {
+          b->buckets[j].block = bs;
+          b->only_bucket = (int)j;
+          double us = 0;
+          const int st = time_launches(1, 4, &us);
+          b->only_bucket = -1;
+          if (st != XG_OK) return st;
+          if (us < bj) { bj = us; bb[j] = bs; }
+        }
+        b->buckets[j].block = bb[j];
+        total += bj;
+      }
+      total /= kk;
+      if (total < best_total) { best_total = total; best_kk = kk; best_blocks = bb; }
+    }
+    b->args.iters = best_kk;
+    for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = best_blocks[j];
+    best_k = best_kk;
+    best = best_total;
+  }
   b->drop_graph();
   b->tune_us = best;
-  if (result) { result[0] = best_b; result[1] = best_o; result[2] = best; result[3] = best_s; result[4] = best_k; }
+  if (result) {
+    result[0] = best_b; result[1] = best_o; result[2] = best; result[3] = best_s; result[4] = best_k;
+    for (int j = 0; j < 4; ++j)
+      result[5 + j] = j < (int)b->buckets.size() ? (b->buckets[j].block ? b->buckets[j].block : b->block) : 0;
+  }
   return XG_OK;
 }
 
@@ -1833,6 +1882,7 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
   if (block) {
     if (block != 64 && block != 128 && block != 256 && block != 512) return XG_E_ARG;
     b->block = block;
+    for (auto& bk : b->buckets) bk.block = 0;
   }
   if (split_eval >= 0) b->split = split_eval;
   if (iters_per_launch) b->args.iters = iters_per_launch;
