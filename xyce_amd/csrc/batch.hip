@@ -1191,6 +1191,11 @@ struct xg_batch {
   DArr<float> d_pred_out, d_pair_t, d_pair_m;
   BatchArgs args{};
   hipStream_t stream = nullptr;
+  // the LDS groups run side by side on streams of their own (default
+  // priority), forked from and joined back into `stream` at every launch
+  hipStream_t gstream[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t fork_ev = nullptr, join_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool concurrent_groups = false;   // chosen by the autotuner
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int block = 256;
   int only_bucket = -1;           // autotuner: launch one bucket only
@@ -1216,6 +1221,11 @@ struct xg_batch {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
+    for (int j = 0; j < 4; ++j) {
+      if (gstream[j]) (void)hipStreamDestroy(gstream[j]);
+      if (join_ev[j]) (void)hipEventDestroy(join_ev[j]);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
   }
   static size_t lds_bytes(int n, int nlu, int blob) {
     return sizeof(double) * (((size_t)3 * (n + 1) + nlu + 1) & ~(size_t)1) + sizeof(unsigned short) * (size_t)blob;
@@ -1260,14 +1270,25 @@ struct xg_batch {
       if (args.nt_mos > 0)
         hipLaunchKernelGGL(k_batch_eval, dim3((args.nt_mos + 255) / 256), dim3(256), 0, stream, args);
     }
+    const bool side = concurrent_groups && only_bucket < 0 && buckets.size() > 1 && buckets.size() <= 4;
+    if (side && hipEventRecord(fork_ev, stream) != hipSuccess) return XG_E_HIP;
     for (size_t j = 0; j < buckets.size(); ++j) {
       if (only_bucket >= 0 && (int)j != only_bucket) continue;
       const Bucket& bk = buckets[j];
       BatchArgs a = args;
       a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
       const int bs = bk.block ? bk.block : block;
-      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(bs), bk.shmem, stream, a);
-      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), bk.shmem, stream, a);
+      hipStream_t s = stream;
+      if (side) {
+        s = gstream[j];
+        if (hipStreamWaitEvent(s, fork_ev, 0) != hipSuccess) return XG_E_HIP;
+      }
+      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      if (side) {
+        if (hipEventRecord(join_ev[j], s) != hipSuccess) return XG_E_HIP;
+        if (hipStreamWaitEvent(stream, join_ev[j], 0) != hipSuccess) return XG_E_HIP;
+      }
     }
     if (pred_mode == 2) {
       const int st = mlp_forward_on_stream(net, pred_rows, d_feat.p, d_pred_out.p, stream);
@@ -1569,6 +1590,11 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   }
   const size_t NT = dev_off;
   if (hipStreamCreate(&b->stream) != hipSuccess) return XG_E_HIP;
+  for (int j = 0; j < 4; ++j) {
+    if (hipStreamCreate(&b->gstream[j]) != hipSuccess) return XG_E_HIP;
+    HIPCHK(hipEventCreateWithFlags(&b->join_ev[j], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
   HIPCHK(hipEventCreate(&b->ev0)); HIPCHK(hipEventCreate(&b->ev1));
   HIPCHK(b->d_ci.up(ci)); HIPCHK(b->d_cls.up(b->classes)); HIPCHK(b->d_inst.up(b->insts));
   HIPCHK(b->d_dev_inst.up(dev_inst)); HIPCHK(b->d_lim.zeros(n_inst));
@@ -1866,6 +1892,16 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = best_blocks[j];
     best_k = best_kk;
     best = best_total;
+    // the groups one after another, or side by side on their own streams
+    double seq = 0, con = 0;
+    b->concurrent_groups = false;
+    int st = time_launches(1, 4, &seq);
+    if (st != XG_OK) return st;
+    b->concurrent_groups = true;
+    st = time_launches(1, 4, &con);
+    if (st != XG_OK) return st;
+    b->concurrent_groups = con < seq;
+    best = std::min(seq, con) / best_kk;
   }
   b->drop_graph();
   b->tune_us = best;
