@@ -191,8 +191,10 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
  * the per-instance kernel, one Newton iteration per launch; 0: the
  * evaluation is fused into the per-instance kernel, which then runs
  * iters_per_launch iterations back to back with the instance's state and
- * program kept in LDS (0 = keep); -1 = keep. */
-int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval);
+ * program kept in LDS (0 = keep); -1 = keep.  concurrent_groups 1: the
+ * instances' LDS-sized groups run side by side on streams of their own, 0:
+ * one after another, -1 = keep. */
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups);
 /* Per-phase device timers of the first-scheduled instance (the counterpart of
  * the reference's Residual / Jacobian / Linear-solve StatTop timers,
  * src/UtilityPKG/N_UTL_Stats.h:481-511): enable = 1 resets and starts,

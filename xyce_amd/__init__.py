@@ -373,11 +373,12 @@ class Batch:
         return dict(pairs=int(c[0]), learned_steps=int(c[1]), learned_newton_failures=int(c[2]),
                     windows=int(c[3]), mode=int(c[4]))
 
-    def config(self, block=0, iters_per_launch=0, split_eval=-1):
+    def config(self, block=0, iters_per_launch=0, split_eval=-1, concurrent_groups=-1):
         """block: workgroup size (0 keep); split_eval 1: separate evaluation kernel, one iteration per
-        launch; 0: evaluation fused, iters_per_launch iterations per launch; -1 keep."""
-        _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch), C.c_int(split_eval)),
-               "xg_batch_config")
+        launch; 0: evaluation fused, iters_per_launch iterations per launch; concurrent_groups 1: the
+        LDS-sized instance groups side by side on their own streams; -1 keep."""
+        _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch), C.c_int(split_eval),
+                                     C.c_int(concurrent_groups)), "xg_batch_config")
 
     PHASES = ("x_load", "device_eval", "vector_gather_residual", "norms", "newton_decision", "jacobian_gather",
               "lu_refactor", "pivot_check", "triangular_solves", "update", "step_control")

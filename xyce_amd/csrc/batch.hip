@@ -1913,8 +1913,11 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   return XG_OK;
 }
 
-int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval) {
-  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 1) return XG_E_ARG;
+int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups) {
+  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 1 || concurrent_groups < -1 ||
+      concurrent_groups > 1)
+    return XG_E_ARG;
+  if (concurrent_groups >= 0) b->concurrent_groups = concurrent_groups == 1;
   if (block) {
     if (block != 64 && block != 128 && block != 256 && block != 512) return XG_E_ARG;
     b->block = block;
