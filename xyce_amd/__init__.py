@@ -394,10 +394,11 @@ class Batch:
         return out
 
     def autotune(self):
-        r = np.zeros(9)
+        r = np.zeros(10)
         _check(lib().xg_batch_autotune(self.h, _p(r)), "xg_batch_autotune")
         return dict(block=int(r[0]), order_mode=int(r[1]), us_per_iteration=float(r[2]), split_eval=int(r[3]),
-                    iters_per_launch=int(r[4]), group_blocks=[int(v) for v in r[5:9] if v > 0])
+                    iters_per_launch=int(r[4]), group_blocks=[int(v) for v in r[5:9] if v > 0],
+                    concurrent_groups=int(r[9]))
 
 
 class MLP:

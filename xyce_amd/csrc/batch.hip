@@ -534,11 +534,11 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const int g = I.dev_off + d;
       const double* k = kcls + (size_t)A.mcls[g] * KSTRIDE;
       double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
-      for (int j = 0; j < 6; ++j) volt[j] = xs[lid[6 * d + j]];
+      for (int j = 0; j < 6; ++j) volt[j] = xs[lid[j * C.n_mos + d]];
       for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
       for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
-      const int* tp = tpos + (size_t)d * R_COUNT;
-      auto emit = [&](int plane, double v) { const int p = tp[plane]; if (p >= 0) T[p] = v; };
+      // class arrays are plane-major ([plane][device]): a wavefront's loads are coalesced
+      auto emit = [&](int plane, double v) { const int p = tpos[(size_t)plane * C.n_mos + d]; if (p >= 0) T[p] = v; };
       lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
       for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
       for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
@@ -1070,15 +1070,18 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const bool dc = mode == 0;
   MosFlags fl;
   fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0; fl.newton_iter = newton_iter;
-  const int* lid = A.ci + C.o_lid + 6 * d;
-  const int* tp = A.ci + C.o_tpos + (size_t)d * R_COUNT;
+  const int* lid = A.ci + C.o_lid;
+  const int* tpos = A.ci + C.o_tpos;
   double* T = A.terms + I.term_off;
   const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
   double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
-  for (int j = 0; j < 6; ++j) volt[j] = lid[j] < n ? A.x[off + lid[j]] : 0.0;
+  for (int j = 0; j < 6; ++j) {
+    const int l = lid[j * C.n_mos + d];
+    volt[j] = l < n ? A.x[off + l] : 0.0;
+  }
   for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
   for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
-  auto emit = [&](int plane, double v) { const int p = tp[plane]; if (p >= 0) T[p] = v; };
+  auto emit = [&](int plane, double v) { const int p = tpos[(size_t)plane * C.n_mos + d]; if (p >= 0) T[p] = v; };
   const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
   for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
   for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
@@ -1405,7 +1408,14 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
             tpos[(size_t)d * R_COUNT + p] = p < R_JF0 ? vf[vgroup(d, p)]++ : mf[mgroup(d, p)]++;
           }
       }
-      C.o_tpos = push(tpos); C.o_vptr = push(vptr); C.o_mptr = push(mptr);
+      {
+        // plane-major copies for the device evaluation's coalesced loads
+        std::vector<int> tposT((size_t)hc.n_mos * R_COUNT);
+        for (int d = 0; d < hc.n_mos; ++d)
+          for (int p = 0; p < R_COUNT; ++p) tposT[(size_t)p * hc.n_mos + d] = tpos[(size_t)d * R_COUNT + p];
+        C.o_tpos = push(tposT);
+      }
+      C.o_vptr = push(vptr); C.o_mptr = push(mptr);
       {
         std::vector<int> lg;
         for (int g = 0; g < 4 * n && (int)lg.size() < kMaxLong; ++g) if (vptr[g + 1] - vptr[g] >= kLongSeg) lg.push_back(g);
@@ -1443,7 +1453,12 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         C.o_bgp = push(ptr); C.o_bge = push(ent);
         bgp_h = ptr; bge_h = ent;
       }
-      C.o_lid = push(hc.mos_lid);
+      {
+        std::vector<int> lidT((size_t)hc.n_mos * 6);
+        for (int d = 0; d < hc.n_mos; ++d)
+          for (int j = 0; j < 6; ++j) lidT[(size_t)j * hc.n_mos + d] = hc.mos_lid[(size_t)6 * d + j];
+        C.o_lid = push(lidT);
+      }
       C.nops = (int)sym.f_ops.size() / 2;
       // ---- uint16 program blob, staged into LDS by every launch
       {
@@ -1822,17 +1837,13 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
 
 int xg_batch_autotune(xg_batch* b, double* result) {
   if (!b) return XG_E_ARG;
-  // Candidates: kernel form (separate evaluation kernel + one iteration per
-  // launch, or fused with K iterations per launch) x workgroup size x
-  // instance grouping.  Each candidate runs a few real launches (they advance
-  // the simulation; every launch does valid Newton iterations) and the
-  // fastest time per batched iteration wins.  For the fused form with
-  // LDS-sized groups, each group then gets its own workgroup size and K is
-  // chosen from {4, 8, 16}.
+  // Each candidate runs a few real launches (they advance the simulation;
+  // every launch does valid Newton iterations); the fastest time per batched
+  // iteration wins.  Split form: workgroup size x instance grouping.  Fused
+  // form: workgroup size x grouping with 8 iterations per launch, then, with
+  // LDS-sized groups, each group's own workgroup size, the iterations per
+  // launch (4, 8, 16) and whether the groups run side by side.
   const int blocks[3] = {128, 256, 512};
-  const int fused_iters = b->args.iters > 1 ? b->args.iters : 8;
-  double best = 1e30;
-  int best_b = b->block, best_o = b->order_mode, best_s = b->split, best_k = b->args.iters;
   auto time_launches = [&](int warm, int timed, double* us) -> int {
     int st = xg_batch_advance(b, warm, 0);
     if (st != XG_OK) return st;
@@ -1841,36 +1852,39 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     *us = b->last_seconds / timed * 1e6;
     return XG_OK;
   };
-  for (auto& bk : b->buckets) bk.block = 0;
+  struct Cand { double us; int block, order, k; bool conc; std::vector<int> gblocks; };
+  Cand best_split{1e30, b->block, 1, 1, false, {}}, best_fused{1e30, b->block, 1, 8, false, {}};
+  b->concurrent_groups = false;
   for (int sp = 1; sp >= 0; --sp) {
+    Cand& bc = sp ? best_split : best_fused;
     for (int o = 0; o < 2; ++o) {
       b->set_order(o);
       for (int bs : blocks) {
         b->block = bs;
         b->split = sp;
-        b->args.iters = sp ? 1 : fused_iters;
+        b->args.iters = sp ? 1 : 8;
         b->drop_graph();
         double us = 0;
         const int st = time_launches(2, 8, &us);
         if (st != XG_OK) return st;
         us /= b->args.iters;
-        if (us < best) { best = us; best_b = bs; best_o = o; best_s = sp; best_k = b->args.iters; }
+        if (us < bc.us) { bc.us = us; bc.block = bs; bc.order = o; bc.k = b->args.iters; }
       }
     }
   }
-  b->block = best_b;
-  b->split = best_s;
-  b->args.iters = best_k;
-  b->set_order(best_o);
-  if (!best_s && best_o == 1) {
-    // fused, grouped: per-group workgroup size, then iterations per launch
+  // fused, grouped by LDS need: per-group workgroup size, iterations per
+  // launch, concurrency
+  b->split = 0;
+  b->block = best_fused.block;
+  b->set_order(1);
+  {
     double best_total = 1e30;
-    int best_kk = best_k;
-    std::vector<int> best_blocks(b->buckets.size(), best_b);
+    int best_kk = 8;
+    std::vector<int> best_blocks(b->buckets.size(), best_fused.block);
     for (int kk : {4, 8, 16}) {
       b->args.iters = kk;
       double total = 0;
-      std::vector<int> bb(b->buckets.size(), best_b);
+      std::vector<int> bb(b->buckets.size(), best_fused.block);
       for (size_t j = 0; j < b->buckets.size(); ++j) {
         double bj = 1e30;
         for (int bs : {128, 256}) {
@@ -1890,9 +1904,6 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     }
     b->args.iters = best_kk;
     for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = best_blocks[j];
-    best_k = best_kk;
-    best = best_total;
-    // the groups one after another, or side by side on their own streams
     double seq = 0, con = 0;
     b->concurrent_groups = false;
     int st = time_launches(1, 4, &seq);
@@ -1900,15 +1911,28 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     b->concurrent_groups = true;
     st = time_launches(1, 4, &con);
     if (st != XG_OK) return st;
-    b->concurrent_groups = con < seq;
-    best = std::min(seq, con) / best_kk;
+    const double grouped = std::min(seq, con) / best_kk;
+    if (grouped < best_fused.us) {
+      best_fused = Cand{grouped, best_fused.block, 1, best_kk, con < seq, best_blocks};
+    }
   }
+  // the winner
+  const bool fused = best_fused.us < best_split.us;
+  const Cand& w = fused ? best_fused : best_split;
+  b->split = fused ? 0 : 1;
+  b->block = w.block;
+  b->args.iters = w.k;
+  b->set_order(w.order);
+  b->concurrent_groups = w.conc;
+  if (!w.gblocks.empty() && w.gblocks.size() == b->buckets.size())
+    for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = w.gblocks[j];
   b->drop_graph();
-  b->tune_us = best;
+  b->tune_us = w.us;
   if (result) {
-    result[0] = best_b; result[1] = best_o; result[2] = best; result[3] = best_s; result[4] = best_k;
+    result[0] = w.block; result[1] = w.order; result[2] = w.us; result[3] = b->split; result[4] = w.k;
     for (int j = 0; j < 4; ++j)
       result[5 + j] = j < (int)b->buckets.size() ? (b->buckets[j].block ? b->buckets[j].block : b->block) : 0;
+    result[9] = b->concurrent_groups ? 1 : 0;
   }
   return XG_OK;
 }
