@@ -160,8 +160,21 @@ struct SparseLU {
   }
 };
 
+// Hooks through which a caller's device loads and linear solver stand in for
+// the oracle's own, so that this Newton / time-integration loop (the
+// restatement of the reference's driver) drives them: the arguments are
+// those of orc_load and orc_lu_solve.
+typedef int (*OrcLoadHook)(void* ctx, const int* ssi, const double* ssd, const double* x, const double* sta_cur,
+                           const double* sto_cur, double* sta_next, double* sto_next, double* F, double* Q, double* B,
+                           double* dFdxdVp, double* dQdxdVp, double* dFdx, double* dQdx);
+typedef int (*OrcSolveHook)(void* ctx, int n, const int* rowptr, const int* colind, const double* vals,
+                            const double* b, double* x);
+
 struct Work {
   Circuit* c;
+  OrcLoadHook load_hook = nullptr;
+  OrcSolveHook solve_hook = nullptr;
+  void* hook_ctx = nullptr;
   int n, nnz;
   std::vector<double> x, F, Q, B, dFdxdVp, dQdxdVp, RHS, dFdx, dQdx, J;
   std::vector<double> staCur, staNext, stoCur, stoNext;
@@ -204,6 +217,16 @@ static void load_linear_matrices(Circuit& c) {
 static void load_rhs(Work& w, const SolverState& ss) {
   Circuit& c = *w.c;
   const int n = w.n;
+  if (w.load_hook) {
+    // the caller's loads: vectors and matrices at this x, as orc_load returns them
+    const int ssi[6] = {ss.dcop, ss.tran, ss.initJct, ss.initFix, ss.newtonIter, ss.order};
+    const double ssd[4] = {ss.h, ss.time, w.tstep, w.tstop};
+    w.x[n] = 0.0;
+    w.limited = w.load_hook(w.hook_ctx, ssi, ssd, w.x.data(), w.staCur.data(), w.stoCur.data(), w.staNext.data(),
+                            w.stoNext.data(), w.F.data(), w.Q.data(), w.B.data(), w.dFdxdVp.data(), w.dQdxdVp.data(),
+                            w.dFdx.data(), w.dQdx.data());
+    return;
+  }
   std::fill(w.F.begin(), w.F.end(), 0.0); std::fill(w.Q.begin(), w.Q.end(), 0.0);
   std::fill(w.B.begin(), w.B.end(), 0.0);
   std::fill(w.dFdxdVp.begin(), w.dFdxdVp.end(), 0.0); std::fill(w.dQdxdVp.begin(), w.dQdxdVp.end(), 0.0);
@@ -239,6 +262,7 @@ static void load_rhs(Work& w, const SolverState& ss) {
 // NonlinearEquationLoader::loadJacobian (:514-558): zero, nonlinear device
 // matrices, then FilteredMatrix::addToMatrix of the linear part (:632).
 static void load_jacobian(Work& w, const SolverState& ss) {
+  if (w.load_hook) return;   // loaded with the vectors at the same x
   Circuit& c = *w.c;
   std::fill(w.dFdx.begin(), w.dFdx.end(), 0.0); std::fill(w.dQdx.begin(), w.dQdx.end(), 0.0);
   for (auto& mi : c.mos) mos1_matrices_ext(mi, ss, w.dFdx.data(), w.dQdx.data());
@@ -373,7 +397,9 @@ static NewtonStats newton(Work& w, SolverState ss, const NLP& p, const std::vect
     nlStep++;
     load_jacobian(w, ss);
     combine_jacobian(w, ss);
-    int ls = lu_solve_sparse(w.slu, n, w.c->rowptr.data(), w.c->colind.data(), w.J.data(), w.RHS.data(), dx.data());
+    int ls = w.solve_hook
+                 ? w.solve_hook(w.hook_ctx, n, w.c->rowptr.data(), w.c->colind.data(), w.J.data(), w.RHS.data(), dx.data())
+                 : lu_solve_sparse(w.slu, n, w.c->rowptr.data(), w.c->colind.data(), w.J.data(), w.RHS.data(), dx.data());
     for (int i = 0; i < n; ++i) w.x[i] += dx[i];
     rhs(nlStep);
     normRHS = two_norm(w.RHS, n);
@@ -811,6 +837,13 @@ int orc_dcop(OrcHandle* h, double* x, double* sta, double* sto, double* Q, doubl
 }
 
 // Full transient.  stats: [steps, rejects, newton_iters, loads, seconds, status, npoints]
+int orc_set_hooks(OrcHandle* h, OrcLoadHook load, OrcSolveHook solve, void* ctx) {
+  h->w->load_hook = load;
+  h->w->solve_hook = solve;
+  h->w->hook_ctx = ctx;
+  return 0;
+}
+
 // Transient tolerances (a tighter run is the accurate reference that the
 // tests compare runs with different Newton paths against).
 int orc_set_tolerances(OrcHandle* h, double tia_reltol, double tia_abstol, double nl_reltol, double nl_abstol) {

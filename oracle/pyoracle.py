@@ -131,6 +131,17 @@ class Oracle:
         st = lib().orc_dcop(self.h, _p(x), _p(sta), _p(sto), _p(Q), _p(F), _p(B))
         return st, x, sta[: self.nsta], sto[: self.nsto], Q, F, B
 
+    LOAD_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_int), *([C.POINTER(C.c_double)] * 13))
+    SOLVE_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+    def set_hooks(self, load=None, solve=None):
+        """Let this oracle's Newton / time-integration loop (the restatement of
+        the reference's driver) call ``load`` and ``solve`` (Python callables
+        with orc_load / orc_lu_solve arguments) instead of its own loads and LU."""
+        self._hooks = (self.LOAD_HOOK(load) if load else self.LOAD_HOOK(), self.SOLVE_HOOK(solve) if solve else self.SOLVE_HOOK())
+        lib().orc_set_hooks(self.h, self._hooks[0], self._hooks[1], None)
+
     def set_tolerances(self, tia_reltol=1e-3, tia_abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
         lib().orc_set_tolerances(self.h, C.c_double(tia_reltol), C.c_double(tia_abstol), C.c_double(nl_reltol),
                                  C.c_double(nl_abstol))

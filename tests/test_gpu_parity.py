@@ -287,3 +287,66 @@ C2 out 0 5f
         col = gid[an["nodes"][nm]]
         err = waveform_error(t, cols[f"V({nm})"][:, None], r["t"], r["x"][:, [col]])
         assert err <= 1e-3 * 1.8 + 1e-6, (nm, err)
+
+
+def test_reference_driver_drives_the_dropin(gpu):
+    """The drop-in boundary end to end: the oracle's restatement of the
+    reference's driver (DampedNewton, OneStep, DC operating point) calls the
+    GPU device loads (xg_update_state / xg_load_vectors / xg_load_matrices)
+    and the GPU LU (xg_lu_numeric / xg_lu_solve) through the C ABI at every
+    Newton iteration, as Xyce's DeviceMgr and Linear::Solver would be called;
+    the run reproduces the all-CPU oracle run."""
+    c = C.ring(5, seed=5)
+    tstop = 0.4e-9
+    ref = Oracle(c).tran(tstop, 1e-12, 1e-11)
+    assert ref["status"] == 0
+    drv = Oracle(c)
+    dm = X.DeviceMgr(c)
+    p = dm.pattern()
+    ls = X.DirectSolver(dm.n, p["rowptr"], p["colind"])
+    n, nnz, nsta, nsto = dm.n, dm.nnz, dm.nsta, dm.nsto
+    arr = lambda ptr, k: np.ctypeslib.as_array(ptr, shape=(max(k, 1),))
+    errors = []
+    calls = {"load": 0, "solve": 0}
+
+    def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
+        try:
+            si, sd = arr(ssi, 6), arr(ssd, 4)
+            dm.set_solver_state(dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
+                                order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]),
+                                tstop=float(sd[3]))
+            sta_n, sto_n, lim = dm.updateState(arr(x, n)[:n].copy(), arr(sc, nsta)[:nsta].copy(),
+                                               arr(oc, nsto)[:nsto].copy(), sto_next=arr(on, nsto)[:nsto].copy())
+            arr(sn, nsta)[:nsta] = sta_n
+            arr(on, nsto)[:nsto] = sto_n
+            v = dm.loadDAEVectors()
+            for name, ptr in (("F", F), ("Q", Q), ("B", B), ("dFdxdVp", dFv), ("dQdxdVp", dQv)):
+                arr(ptr, n)[:n] = v[name]
+            dF, dQ = dm.loadDAEMatrices()
+            arr(dFdx, nnz)[:nnz] = dF
+            arr(dQdx, nnz)[:nnz] = dQ
+            calls["load"] += 1
+            return int(lim)
+        except Exception as e:      # surfaced after the run
+            errors.append(repr(e))
+            return 0
+
+    def solve(ctx, nn, rp, ci, vals, b, x):
+        try:
+            st, xs = ls.doSolve(arr(vals, nnz)[:nnz].copy(), arr(b, nn)[:nn].copy())
+            arr(x, nn)[:nn] = xs
+            calls["solve"] += 1
+            return int(st)
+        except Exception as e:
+            errors.append(repr(e))
+            return -1
+
+    drv.set_hooks(load, solve)
+    r = drv.tran(tstop, 1e-12, 1e-11)
+    assert not errors, errors[:3]
+    assert r["status"] == 0
+    print(f"drop-in run: {calls['load']} loads, {calls['solve']} solves; steps {r['steps']} vs {ref['steps']}")
+    assert calls["solve"] == r["newton_iters"] or calls["solve"] >= r["newton_iters"]
+    assert abs(r["steps"] - ref["steps"]) <= 0.02 * ref["steps"] + 2
+    err = waveform_error(r["t"], r["x"], ref["t"], ref["x"])
+    assert err <= 1e-3 * 1.8 + 1e-6, err

@@ -9,7 +9,9 @@ step control and sparse LU against closed forms and library routines.
 * a linear circuit converges in one Newton solve per step once the
   predictor is exact to the tolerance, and never needs more than two;
 * the sparse LU (min-degree + threshold partial pivoting) against numpy on
-  ring Jacobians and random diagonally weak matrices.
+  ring Jacobians and random diagonally weak matrices;
+* the driver hooks (caller-provided loads and LU) carry exactly what the
+  oracle's own would.
 """
 import numpy as np
 import pytest
@@ -130,3 +132,44 @@ def test_sparse_lu_matches_numpy(seed):
     xn = np.linalg.solve(A, b)
     assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b) * np.linalg.cond(A) ** 0.5
     np.testing.assert_allclose(x, xn, rtol=1e-8, atol=1e-10 * np.max(np.abs(xn)))
+
+
+def test_driver_hooks_with_oracle_loads_reproduce_the_run():
+    """The load / solve hooks (through which the GPU drop-in is driven in the
+    GPU tests) carry exactly what the oracle's own loads and LU would: a run
+    whose hooks call a second oracle's orc_load and LU is bit-identical."""
+    c = C.ring(5, seed=5)
+    ref = Oracle(c).tran(0.3e-9, 1e-12, 1e-11)
+    inner = Oracle(c)
+    drv = Oracle(c)
+    n, nnz, nsta, nsto = inner.n, inner.nnz, inner.nsta, inner.nsto
+    arr = lambda ptr, k: np.ctypeslib.as_array(ptr, shape=(max(k, 1),))
+    p = inner.pattern()
+
+    def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
+        si, sd = arr(ssi, 6), arr(ssd, 4)
+        out = inner.load(arr(x, n)[:n].copy(), arr(sc, nsta)[:nsta].copy(), arr(oc, nsto)[:nsto].copy(),
+                         sta_next=arr(sn, nsta)[:nsta].copy(), sto_next=arr(on, nsto)[:nsto].copy(),
+                         dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
+                         order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]), tstop=float(sd[3]))
+        arr(sn, nsta)[:nsta] = out["sta_next"][:nsta]
+        arr(on, nsto)[:nsto] = out["sto_next"][:nsto]
+        for name, ptr in (("F", F), ("Q", Q), ("B", B), ("dFdxdVp", dFv), ("dQdxdVp", dQv)):
+            arr(ptr, n)[:n] = out[name]
+        arr(dFdx, nnz)[:nnz] = out["dFdx"]
+        arr(dQdx, nnz)[:nnz] = out["dQdx"]
+        return int(out["limited"])
+
+    from oracle.pyoracle import lib as olib, _p
+
+    def solve(ctx, nn, rp, ci, vals, b, x):
+        xs = np.zeros(nn)
+        st = olib().orc_lu_solve_sparse(nn, _p(p["rowptr"]), _p(p["colind"]),
+                                        _p(arr(vals, nnz)[:nnz].copy()), _p(arr(b, nn)[:nn].copy()), _p(xs))
+        arr(x, nn)[:nn] = xs
+        return int(st)
+
+    drv.set_hooks(load, solve)
+    r = drv.tran(0.3e-9, 1e-12, 1e-11)
+    assert r["status"] == 0 and r["steps"] == ref["steps"] and r["newton_iters"] == ref["newton_iters"]
+    np.testing.assert_array_equal(r["x"], ref["x"])
