@@ -76,11 +76,11 @@ class SolverState(C.Structure):
 
 
 def lib():
-    """Load libxyce_amd.so (built by ``__graft_entry__.build()``)."""
+    """Load libxyce_amd.so (built by ``make -C xyce_amd/csrc``)."""
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
-            raise XgError(f"HIP library missing: {LIB_PATH}; run __graft_entry__.build()")
+            raise XgError(f"HIP library missing: {LIB_PATH}; build it with make -C xyce_amd/csrc")
         L = C.CDLL(LIB_PATH)
         for name in EXPORTS:
             getattr(L, name).restype = C.c_int

@@ -83,7 +83,8 @@ struct InstState {
   int n_acc, feat_valid, pred_skip, used_learned;
   long long n_pairs, n_learned, n_learned_rejects;
   // Newton iterations per step started from each predictor (decaying
-  // averages): the selector that lets Newton convergence judge the network
+  // averages): Newton convergence decides between the network and the
+  // polynomial predictor
   double it_learned, st_learned, it_poly, st_poly;
   long long n_choice;
   double pad[22];
