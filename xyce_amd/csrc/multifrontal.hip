@@ -167,6 +167,10 @@ struct MfDev {
   const int* cptr; const int* clist; const int* eptr; const int* emap; const int* vo;
 };
 
+constexpr int kMfTiny = 64;     // fronts up to this order: one wavefront, the front in LDS
+constexpr int kMfPanel = 32;    // panel width of the blocked factorisation
+constexpr int kMfSolveBlock = 512;
+
 template <class T>
 __global__ void k_mf_fill(T* F, long long total) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,6 +181,24 @@ template <class T>
 __global__ void k_mf_scatter(int nnz, const double* vals, const long long* amap, T* F) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nnz) F[amap[k]] += (T)vals[k];
+}
+
+// value of v in lane `src` (wave-uniform) of the wavefront
+__device__ __forceinline__ double mf_bcast(double v, int src) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffull), src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float mf_bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+template <class T>
+__device__ __forceinline__ T mf_wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
 // extend-add of one child's Schur complement into its parent: block.x picks
@@ -209,135 +231,194 @@ __device__ __forceinline__ bool mf_bad_pivot(T d) {
   return !(fabs(v) > 0.0) || !isfinite(v);
 }
 
-// small fronts: one workgroup per front, unblocked right-looking over the
-// pivots, the whole front updated (its trailing block is the Schur complement)
+// fronts of order <= 64: one wavefront per front, the front staged in LDS
+// (row stride m + 1), right-looking over its pivots with lane i owning row i
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_factor_small(MfDev d, const int* list, T* F, int* bad) {
+__global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list, T* F, int* bad) {
+  extern __shared__ unsigned char mf_sm[];
+  T* A = reinterpret_cast<T*>(mf_sm);
   const int s = list[blockIdx.x];
   const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const int ld = ms + 1;
   T* Fs = F + d.fo[s];
-  __shared__ int flag;
-  if (threadIdx.x == 0) flag = 0;
+  const int t = threadIdx.x;
+  for (int i = 0; i < ms; ++i)
+    if (t < ms) A[i * ld + t] = Fs[(long long)i * ms + t];
   __syncthreads();
+  bool fail = false;
   for (int k = 0; k < np; ++k) {
-    const T piv = Fs[(long long)k * ms + k];
-    if (mf_bad_pivot(piv)) { if (threadIdx.x == 0) flag = 1; break; }
-    for (int i = k + 1 + threadIdx.x; i < ms; i += blockDim.x) {
-      const T l = Fs[(long long)i * ms + k] / piv;
-      Fs[(long long)i * ms + k] = l;
-      if (!(fabs((double)l) <= 1.0e3)) flag = 1;
-    }
-    __syncthreads();
-    const int w = ms - k - 1;
-    for (long long e = threadIdx.x; e < (long long)w * w; e += blockDim.x) {
-      const int i = k + 1 + (int)(e / w), j = k + 1 + (int)(e % w);
-      Fs[(long long)i * ms + j] -= Fs[(long long)i * ms + k] * Fs[(long long)k * ms + j];
+    const T piv = A[k * ld + k];
+    if (mf_bad_pivot(piv)) { fail = true; break; }
+    if (t > k && t < ms) {
+      T* ri = A + t * ld;
+      const T* rk = A + k * ld;
+      const T l = ri[k] / piv;
+      ri[k] = l;
+      if (!(fabs((double)l) <= 1.0e3)) fail = true;
+      int j = k + 1;
+      for (; j + 8 <= ms; j += 8) {
+        T u[8], a[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u[q] = rk[j + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a[q] = ri[j + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ri[j + q] = a[q] - l * u[q];
+      }
+      for (; j < ms; ++j) ri[j] -= l * rk[j];
     }
     __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && flag) atomicMin(bad, d.sfirst[s] + 1);
+  for (int i = 0; i < ms; ++i)
+    if (t < ms) Fs[(long long)i * ms + t] = A[i * ld + t];
+  if (__any(fail) && t == 0) atomicMin(bad, d.sfirst[s] + 1);
 }
 
-// large fronts, blocked by panels of pw <= 32 columns
-// (1) one launch per panel: every workgroup (one wavefront) factors the
-// pw x pw diagonal block in LDS (workgroup 0 also stores the factors in
-// `dbuf`, which the update launch copies into the front: the other
-// workgroups may still be reading the unfactored block), then solves a 64-row block of L21 (x U11 = row) or a 64-column
-// block of U12 (L11 x = column), its tile staged through LDS so the global
-// loads and stores are coalesced.
+// larger fronts, blocked by panels of kMfPanel columns; all fronts of a level
+// in one launch per panel index (entries {front, block, dbuf slot, 0}).
+// (1) every workgroup (one wavefront) factors the pw x pw diagonal block in
+// registers (lane t owns row t, the pivot row broadcast across lanes), then
+// solves a 64-row block of L21 (x U11 = row) or a 64-column block of U12
+// (L11 x = column).  Block 0 also stores the block's factors in its dbuf
+// slot; the update launch copies them into the front (the front's other
+// workgroups may still be reading the unfactored block).
 template <class T>
-__global__ void __launch_bounds__(64) k_mf_panel_solve(T* Fs, int ms, int p0, int pw, int nrb, int col0, T* dbuf,
-                                                       int* bad) {
-  __shared__ T D[32][33];
-  __shared__ T Tl[64][33];
-  __shared__ int flag;
+__global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p0, T* F, T* dbuf, int* bad) {
+  __shared__ T D[kMfPanel][kMfPanel + 1];
+  __shared__ T Tl[64][kMfPanel + 1];
+  const int4 e = ent[blockIdx.x];
+  const int s = e.x, blk = e.y, slot = e.z;
+  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const int pw = min(kMfPanel, np - p0);
+  const int base = p0 + pw, rest = ms - base, nrb = (rest + 63) / 64;
+  T* Fs = F + d.fo[s];
   const int t = threadIdx.x;
-  if (t == 0) flag = 0;
-  for (int e = t; e < pw * pw; e += 64) D[e / pw][e % pw] = Fs[(long long)(p0 + e / pw) * ms + p0 + e % pw];
-  __syncthreads();
-  for (int k = 0; k < pw; ++k) {
-    const T piv = D[k][k];
-    if (mf_bad_pivot(piv)) { if (t == 0) flag = 1; break; }
-    if (t > k && t < pw) {
-      const T l = D[t][k] / piv;
-      D[t][k] = l;
-      if (!(fabs((double)l) <= 1.0e3)) flag = 1;
-      for (int j = k + 1; j < pw; ++j) D[t][j] -= l * D[k][j];
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  const int base = p0 + pw;
-  if (blockIdx.x == 0) {
-    for (int e = t; e < pw * pw; e += 64) dbuf[e] = D[e / pw][e % pw];
-  }
-  if ((int)blockIdx.x < nrb) {
-    // L21 rows [r0, r0 + 64)
-    const int r0 = base + blockIdx.x * 64;
-    for (int e = t; e < 64 * pw; e += 64) {
-      const int rr = e / pw, q = e % pw;
-      Tl[rr][q] = r0 + rr < ms ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
-    }
-    __syncthreads();
-    if (r0 + t < ms) {
-      for (int q = 0; q < pw; ++q) {
-        T a = Tl[t][q];
-        for (int r = 0; r < q; ++r) a -= Tl[t][r] * D[r][q];
-        a /= D[q][q];
-        if (!(fabs((double)a) <= 1.0e3)) flag = 1;
-        Tl[t][q] = a;
+  T r[kMfPanel];
+#pragma unroll
+  for (int q = 0; q < kMfPanel; ++q)
+    r[q] = (t < pw && q < pw) ? Fs[(long long)(p0 + t) * ms + p0 + q] : (T)0;
+  // (loops keep constant trip counts so that the register arrays stay
+  // register-resident after full unrolling)
+  bool fail = false, stop = false;
+#pragma unroll
+  for (int k = 0; k < kMfPanel; ++k) {
+    if (k < pw && !stop) {
+      const T piv = mf_bcast(r[k], k);
+      if (mf_bad_pivot(piv)) {
+        fail = true;
+        stop = true;
+      } else {
+        const bool upd = t > k && t < pw;
+        const T l = upd ? r[k] / piv : (T)0;
+        if (upd) {
+          r[k] = l;
+          if (!(fabs((double)l) <= 1.0e3)) fail = true;
+        }
+#pragma unroll
+        for (int q = 0; q < kMfPanel; ++q) {
+          if (q > k) {
+            const T u = mf_bcast(r[q], k);
+            if (upd) r[q] -= l * u;
+          }
+        }
       }
     }
-    __syncthreads();
-    for (int e = t; e < 64 * pw; e += 64) {
-      const int rr = e / pw, q = e % pw;
-      if (r0 + rr < ms) Fs[(long long)(r0 + rr) * ms + p0 + q] = Tl[rr][q];
-    }
-  } else {
-    // U12 columns [c0, c0 + 64)
-    const int c0 = base + (blockIdx.x - nrb) * 64;
-    for (int e = t; e < 64 * pw; e += 64) {
-      const int q = e / 64, cc = e % 64;
-      Tl[cc][q] = c0 + cc < ms ? Fs[(long long)(p0 + q) * ms + c0 + cc] : (T)0;
-    }
-    __syncthreads();
-    for (int r = 1; r < pw; ++r) {
-      T a = Tl[t][r];
-      for (int q = 0; q < r; ++q) a -= D[r][q] * Tl[t][q];
-      Tl[t][r] = a;
-    }
-    __syncthreads();
-    for (int e = t; e < 64 * pw; e += 64) {
-      const int q = e / 64, cc = e % 64;
-      if (c0 + cc < ms) Fs[(long long)(p0 + q) * ms + c0 + cc] = Tl[cc][q];
-    }
+  }
+  if (t < kMfPanel) {
+#pragma unroll
+    for (int q = 0; q < kMfPanel; ++q) D[t][q] = r[q];
   }
   __syncthreads();
-  if (t == 0 && flag && (blockIdx.x == 0 || (int)blockIdx.x < nrb)) atomicMin(bad, col0 + p0 + 1);
+  if (blk == 0) {
+    T* db = dbuf + (long long)slot * kMfPanel * kMfPanel;
+    for (int i = 0; i < pw; ++i)
+      if (t < pw) db[i * kMfPanel + t] = D[i][t];
+  }
+  if (rest > 0 && blk < nrb) {
+    // L21 rows [r0, r0 + 64), staged through LDS for coalesced access
+    const int r0 = base + blk * 64;
+    for (int it = 0; it < kMfPanel; ++it) {
+      const int rr = (t >> 5) + 2 * it, q = t & 31;
+      Tl[rr][q] = (r0 + rr < ms && q < pw) ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
+    }
+    __syncthreads();
+    T a[kMfPanel];
+#pragma unroll
+    for (int q = 0; q < kMfPanel; ++q) a[q] = Tl[t][q];
+    const bool valid = r0 + t < ms;
+#pragma unroll
+    for (int k = 0; k < kMfPanel; ++k) {
+      if (k < pw) {
+        a[k] = a[k] / D[k][k];
+        if (valid && !(fabs((double)a[k]) <= 1.0e3)) fail = true;
+#pragma unroll
+        for (int q = 0; q < kMfPanel; ++q)
+          if (q > k) a[q] -= a[k] * D[k][q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kMfPanel; ++q) Tl[t][q] = a[q];
+    __syncthreads();
+    for (int it = 0; it < kMfPanel; ++it) {
+      const int rr = (t >> 5) + 2 * it, q = t & 31;
+      if (r0 + rr < ms && q < pw) Fs[(long long)(r0 + rr) * ms + p0 + q] = Tl[rr][q];
+    }
+  } else if (rest > 0) {
+    // U12 columns [c0, c0 + 64): lane t owns column c0 + t
+    const int c = base + (blk - nrb) * 64 + t;
+    const bool valid = c < ms;
+    T b[kMfPanel];
+#pragma unroll
+    for (int q = 0; q < kMfPanel; ++q) b[q] = (valid && q < pw) ? Fs[(long long)(p0 + q) * ms + c] : (T)0;
+#pragma unroll
+    for (int k = 0; k < kMfPanel; ++k) {
+      if (k < pw) {
+#pragma unroll
+        for (int q = 0; q < kMfPanel; ++q)
+          if (q > k) b[q] -= D[q][k] * b[k];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kMfPanel; ++q)
+      if (valid && q < pw) Fs[(long long)(p0 + q) * ms + c] = b[q];
+  }
+  if (__any(fail) && t == 0) atomicMin(bad, d.sfirst[s] + p0 + 1);
 }
 
-// (2) trailing update F[i][j] -= sum_q L[i][q] U[q][j], i, j >= p0 + pw,
-// 64x64 tiles, the panel's L rows and U columns staged in LDS
+// (2) trailing update F[i][j] -= sum_q L[i][q] U[q][j], i, j >= p0 + pw, one
+// 64x64 tile per workgroup (entries {front, tile row, tile column, dbuf
+// slot}; tile row -1: the front has nothing below the panel), the panel's L
+// rows and U columns staged in LDS.  Tile (0, 0) or the -1 entry copies the
+// panel's diagonal-block factors from dbuf into the front.
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_update(T* Fs, int ms, int p0, int pw, const T* dbuf) {
+__global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int p0, T* F, const T* dbuf) {
   constexpr int TB = 64, KB = 16;
   __shared__ T Ls[TB][KB + 1];
   __shared__ T Us[KB][TB + 1];
+  const int4 e = ent[blockIdx.x];
+  const int s = e.x;
+  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const int pw = min(kMfPanel, np - p0);
   const int base = p0 + pw;
-  if (blockIdx.x == 0 && blockIdx.y == 0)    // the panel's diagonal block factors
-    for (int e = threadIdx.x; e < pw * pw; e += blockDim.x) Fs[(long long)(p0 + e / pw) * ms + p0 + e % pw] = dbuf[e];
-  if (base >= ms) return;
-  const int i0 = base + blockIdx.y * TB, j0 = base + blockIdx.x * TB;
+  T* Fs = F + d.fo[s];
+  if (e.y <= 0 && e.z <= 0) {
+    const T* db = dbuf + (long long)e.w * kMfPanel * kMfPanel;
+    for (int q = threadIdx.x; q < pw * kMfPanel; q += blockDim.x) {
+      const int i = q >> 5, j = q & 31;
+      if (j < pw) Fs[(long long)(p0 + i) * ms + p0 + j] = db[q];
+    }
+  }
+  if (e.y < 0) return;
+  const int i0 = base + e.y * TB, j0 = base + e.z * TB;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4 x 4 outputs each
   T acc[4][4];
   for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
   for (int q0 = 0; q0 < pw; q0 += KB) {
-    for (int e = threadIdx.x; e < TB * KB; e += 256) {
-      const int r = e / KB, q = e % KB;
+    for (int x = threadIdx.x; x < TB * KB; x += 256) {
+      const int r = x / KB, q = x % KB;
       const int gi = i0 + r, gq = p0 + q0 + q;
       Ls[r][q] = (gi < ms && q0 + q < pw) ? Fs[(long long)gi * ms + gq] : (T)0;
-      const int qq = e / TB, c = e % TB;
+      const int qq = x / TB, c = x % TB;
       const int gj = j0 + c, gq2 = p0 + q0 + qq;
       Us[qq][c] = (gj < ms && q0 + qq < pw) ? Fs[(long long)gq2 * ms + gj] : (T)0;
     }
@@ -360,55 +441,174 @@ __global__ void __launch_bounds__(256) k_mf_update(T* Fs, int ms, int p0, int pw
   }
 }
 
-// forward solve of a level: V_s = b(pivot rows) + children's updates, then
-// y = L11^-1 V (unit), V(rows below) -= L21 y; y written to yg
+// ---- solves.  V holds, per front, the vector of its index list: y on the
+// pivot rows, the update for the parent below them.
+
+// sum the children's updates into the front's vector Vl (LDS)
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_fwd(MfDev d, const int* list, const T* F, T* V, const double* b, T* yg) {
-  const int s = list[blockIdx.x];
-  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
-  const T* Fs = F + d.fo[s];
-  T* Vs = V + d.vo[s];
-  const int* ix = d.idx + d.iptr[s];
-  for (int t = threadIdx.x; t < ms; t += blockDim.x) Vs[t] = t < np ? (T)b[ix[t]] : (T)0;
-  __syncthreads();
+__device__ __forceinline__ void mf_gather_children(const MfDev& d, int s, const T* V, T* Vl) {
   for (int c = d.cptr[s]; c < d.cptr[s + 1]; ++c) {
     const int ch = d.clist[c];
     const int npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = d.m[ch] - npc;
     const T* Vc = V + d.vo[ch];
     const int* em = d.emap + d.eptr[ch];
-    for (int a = threadIdx.x; a < nu; a += blockDim.x) Vs[em[a]] += Vc[npc + a];
+    for (int a = threadIdx.x; a < nu; a += blockDim.x) Vl[em[a]] += Vc[npc + a];
     __syncthreads();
   }
-  for (int k = 0; k < np; ++k) {
-    const T yk = Vs[k];
-    for (int i = k + 1 + threadIdx.x; i < ms; i += blockDim.x) Vs[i] -= Fs[(long long)i * ms + k] * yk;
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < np; t += blockDim.x) yg[ix[t]] = Vs[t];
 }
 
-// backward solve of a level (roots first): x = U11^-1 (y - U12 x_below)
+// forward, fronts of order <= 64: one wavefront, lane i owns row i
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_bwd(MfDev d, const int* list, const T* F, T* V, T* x) {
+__global__ void __launch_bounds__(64) k_mf_fwd_tiny(MfDev d, const int* list, const T* F, T* V, const double* b,
+                                                    T* yg) {
+  __shared__ T Vl[kMfTiny];
   const int s = list[blockIdx.x];
   const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
   const T* Fs = F + d.fo[s];
-  T* Vs = V + d.vo[s];
   const int* ix = d.idx + d.iptr[s];
-  for (int t = threadIdx.x; t < np; t += blockDim.x) {
-    T v = x[ix[t]];                 // holds y on entry
-    for (int u = np; u < ms; ++u) v -= Fs[(long long)t * ms + u] * x[ix[u]];
-    Vs[t] = v;
+  const int t = threadIdx.x;
+  Vl[t] = t < np ? (T)b[ix[t]] : (T)0;
+  __syncthreads();
+  mf_gather_children(d, s, V, Vl);
+  T v = Vl[t];
+  for (int k = 0; k < np; ++k) {
+    const T yk = mf_bcast(v, k);
+    if (t > k && t < ms) v -= Fs[(long long)t * ms + k] * yk;
+  }
+  if (t < ms) V[d.vo[s] + t] = v;
+  if (t < np) yg[ix[t]] = v;
+}
+
+// forward, larger fronts: one workgroup per front, the vector in LDS; the
+// unit-lower L11 by 64-row blocks (rows updated by a row-wise product with the
+// solved part, then the block's triangle solved by one wavefront from LDS),
+// then the rows below the pivots updated by L21 y
+template <class T>
+__global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int* list, const T* F, T* V,
+                                                             const double* b, T* yg, int ldsv) {
+  extern __shared__ unsigned char mf_sm[];
+  T* Bl = reinterpret_cast<T*>(mf_sm);          // 64 x 65 block
+  const int s = list[blockIdx.x];
+  T* Vl = ldsv ? Bl + 64 * 65 : V + d.vo[s];    // the front's vector (ms)
+  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const T* Fs = F + d.fo[s];
+  const int* ix = d.idx + d.iptr[s];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  for (int i = t; i < ms; i += blockDim.x) Vl[i] = i < np ? (T)b[ix[i]] : (T)0;
+  __syncthreads();
+  mf_gather_children(d, s, V, Vl);
+  for (int b0 = 0; b0 < np; b0 += 64) {
+    const int bw = min(64, np - b0);
+    if (b0 > 0) {
+      for (int rr = wave; rr < bw; rr += nw) {
+        const T* Lr = Fs + (long long)(b0 + rr) * ms;
+        T acc = (T)0;
+        for (int c = lane; c < b0; c += 64) acc += Lr[c] * Vl[c];
+        acc = mf_wave_sum(acc);
+        if (lane == 0) Vl[b0 + rr] -= acc;
+      }
+    }
+    for (int x = t; x < bw * 64; x += blockDim.x) {
+      const int i = x >> 6, j = x & 63;
+      if (j < i) Bl[i * 65 + j] = Fs[(long long)(b0 + i) * ms + b0 + j];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      T v = lane < bw ? Vl[b0 + lane] : (T)0;
+      for (int k = 0; k < bw; ++k) {
+        const T yk = mf_bcast(v, k);
+        if (lane > k && lane < bw) v -= Bl[lane * 65 + k] * yk;
+      }
+      if (lane < bw) Vl[b0 + lane] = v;
+    }
+    __syncthreads();
+  }
+  for (int rr = np + wave; rr < ms; rr += nw) {
+    const T* Lr = Fs + (long long)rr * ms;
+    T acc = (T)0;
+    for (int c = lane; c < np; c += 64) acc += Lr[c] * Vl[c];
+    acc = mf_wave_sum(acc);
+    if (lane == 0) Vl[rr] -= acc;
   }
   __syncthreads();
+  if (ldsv)
+    for (int i = t; i < ms; i += blockDim.x) V[d.vo[s] + i] = Vl[i];
+  for (int i = t; i < np; i += blockDim.x) yg[ix[i]] = Vl[i];
+}
+
+// backward, fronts of order <= 64 (roots first): x = U11^-1 (y - U12 x_below)
+template <class T>
+__global__ void __launch_bounds__(64) k_mf_bwd_tiny(MfDev d, const int* list, const T* F, T* x) {
+  __shared__ T Xl[kMfTiny];
+  const int s = list[blockIdx.x];
+  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const T* Fs = F + d.fo[s];
+  const int* ix = d.idx + d.iptr[s];
+  const int t = threadIdx.x;
+  if (t < ms) Xl[t] = x[ix[t]];           // y on the pivot rows, solved x below
+  __syncthreads();
+  T v = Xl[t];
+  if (t < np)
+    for (int u = np; u < ms; ++u) v -= Fs[(long long)t * ms + u] * Xl[u];
   for (int k = np - 1; k >= 0; --k) {
-    if (threadIdx.x == 0) Vs[k] = Vs[k] / Fs[(long long)k * ms + k];
+    const T xk = mf_bcast(v, k) / Fs[(long long)k * ms + k];
+    if (t == k) v = xk;
+    else if (t < k) v -= Fs[(long long)t * ms + k] * xk;
+  }
+  if (t < np) x[ix[t]] = v;
+}
+
+// backward, larger fronts: rows minus U12 x_below (row-wise products), then
+// U11 by 64-row blocks from the bottom
+template <class T>
+__global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int* list, const T* F, T* V, T* x,
+                                                             int ldsv) {
+  extern __shared__ unsigned char mf_sm[];
+  T* Bl = reinterpret_cast<T*>(mf_sm);          // 64 x 65 block
+  const int s = list[blockIdx.x];
+  T* Xl = ldsv ? Bl + 64 * 65 : V + d.vo[s];    // scratch (ms); V is not read by the backward solve
+  const int ms = d.m[s], np = d.sfirst[s + 1] - d.sfirst[s];
+  const T* Fs = F + d.fo[s];
+  const int* ix = d.idx + d.iptr[s];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
+  for (int i = t; i < ms; i += blockDim.x) Xl[i] = x[ix[i]];
+  __syncthreads();
+  for (int rr = wave; rr < np; rr += nw) {
+    const T* Ur = Fs + (long long)rr * ms;
+    T acc = (T)0;
+    for (int c = np + lane; c < ms; c += 64) acc += Ur[c] * Xl[c];
+    acc = mf_wave_sum(acc);
+    if (lane == 0) Xl[rr] -= acc;
+  }
+  __syncthreads();
+  for (int b1 = np; b1 > 0; b1 -= 64) {
+    const int b0 = max(0, b1 - 64), bw = b1 - b0;
+    if (b1 < np) {
+      for (int rr = wave; rr < bw; rr += nw) {
+        const T* Ur = Fs + (long long)(b0 + rr) * ms;
+        T acc = (T)0;
+        for (int c = b1 + lane; c < np; c += 64) acc += Ur[c] * Xl[c];
+        acc = mf_wave_sum(acc);
+        if (lane == 0) Xl[b0 + rr] -= acc;
+      }
+    }
+    for (int q = t; q < bw * 64; q += blockDim.x) {
+      const int i = q >> 6, j = q & 63;
+      if (j >= i && j < bw) Bl[i * 65 + j] = Fs[(long long)(b0 + i) * ms + b0 + j];
+    }
     __syncthreads();
-    const T xk = Vs[k];
-    for (int t = threadIdx.x; t < k; t += blockDim.x) Vs[t] -= Fs[(long long)t * ms + k] * xk;
+    if (wave == 0) {
+      T v = lane < bw ? Xl[b0 + lane] : (T)0;
+      for (int k = bw - 1; k >= 0; --k) {
+        const T xk = mf_bcast(v, k) / Bl[k * 65 + k];
+        if (lane == k) v = xk;
+        else if (lane < k) v -= Bl[lane * 65 + k] * xk;
+      }
+      if (lane < bw) Xl[b0 + lane] = v;
+    }
     __syncthreads();
   }
-  for (int t = threadIdx.x; t < np; t += blockDim.x) x[ix[t]] = Vs[t];
+  for (int i = t; i < np; i += blockDim.x) x[ix[i]] = Xl[i];
 }
 
 template <class T>
@@ -452,23 +652,24 @@ struct DevVec {
   }
 };
 
-constexpr int kMfSmall = 160;   // fronts up to this order are factored by one workgroup
-constexpr int kMfPanel = 32;
-
 struct MfLu::Impl {
   MfPlan P;
   hipStream_t st = nullptr;
-  DevVec<int> sfirst, m, iptr, idx, cptr, clist, eptr, emap, vo, lvl_s, prow, pcol, bad;
+  DevVec<int> sfirst, m, iptr, idx, cptr, clist, eptr, emap, vo, prow, pcol, bad;
   DevVec<long long> fo, amap;
   DevVec<double> F64, V64, y64, b64;
   DevVec<float> F32, V32, y32;
-  std::vector<std::vector<int>> small_of_level, large_of_level;
-  DevVec<int> small_list, ea_list, sparent;
-  DevVec<double> dbuf;            // one panel's diagonal-block factors (FP64 or FP32)
-  std::vector<int> small_off;     // per level: offset into small_list
+  DevVec<int> tiny_list, big_list, ea_list, sparent;
+  DevVec<int4> pent, uent;        // panel and update launch entries
+  DevVec<double> dbuf;            // diagonal-block factors, one 32x32 slot per front of a panel launch
+  std::vector<int> tiny_off, tiny_maxm, big_off;
+  int big_maxm = 0;
   // extend-add launches: per level, per child index j: [offset, count, max rows]
   struct EaLaunch { int off, count, max_nu; };
   std::vector<std::vector<EaLaunch>> ea_of_level;
+  // blocked launches: per level, per panel index j
+  struct PanelLaunch { int p0, poff, pcnt, uoff, ucnt; };
+  std::vector<std::vector<PanelLaunch>> pl_of_level;
   MfDev dev() const {
     MfDev d;
     d.sfirst = sfirst.p; d.m = m.p; d.fo = fo.p; d.iptr = iptr.p; d.idx = idx.p; d.cptr = cptr.p;
@@ -490,32 +691,78 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   for (int v : P.idx) if (v < 0 || v >= P.n) return XG_E_SETUP;
   for (long long a : P.amap) if (a < 0 || a >= P.fo[P.ns]) return XG_E_SETUP;
   for (int c : P.clist) if (c < 0 || c >= P.ns) return XG_E_SETUP;
-  for (int t = 0; t < P.ns; ++t)
+  for (int t = 0; t < P.ns; ++t) {
+    if (P.m[t] < P.sfirst[t + 1] - P.sfirst[t] || P.sfirst[t + 1] <= P.sfirst[t]) return XG_E_SETUP;
     for (int e = P.eptr[t]; e < P.eptr[t + 1]; ++e)
       if (P.sparent[t] < 0 || P.emap[e] < 0 || P.emap[e] >= P.m[P.sparent[t]]) return XG_E_SETUP;
+  }
   hipStream_t st = I.st;
   MFCHK(I.sfirst.up(P.sfirst, st)); MFCHK(I.m.up(P.m, st)); MFCHK(I.iptr.up(P.iptr, st)); MFCHK(I.idx.up(P.idx, st));
   MFCHK(I.cptr.up(P.cptr, st)); MFCHK(I.clist.up(P.clist, st)); MFCHK(I.eptr.up(P.eptr, st));
   MFCHK(I.emap.up(P.emap, st)); MFCHK(I.vo.up(P.vo, st)); MFCHK(I.fo.up(P.fo, st)); MFCHK(I.amap.up(P.amap, st));
   MFCHK(I.prow.up(s.perm_row, st)); MFCHK(I.pcol.up(s.perm_col, st));
   MFCHK(I.bad.alloc(2));
-  MFCHK(I.dbuf.alloc(32 * 32));
   MFCHK(I.F64.alloc(P.fo[P.ns])); MFCHK(I.V64.alloc(P.vo[P.ns])); MFCHK(I.y64.alloc(P.n)); MFCHK(I.b64.alloc(P.n));
   if (want_fp32) { MFCHK(I.F32.alloc(P.fo[P.ns])); MFCHK(I.V32.alloc(P.vo[P.ns])); MFCHK(I.y32.alloc(P.n)); }
-  // per level: small fronts (one launch) and large fronts (blocked, one by one)
+  // per level: tiny fronts (one launch), larger fronts (blocked panels, all
+  // fronts of the level in each panel launch)
   const int nl = (int)P.lvl_ptr.size() - 1;
-  std::vector<int> sl;
-  I.small_off.assign(nl + 1, 0);
-  I.large_of_level.assign(nl, {});
+  std::vector<int> tl, bl;
+  std::vector<int4> pe, ue;
+  int max_slots = 1;
+  I.tiny_off.assign(nl + 1, 0);
+  I.big_off.assign(nl + 1, 0);
+  I.tiny_maxm.assign(nl, 0);
+  I.pl_of_level.assign(nl, {});
+  I.big_maxm = 0;
   for (int l = 0; l < nl; ++l) {
-    I.small_off[l] = (int)sl.size();
+    I.tiny_off[l] = (int)tl.size();
+    I.big_off[l] = (int)bl.size();
+    int maxp = 0;
     for (int q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
       const int t = P.lvl_s[q];
-      if (P.m[t] <= kMfSmall) sl.push_back(t); else I.large_of_level[l].push_back(t);
+      if (P.m[t] <= kMfTiny) {
+        tl.push_back(t);
+        I.tiny_maxm[l] = std::max(I.tiny_maxm[l], P.m[t]);
+      } else {
+        bl.push_back(t);
+        I.big_maxm = std::max(I.big_maxm, P.m[t]);
+        maxp = std::max(maxp, (P.sfirst[t + 1] - P.sfirst[t] + kMfPanel - 1) / kMfPanel);
+      }
+    }
+    for (int j = 0; j < maxp; ++j) {
+      Impl::PanelLaunch L{j * kMfPanel, (int)pe.size(), 0, (int)ue.size(), 0};
+      int slot = 0;
+      for (int b = I.big_off[l]; b < (int)bl.size(); ++b) {
+        const int t = bl[b];
+        const int np = P.sfirst[t + 1] - P.sfirst[t];
+        if (np <= L.p0) continue;
+        const int pw = std::min(kMfPanel, np - L.p0);
+        const int rest = P.m[t] - L.p0 - pw;
+        const int nrb = (rest + 63) / 64;
+        if (rest > 0) {
+          for (int blk = 0; blk < 2 * nrb; ++blk) pe.push_back(make_int4(t, blk, slot, 0));
+          for (int ti = 0; ti < nrb; ++ti)
+            for (int tj = 0; tj < nrb; ++tj) ue.push_back(make_int4(t, ti, tj, slot));
+        } else {
+          pe.push_back(make_int4(t, 0, slot, 0));
+          ue.push_back(make_int4(t, -1, -1, slot));
+        }
+        ++slot;
+      }
+      L.pcnt = (int)pe.size() - L.poff;
+      L.ucnt = (int)ue.size() - L.uoff;
+      max_slots = std::max(max_slots, slot);
+      I.pl_of_level[l].push_back(L);
     }
   }
-  I.small_off[nl] = (int)sl.size();
-  MFCHK(I.small_list.up(sl, st));
+  I.tiny_off[nl] = (int)tl.size();
+  I.big_off[nl] = (int)bl.size();
+  MFCHK(I.tiny_list.up(tl, st));
+  MFCHK(I.big_list.up(bl, st));
+  MFCHK(I.pent.up(pe, st));
+  MFCHK(I.uent.up(ue, st));
+  MFCHK(I.dbuf.alloc((size_t)max_slots * kMfPanel * kMfPanel));
   // extend-add launches: the j-th child of every front of a level together
   std::vector<int> ea;
   I.ea_of_level.assign(nl, {});
@@ -542,7 +789,6 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   }
   MFCHK(I.ea_list.up(ea, st));
   MFCHK(I.sparent.up(P.sparent, st));
-  MFCHK(I.lvl_s.up(P.lvl_s, st));
   MFCHK(hipStreamSynchronize(st));
   return XG_OK;
 }
@@ -562,30 +808,23 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
   const int init[2] = {0x7fffffff, 0};
   MFCHK(hipMemcpyAsync(I.bad.p, init, sizeof(init), hipMemcpyHostToDevice, st));
   const MfDev d = I.dev();
+  T* dbuf = reinterpret_cast<T*>(I.dbuf.p);
   const int nl = (int)P.lvl_ptr.size() - 1;
   for (int l = 0; l < nl; ++l) {
-    const int cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
-    (void)cnt;
     for (const auto& L : I.ea_of_level[l])
       hipLaunchKernelGGL(k_mf_extend_add<T>, dim3(L.count, (L.max_nu + 15) / 16), dim3(256), 0, st, d,
                          I.ea_list.p + L.off, I.sparent.p, F);
-    const int ns_small = I.small_off[l + 1] - I.small_off[l];
-    if (ns_small)
-      hipLaunchKernelGGL(k_mf_factor_small<T>, dim3(ns_small), dim3(256), 0, st, d, I.small_list.p + I.small_off[l], F,
+    const int nt = I.tiny_off[l + 1] - I.tiny_off[l];
+    if (nt) {
+      const int mm = I.tiny_maxm[l];
+      hipLaunchKernelGGL(k_mf_factor_tiny<T>, dim3(nt), dim3(64), (size_t)mm * (mm + 1) * sizeof(T), st, d,
+                         I.tiny_list.p + I.tiny_off[l], F, I.bad.p);
+    }
+    for (const auto& L : I.pl_of_level[l]) {
+      hipLaunchKernelGGL(k_mf_panel<T>, dim3(L.pcnt), dim3(64), 0, st, d, I.pent.p + L.poff, L.p0, F, dbuf,
                          I.bad.p);
-    for (int t : I.large_of_level[l]) {
-      T* Fs = F + P.fo[t];
-      const int ms = P.m[t], np = P.sfirst[t + 1] - P.sfirst[t];
-      for (int p0 = 0; p0 < np; p0 += kMfPanel) {
-        const int pw = std::min(kMfPanel, np - p0);
-        const int rest = ms - p0 - pw;
-        const int nb = (rest + 63) / 64;
-        T* dbuf = reinterpret_cast<T*>(I.dbuf.p);
-        hipLaunchKernelGGL(k_mf_panel_solve<T>, dim3(std::max(1, 2 * nb)), dim3(64), 0, st, Fs, ms, p0, pw, nb,
-                           P.sfirst[t], dbuf, I.bad.p);
-        const unsigned tiles = (unsigned)std::max(1, (rest + 63) / 64);
-        hipLaunchKernelGGL(k_mf_update<T>, dim3(tiles, tiles), dim3(256), 0, st, Fs, ms, p0, pw, (const T*)dbuf);
-      }
+      hipLaunchKernelGGL(k_mf_update<T>, dim3(L.ucnt), dim3(256), 0, st, d, I.uent.p + L.uoff, L.p0, F,
+                         (const T*)dbuf);
     }
   }
   MFCHK(hipGetLastError());
@@ -602,15 +841,29 @@ static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, const double* b_dev
   hipStream_t st = I.st;
   const MfDev d = I.dev();
   const int n = P.n;
+  // the larger fronts keep their vector in LDS when it fits in 64 KB with the
+  // 64x65 block, else in their slice of V
+  const size_t blk_bytes = 64 * 65 * sizeof(T);
+  const bool ldsv = blk_bytes + (size_t)I.big_maxm * sizeof(T) <= 65536;
+  const size_t big_lds = blk_bytes + (ldsv ? (size_t)I.big_maxm * sizeof(T) : 0);
   hipLaunchKernelGGL(k_mf_perm_in<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, I.prow.p, b_dev, I.b64.p);
   const int nl = (int)P.lvl_ptr.size() - 1;
   for (int l = 0; l < nl; ++l) {
-    const int cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
-    hipLaunchKernelGGL(k_mf_fwd<T>, dim3(cnt), dim3(256), 0, st, d, I.lvl_s.p + P.lvl_ptr[l], F, V, I.b64.p, y);
+    const int nt = I.tiny_off[l + 1] - I.tiny_off[l], nb = I.big_off[l + 1] - I.big_off[l];
+    if (nt)
+      hipLaunchKernelGGL(k_mf_fwd_tiny<T>, dim3(nt), dim3(64), 0, st, d, I.tiny_list.p + I.tiny_off[l], F, V,
+                         I.b64.p, y);
+    if (nb)
+      hipLaunchKernelGGL(k_mf_fwd_big<T>, dim3(nb), dim3(kMfSolveBlock), big_lds, st, d, I.big_list.p + I.big_off[l],
+                         F, V, I.b64.p, y, (int)ldsv);
   }
   for (int l = nl - 1; l >= 0; --l) {
-    const int cnt = P.lvl_ptr[l + 1] - P.lvl_ptr[l];
-    hipLaunchKernelGGL(k_mf_bwd<T>, dim3(cnt), dim3(256), 0, st, d, I.lvl_s.p + P.lvl_ptr[l], F, V, y);
+    const int nt = I.tiny_off[l + 1] - I.tiny_off[l], nb = I.big_off[l + 1] - I.big_off[l];
+    if (nb)
+      hipLaunchKernelGGL(k_mf_bwd_big<T>, dim3(nb), dim3(kMfSolveBlock), big_lds, st, d, I.big_list.p + I.big_off[l],
+                         F, V, y, (int)ldsv);
+    if (nt)
+      hipLaunchKernelGGL(k_mf_bwd_tiny<T>, dim3(nt), dim3(64), 0, st, d, I.tiny_list.p + I.tiny_off[l], F, y);
   }
   hipLaunchKernelGGL(k_mf_perm_out<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, I.pcol.p, y, x_dev, nan_flag);
   MFCHK(hipGetLastError());
