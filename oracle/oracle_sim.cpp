@@ -185,6 +185,10 @@ struct Work {
   // defaults are the reference's
   double tia_reltol = 1e-3, tia_abstol = 1e-6, nl_reltol = 1e-2, nl_abstol = 1e-6;
   SparseLU slu;
+  // recorded quantities: empty = the whole solution at each accepted point;
+  // else probe_k rows of probe_w (row-major, n each): the products w_r . x
+  int probe_k = 0;
+  std::vector<double> probe_w;
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
     x.assign(n + 1, 0.0); F.assign(n + 1, 0.0); Q.assign(n + 1, 0.0); B.assign(n + 1, 0.0);
@@ -452,6 +456,7 @@ static std::vector<double> breakpoints(const Circuit& c, double tstep, double ts
 
 struct TranResult {
   std::vector<double> t, x;   // accepted points
+  std::vector<int> ord;       // integration order of the step that ended at each point (0 = DC point)
   long newton_iters = 0, steps = 0, rejects = 0, loads = 0;
   double seconds = 0;
   int status = 0;
@@ -483,6 +488,29 @@ static int dcop(Work& w, long* iters) {
 // (:1390).  Error weights: DataStore::setErrorWtVector with NEWLTE=1
 // (N_TIA_DataStore.C:1304), estimate ck*WRMS(x - xn0) (N_TIA_OneStep.h:93).
 // Breakpoints restart the integrator at order 1.
+static void record_point(const Work& w, TranResult& R, double t, const std::vector<double>& x, int ord) {
+  R.t.push_back(t);
+  R.ord.push_back(ord);
+  if (w.probe_k == 0) {
+    R.x.insert(R.x.end(), x.begin(), x.begin() + w.n);
+    return;
+  }
+  // compensated (Neumaier) sums: a probe such as the total charge of a large
+  // network is a sum of many like terms, whose plain rounding would swamp the
+  // small changes a step makes to it
+  for (int r = 0; r < w.probe_k; ++r) {
+    const double* wr = w.probe_w.data() + (size_t)r * w.n;
+    double s = 0.0, comp = 0.0;
+    for (int i = 0; i < w.n; ++i) {
+      const double v = wr[i] * x[i];
+      const double t = s + v;
+      comp += std::fabs(s) >= std::fabs(v) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+    R.x.push_back(s + comp);
+  }
+}
+
 static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool record, double t_record_stop) {
   TranResult R;
   Circuit& c = *w.c;
@@ -540,7 +568,7 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
   SolverState ss; ss.dcop = 0; ss.tran = 1;
   NLP p{w.nl_reltol, w.nl_abstol, 0.33, 1e-2, 1e-6, 20, true};
   std::vector<double> xn0(n), xcurr(n), corr(n), errwt(n);
-  if (record) { R.t.push_back(0.0); R.x.insert(R.x.end(), xH0.begin(), xH0.end()); }
+  if (record) record_point(w, R, 0.0, xH0, 0);
   while (t < tstop * (1 - 1e-12)) {
     double stopTime = next_bp();
     if (t + h > stopTime) { savedTimeStep = h; h = stopTime - t; currentTimeStepSum = h + lastTimeStep; }
@@ -607,7 +635,7 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
       }
       w.staCur = w.staNext; w.stoCur = w.stoNext;
       t = tnew;
-      if (record && t <= t_record_stop * (1 + 1e-12)) { R.t.push_back(t); R.x.insert(R.x.end(), xH0.begin(), xH0.end()); }
+      if (record && t <= t_record_stop * (1 + 1e-12)) record_point(w, R, t, xH0, usedOrder);
       newh = std::max(newh, hmin);
       if (hmax > 0) newh = std::min(newh, hmax);
       beginning = false;
@@ -865,6 +893,20 @@ void orc_tran_points(OrcHandle* h, double* t, double* x) {
   const TranResult& R = h->last;
   std::copy(R.t.begin(), R.t.end(), t);
   std::copy(R.x.begin(), R.x.end(), x);
+}
+
+void orc_tran_orders(OrcHandle* h, int* ord) {
+  const TranResult& R = h->last;
+  std::copy(R.ord.begin(), R.ord.end(), ord);
+}
+
+// Record k products w_r . x per accepted point instead of the solution
+// (k = 0 restores whole solutions); w is k x n, row-major.
+int orc_set_probe(OrcHandle* h, int k, const double* w) {
+  if (k < 0 || (k > 0 && !w)) return -1;
+  h->w->probe_k = k;
+  h->w->probe_w.assign(w, w + (size_t)k * h->w->n);
+  return 0;
 }
 
 }  // extern "C"

@@ -38,6 +38,9 @@ def lib():
         L.orc_tran.argtypes = [P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, P]
         L.orc_tran.restype = C.c_int
         L.orc_tran_points.argtypes = [P, P, P]
+        L.orc_tran_orders.argtypes = [P, P]
+        L.orc_set_probe.argtypes = [P, C.c_int, P]
+        L.orc_set_probe.restype = C.c_int
         for nm in ("orc_limvds", "orc_fetlim", "orc_qmeyer", "orc_pnjlim"):
             getattr(L, nm).restype = C.c_int
         _LIB = L
@@ -146,16 +149,32 @@ class Oracle:
         lib().orc_set_tolerances(self.h, C.c_double(tia_reltol), C.c_double(tia_abstol), C.c_double(nl_reltol),
                                  C.c_double(nl_abstol))
 
+    def set_probe(self, w=None):
+        """Record, per accepted point, the products of the rows of ``w``
+        (k x n) with the solution instead of the whole solution; None
+        restores whole solutions."""
+        if w is None:
+            self._probe_k = 0
+            lib().orc_set_probe(self.h, 0, None)
+            return
+        w = np.ascontiguousarray(w, np.float64).reshape(-1, self.n)
+        self._probe_k = w.shape[0]
+        lib().orc_set_probe(self.h, w.shape[0], _p(w))
+
     def tran(self, tstop, tstep, hmax, record=True, t_record_stop=None):
         stats = np.zeros(7)
         st = lib().orc_tran(self.h, tstop, tstep, hmax, 1 if record else 0,
                             tstop if t_record_stop is None else t_record_stop, _p(stats))
         npts = int(stats[6])
+        width = getattr(self, "_probe_k", 0) or self.n
         t = np.zeros(max(npts, 1))
-        x = np.zeros(max(npts * self.n, 1))
+        x = np.zeros(max(npts * width, 1))
+        order = np.zeros(max(npts, 1), np.int32)
         lib().orc_tran_points(self.h, _p(t), _p(x))
+        lib().orc_tran_orders(self.h, _p(order))
         return dict(status=st, steps=int(stats[0]), rejects=int(stats[1]), newton_iters=int(stats[2]),
-                    loads=int(stats[3]), seconds=float(stats[4]), t=t[:npts], x=x[: npts * self.n].reshape(npts, self.n))
+                    loads=int(stats[3]), seconds=float(stats[4]), t=t[:npts], x=x[: npts * width].reshape(npts, width),
+                    order=order[:npts])
 
 
 def lu_solve(n, rowptr, colind, vals, b):

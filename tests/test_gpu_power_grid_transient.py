@@ -1,0 +1,136 @@
+"""Power-grid transient driven through the drop-in boundary.
+
+The oracle's restatement of the reference's driver (DC operating point,
+DampedNewton, OneStep trapezoid with breakpoints) calls the GPU device loads
+(xg_update_state / xg_load_vectors / xg_load_matrices) and the GPU LU
+(xg_lu_symbolic picks the multifrontal factorisation for the large mesh)
+through the C ABI at every Newton iteration, as Xyce's TimeIntg and
+DampedNewton drive DeviceMgr and Linear::Solver.
+
+* 40x40 mesh, 0-0.4 ns, with the level-scheduled and with the multifrontal
+  LU: the run equals the all-CPU oracle run (same step sequence, node
+  voltages within 1e-9 V);
+* 512x512 mesh at full size (262,208 unknowns, 4096 PWL sinks), 0-25 ps:
+  status, at most two solves per step (a linear network), pads held at VDD,
+  and the summed node equations (charge balance, ``grid_charge_balance``)
+  met to round-off at every accepted step.  The oracle's own sparse LU needs
+  minutes per factorisation at this size, so the full-size run is checked
+  through these size-independent properties; the time per Newton iteration
+  is printed.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import xyce_amd as X
+from oracle.pyoracle import Oracle
+from tests.helpers import grid_charge_balance, grid_probe, waveform_error
+from xyce_amd import circuits as C
+
+pytestmark = pytest.mark.gpu
+
+VDD = 1.8
+
+
+def _drive_with_gpu(c, drv, method=0):
+    """Hooks on oracle ``drv`` that route its loads and solves to the GPU."""
+    dm = X.DeviceMgr(c)
+    p = dm.pattern()
+    ls = X.DirectSolver(dm.n, p["rowptr"], p["colind"], method=method)
+    n, nnz, nsta, nsto = dm.n, dm.nnz, dm.nsta, dm.nsto
+    # a circuit without MOSFETs has no state or store vectors: those pointers are NULL
+    arr = lambda ptr, k: np.ctypeslib.as_array(ptr, shape=(k,)) if k > 0 else np.zeros(0)
+    errors = []
+    calls = {"load": 0, "solve": 0, "t_solve": 0.0, "t_load": 0.0}
+
+    def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
+        try:
+            t0 = time.perf_counter()
+            si, sd = arr(ssi, 6), arr(ssd, 4)
+            dm.set_solver_state(dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
+                                order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]),
+                                tstop=float(sd[3]))
+            sta_n, sto_n, lim = dm.updateState(arr(x, n).copy(), arr(sc, nsta).copy(), arr(oc, nsto).copy(),
+                                               sto_next=arr(on, nsto).copy())
+            if nsta:
+                arr(sn, nsta)[:] = sta_n[:nsta]
+            if nsto:
+                arr(on, nsto)[:] = sto_n[:nsto]
+            v = dm.loadDAEVectors()
+            for name, ptr in (("F", F), ("Q", Q), ("B", B), ("dFdxdVp", dFv), ("dQdxdVp", dQv)):
+                arr(ptr, n)[:n] = v[name]
+            dF, dQ = dm.loadDAEMatrices()
+            arr(dFdx, nnz)[:nnz] = dF
+            arr(dQdx, nnz)[:nnz] = dQ
+            calls["load"] += 1
+            calls["t_load"] += time.perf_counter() - t0
+            return int(lim)
+        except Exception as e:      # surfaced after the run
+            errors.append(repr(e))
+            return 0
+
+    def solve(ctx, nn, rp, ci, vals, b, x):
+        try:
+            t0 = time.perf_counter()
+            st, xs = ls.doSolve(arr(vals, nnz)[:nnz], arr(b, nn)[:nn])
+            arr(x, nn)[:nn] = xs
+            calls["solve"] += 1
+            calls["t_solve"] += time.perf_counter() - t0
+            return int(st)
+        except Exception as e:
+            errors.append(repr(e))
+            return -1
+
+    drv.set_hooks(load, solve)
+    return ls, calls, errors
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_small_grid_dropin_run_equals_oracle_run(gpu, method):
+    c = C.power_grid(40, 40, seed=7, pad_pitch=16, n_sinks=48, tstop=0.8e-9)
+    tstop = 0.4e-9
+    ref = Oracle(c).tran(tstop, 1e-12, 1e-12)
+    assert ref["status"] == 0
+    drv = Oracle(c)
+    ls, calls, errors = _drive_with_gpu(c, drv, method)
+    r = drv.tran(tstop, 1e-12, 1e-12)
+    assert not errors, errors[:3]
+    assert r["status"] == 0 and ls.info()["method"] == method
+    print(f"40x40 grid, LU method {method}: {r['steps']} steps, {calls['solve']} GPU solves "
+          f"(oracle run {ref['steps']} steps)")
+    assert r["steps"] == ref["steps"] and r["newton_iters"] == ref["newton_iters"]
+    np.testing.assert_array_equal(r["t"], ref["t"])
+    assert waveform_error(r["t"], r["x"], ref["t"], ref["x"]) <= 1e-9
+
+
+def test_power_grid_512_transient_full_size(gpu):
+    nx = 512
+    tstop_run, tstop_circuit = 25e-12, 2e-9
+    c = C.power_grid(nx, nx, seed=2, pad_pitch=64, n_sinks=4096, tstop=tstop_circuit)
+    drv = Oracle(c)
+    pad = c.vsrc[0][0]
+    center = 1 + (nx // 2) * nx + nx // 2
+    sink_nodes = [a for a, _, _ in c.isrc[:4]]
+    w = grid_probe(c, drv, extra_nodes=[pad, center] + sink_nodes)
+    drv.set_probe(w)
+    ls, calls, errors = _drive_with_gpu(c, drv)
+    t0 = time.perf_counter()
+    r = drv.tran(tstop_run, 1e-12, 1e-12)
+    wall = time.perf_counter() - t0
+    assert not errors, errors[:3]
+    assert r["status"] == 0
+    info = ls.info()
+    print(f"512x512 grid, 0-{tstop_run * 1e12:.0f} ps: {r['steps']} steps, {r['newton_iters']} Newton iterations, "
+          f"{calls['solve']} GPU solves in {wall:.1f} s ({1e3 * wall / max(calls['solve'], 1):.1f} ms per Newton "
+          f"iteration: loads {1e3 * calls['t_load'] / max(calls['load'], 1):.1f} ms, LU "
+          f"{1e3 * calls['t_solve'] / max(calls['solve'], 1):.1f} ms, host arrays); LU method {info['method']}")
+    assert info["method"] == 2
+    assert r["steps"] >= 25 and r["rejects"] == 0
+    assert r["newton_iters"] <= 2 * r["steps"] + 4
+    y = r["x"]
+    assert np.max(np.abs(y[:, 3] - VDD)) <= 1e-9                      # pad node
+    assert np.all(y[:, 4:] <= VDD + 1e-9) and np.all(y[:, 4:] >= VDD - 0.2)
+    bal = grid_charge_balance(c, drv, r, 1e-12, tstop_circuit)
+    print(f"charge balance: worst defect {bal:.3g} of the round-off allowance")
+    assert bal <= 1.0

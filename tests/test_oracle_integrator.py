@@ -173,3 +173,33 @@ def test_driver_hooks_with_oracle_loads_reproduce_the_run():
     r = drv.tran(0.3e-9, 1e-12, 1e-11)
     assert r["status"] == 0 and r["steps"] == ref["steps"] and r["newton_iters"] == ref["newton_iters"]
     np.testing.assert_array_equal(r["x"], ref["x"])
+
+
+def test_power_grid_charge_balance_and_planted_mistake():
+    """The oracle's trapezoid/BE integration of a linear power grid conserves
+    charge step by step to round-off (summed node equations, ``grid_probe``);
+    the same check on a run with one decap recorded 10 % off, or with the
+    step's order taken as the other one, fails by orders of magnitude."""
+    from helpers import grid_charge_balance, grid_probe
+    c = C.power_grid(12, 12, seed=5, pad_pitch=6, n_sinks=16, tstop=0.4e-9)
+    o = Oracle(c)
+    w = grid_probe(c, o, extra_nodes=[1, 2])
+    o.set_probe(w)
+    r = o.tran(0.4e-9, 1e-12, 1e-12)
+    assert r["status"] == 0 and r["x"].shape == (len(r["t"]), 5)
+    assert set(np.unique(r["order"][1:])) == {1, 2}
+    assert grid_charge_balance(c, o, r, 1e-12, 0.4e-9) <= 1.0
+    # the probe rows agree with whole-solution recording
+    full = Oracle(c).tran(0.4e-9, 1e-12, 1e-12)
+    np.testing.assert_allclose(r["x"], full["x"] @ w.T, rtol=1e-12, atol=1e-24)
+    # planted mistakes
+    bad = dict(r)
+    bad["x"] = r["x"].copy()
+    gid = o.pattern()["node_gid"]
+    a, _, cap = c.cap[7]
+    bad["x"][:, 0] += 0.1 * cap * full["x"][:, gid[a]]
+    assert grid_charge_balance(c, o, bad, 1e-12, 0.4e-9) > 10.0
+    flipped = dict(r)
+    flipped["order"] = np.where(r["order"] == 2, 1, 2)
+    flipped["order"][0] = 0
+    assert grid_charge_balance(c, o, flipped, 1e-12, 0.4e-9) > 100.0
