@@ -169,7 +169,7 @@ struct MfDev {
 
 constexpr int kMfTiny = 64;     // fronts up to this order: one wavefront, the front in LDS
 constexpr int kMfPanel = 32;    // panel width of the blocked factorisation
-constexpr int kMfSolveBlock = 512;
+constexpr int kMfSolveBlock = 1024;
 
 template <class T>
 __global__ void k_mf_fill(T* F, long long total) {
@@ -201,27 +201,36 @@ __device__ __forceinline__ T mf_wave_sum(T v) {
   return v;
 }
 
-// extend-add of one child's Schur complement into its parent: block.x picks
-// the child from `list`, block.y a chunk of 16 of its rows.  Children of one
-// parent go in separate launches (their j-th child in launch j), so every
-// parent element is added to in a fixed order and no two blocks of a launch
-// write the same element.
+// assembly of the children's Schur complements into a larger front: one
+// workgroup per kMfAsmRows rows of the parent (entries {front, first row,
+// offset into `alo`}); the children are added one after another in their
+// fixed order, so every parent element receives the same sums in the same
+// order on every refactor (no atomics).  A child's rows map to increasing
+// parent rows; alo holds, per entry and child, the first child row that lands
+// at or after the entry's first row.  The four wavefronts take the child rows
+// in turn, lanes over the columns.
+constexpr int kMfAsmRows = 16;
+
 template <class T>
-__global__ void __launch_bounds__(256) k_mf_extend_add(MfDev d, const int* list, const int* sparent, T* F) {
-  const int ch = list[blockIdx.x];
-  const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
-  const int a0 = blockIdx.y * 16;
-  if (a0 >= nu) return;
-  const int pa = sparent[ch];
-  const int ms = d.m[pa];
-  T* Fs = F + d.fo[pa];
-  const T* Fc = F + d.fo[ch];
-  const int* em = d.emap + d.eptr[ch];
-  const int a1 = min(nu, a0 + 16);
-  for (int a = a0; a < a1; ++a) {
-    const long long prow = (long long)em[a] * ms;
-    const T* crow = Fc + (long long)(npc + a) * mc + npc;
-    for (int b = threadIdx.x; b < nu; b += blockDim.x) Fs[prow + em[b]] += crow[b];
+__global__ void __launch_bounds__(256) k_mf_assemble(MfDev d, const int4* ent, const int* alo, T* F) {
+  const int4 e = ent[blockIdx.x];
+  const int s = e.x, ms = d.m[s];
+  const int r1 = min(ms, e.y + kMfAsmRows);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  T* Fs = F + d.fo[s];
+  for (int c = d.cptr[s]; c < d.cptr[s + 1]; ++c) {
+    const int ch = d.clist[c];
+    const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
+    const int* em = d.emap + d.eptr[ch];
+    const T* Fc = F + d.fo[ch];
+    for (int a = alo[e.z + (c - d.cptr[s])] + wave; a < nu; a += 4) {
+      const int pr = em[a];
+      if (pr >= r1) break;
+      const T* crow = Fc + (long long)(npc + a) * mc + npc;
+      T* prow = Fs + (long long)pr * ms;
+      for (int b = lane; b < nu; b += 64) prow[em[b]] += crow[b];
+    }
+    __syncthreads();
   }
 }
 
@@ -232,7 +241,8 @@ __device__ __forceinline__ bool mf_bad_pivot(T d) {
 }
 
 // fronts of order <= 64: one wavefront per front, the front staged in LDS
-// (row stride m + 1), right-looking over its pivots with lane i owning row i
+// (row stride m + 1) and its children's updates added there, then
+// right-looking over its pivots with lane i owning row i
 template <class T>
 __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list, T* F, int* bad) {
   extern __shared__ unsigned char mf_sm[];
@@ -245,6 +255,32 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
   for (int i = 0; i < ms; ++i)
     if (t < ms) A[i * ld + t] = Fs[(long long)i * ms + t];
   __syncthreads();
+  // the children's Schur complements, one child after another (a child's
+  // update has at most ms rows and columns: lane t adds column t)
+  for (int c = d.cptr[s]; c < d.cptr[s + 1]; ++c) {
+    const int ch = d.clist[c];
+    const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
+    const int* em = d.emap + d.eptr[ch];
+    const T* Fc = F + d.fo[ch];
+    const int eb = t < nu ? em[t] : 0;     // lane t: parent position of the child's row/column t
+    const T* cb = Fc + (long long)npc * mc + npc + t;
+    int a = 0;
+    for (; a + 4 <= nu; a += 4) {
+      T v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = t < nu ? cb[(long long)(a + q) * mc] : (T)0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int pr = __builtin_amdgcn_readlane(eb, a + q);
+        if (t < nu) A[pr * ld + eb] += v[q];
+      }
+    }
+    for (; a < nu; ++a) {
+      const int pr = __builtin_amdgcn_readlane(eb, a);
+      if (t < nu) A[pr * ld + eb] += cb[(long long)a * mc];
+    }
+    __syncthreads();
+  }
   bool fail = false;
   for (int k = 0; k < np; ++k) {
     const T piv = A[k * ld + k];
@@ -441,6 +477,22 @@ __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int
   }
 }
 
+// sum over c in [c0, c1) of row[c] v[c] by one wavefront (lanes over c, four
+// loads in flight per lane), returned in every lane
+template <class T>
+__device__ __forceinline__ T mf_row_dot(const T* row, const T* v, int c0, int c1, int lane) {
+  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
+  int c = c0 + lane;
+  for (; c + 192 < c1; c += 256) {
+    a0 += row[c] * v[c];
+    a1 += row[c + 64] * v[c + 64];
+    a2 += row[c + 128] * v[c + 128];
+    a3 += row[c + 192] * v[c + 192];
+  }
+  for (; c < c1; c += 64) a0 += row[c] * v[c];
+  return mf_wave_sum((a0 + a1) + (a2 + a3));
+}
+
 // ---- solves.  V holds, per front, the vector of its index list: y on the
 // pivot rows, the update for the parent below them.
 
@@ -501,10 +553,7 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int
     const int bw = min(64, np - b0);
     if (b0 > 0) {
       for (int rr = wave; rr < bw; rr += nw) {
-        const T* Lr = Fs + (long long)(b0 + rr) * ms;
-        T acc = (T)0;
-        for (int c = lane; c < b0; c += 64) acc += Lr[c] * Vl[c];
-        acc = mf_wave_sum(acc);
+        const T acc = mf_row_dot(Fs + (long long)(b0 + rr) * ms, Vl, 0, b0, lane);
         if (lane == 0) Vl[b0 + rr] -= acc;
       }
     }
@@ -524,10 +573,7 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int
     __syncthreads();
   }
   for (int rr = np + wave; rr < ms; rr += nw) {
-    const T* Lr = Fs + (long long)rr * ms;
-    T acc = (T)0;
-    for (int c = lane; c < np; c += 64) acc += Lr[c] * Vl[c];
-    acc = mf_wave_sum(acc);
+    const T acc = mf_row_dot(Fs + (long long)rr * ms, Vl, 0, np, lane);
     if (lane == 0) Vl[rr] -= acc;
   }
   __syncthreads();
@@ -574,10 +620,7 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int
   for (int i = t; i < ms; i += blockDim.x) Xl[i] = x[ix[i]];
   __syncthreads();
   for (int rr = wave; rr < np; rr += nw) {
-    const T* Ur = Fs + (long long)rr * ms;
-    T acc = (T)0;
-    for (int c = np + lane; c < ms; c += 64) acc += Ur[c] * Xl[c];
-    acc = mf_wave_sum(acc);
+    const T acc = mf_row_dot(Fs + (long long)rr * ms, Xl, np, ms, lane);
     if (lane == 0) Xl[rr] -= acc;
   }
   __syncthreads();
@@ -585,10 +628,7 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int
     const int b0 = max(0, b1 - 64), bw = b1 - b0;
     if (b1 < np) {
       for (int rr = wave; rr < bw; rr += nw) {
-        const T* Ur = Fs + (long long)(b0 + rr) * ms;
-        T acc = (T)0;
-        for (int c = b1 + lane; c < np; c += 64) acc += Ur[c] * Xl[c];
-        acc = mf_wave_sum(acc);
+        const T acc = mf_row_dot(Fs + (long long)(b0 + rr) * ms, Xl, b1, np, lane);
         if (lane == 0) Xl[b0 + rr] -= acc;
       }
     }
@@ -659,14 +699,14 @@ struct MfLu::Impl {
   DevVec<long long> fo, amap;
   DevVec<double> F64, V64, y64, b64;
   DevVec<float> F32, V32, y32;
-  DevVec<int> tiny_list, big_list, ea_list, sparent;
+  DevVec<int> tiny_list, big_list;
+  DevVec<int4> aent;              // assembly entries of the larger fronts {front, first row, alo offset, 0}
+  DevVec<int> alo;                // per entry and child: first child row at or after the entry's first row
   DevVec<int4> pent, uent;        // panel and update launch entries
   DevVec<double> dbuf;            // diagonal-block factors, one 32x32 slot per front of a panel launch
   std::vector<int> tiny_off, tiny_maxm, big_off;
   int big_maxm = 0;
-  // extend-add launches: per level, per child index j: [offset, count, max rows]
-  struct EaLaunch { int off, count, max_nu; };
-  std::vector<std::vector<EaLaunch>> ea_of_level;
+  std::vector<int> aoff;          // per level: offset into aent
   // blocked launches: per level, per panel index j
   struct PanelLaunch { int p0, poff, pcnt, uoff, ucnt; };
   std::vector<std::vector<PanelLaunch>> pl_of_level;
@@ -693,8 +733,10 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   for (int c : P.clist) if (c < 0 || c >= P.ns) return XG_E_SETUP;
   for (int t = 0; t < P.ns; ++t) {
     if (P.m[t] < P.sfirst[t + 1] - P.sfirst[t] || P.sfirst[t + 1] <= P.sfirst[t]) return XG_E_SETUP;
-    for (int e = P.eptr[t]; e < P.eptr[t + 1]; ++e)
+    for (int e = P.eptr[t]; e < P.eptr[t + 1]; ++e) {
       if (P.sparent[t] < 0 || P.emap[e] < 0 || P.emap[e] >= P.m[P.sparent[t]]) return XG_E_SETUP;
+      if (e > P.eptr[t] && P.emap[e] <= P.emap[e - 1]) return XG_E_SETUP;   // assembly bisects the map
+    }
   }
   hipStream_t st = I.st;
   MFCHK(I.sfirst.up(P.sfirst, st)); MFCHK(I.m.up(P.m, st)); MFCHK(I.iptr.up(P.iptr, st)); MFCHK(I.idx.up(P.idx, st));
@@ -763,32 +805,30 @@ int MfLu::setup(const LuSymbolic& s, const int* rowptr, const int* colind, void*
   MFCHK(I.pent.up(pe, st));
   MFCHK(I.uent.up(ue, st));
   MFCHK(I.dbuf.alloc((size_t)max_slots * kMfPanel * kMfPanel));
-  // extend-add launches: the j-th child of every front of a level together
-  std::vector<int> ea;
-  I.ea_of_level.assign(nl, {});
-  for (int l = 1; l < nl; ++l) {
-    int maxc = 0;
-    for (int q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
-      const int t = P.lvl_s[q];
-      maxc = std::max(maxc, P.cptr[t + 1] - P.cptr[t]);
-    }
-    for (int j = 0; j < maxc; ++j) {
-      Impl::EaLaunch L{(int)ea.size(), 0, 0};
-      for (int q = P.lvl_ptr[l]; q < P.lvl_ptr[l + 1]; ++q) {
-        const int t = P.lvl_s[q];
-        if (P.cptr[t + 1] - P.cptr[t] <= j) continue;
-        const int ch = P.clist[P.cptr[t] + j];
-        const int nu = P.m[ch] - (P.sfirst[ch + 1] - P.sfirst[ch]);
-        if (nu == 0) continue;
-        ea.push_back(ch);
-        L.count++;
-        L.max_nu = std::max(L.max_nu, nu);
+  // assembly of the larger fronts: 32 parent rows per workgroup (the tiny
+  // fronts assemble their children inside their factor kernel)
+  std::vector<int4> ae;
+  std::vector<int> lo;
+  I.aoff.assign(nl + 1, 0);
+  for (int l = 0; l < nl; ++l) {
+    I.aoff[l] = (int)ae.size();
+    for (int b = I.big_off[l]; b < I.big_off[l + 1]; ++b) {
+      const int t = bl[b];
+      if (P.cptr[t + 1] == P.cptr[t]) continue;
+      for (int r0 = 0; r0 < P.m[t]; r0 += kMfAsmRows) {
+        ae.push_back(make_int4(t, r0, (int)lo.size(), 0));
+        for (int c = P.cptr[t]; c < P.cptr[t + 1]; ++c) {
+          const int ch = P.clist[c];
+          const int* eb = P.emap.data() + P.eptr[ch];
+          const int* ee = P.emap.data() + P.eptr[ch + 1];
+          lo.push_back((int)(std::lower_bound(eb, ee, r0) - eb));
+        }
       }
-      if (L.count) I.ea_of_level[l].push_back(L);
     }
   }
-  MFCHK(I.ea_list.up(ea, st));
-  MFCHK(I.sparent.up(P.sparent, st));
+  I.aoff[nl] = (int)ae.size();
+  MFCHK(I.aent.up(ae, st));
+  MFCHK(I.alo.up(lo, st));
   MFCHK(hipStreamSynchronize(st));
   return XG_OK;
 }
@@ -811,9 +851,8 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
   T* dbuf = reinterpret_cast<T*>(I.dbuf.p);
   const int nl = (int)P.lvl_ptr.size() - 1;
   for (int l = 0; l < nl; ++l) {
-    for (const auto& L : I.ea_of_level[l])
-      hipLaunchKernelGGL(k_mf_extend_add<T>, dim3(L.count, (L.max_nu + 15) / 16), dim3(256), 0, st, d,
-                         I.ea_list.p + L.off, I.sparent.p, F);
+    const int na = I.aoff[l + 1] - I.aoff[l];
+    if (na) hipLaunchKernelGGL(k_mf_assemble<T>, dim3(na), dim3(256), 0, st, d, I.aent.p + I.aoff[l], I.alo.p, F);
     const int nt = I.tiny_off[l + 1] - I.tiny_off[l];
     if (nt) {
       const int mm = I.tiny_maxm[l];
