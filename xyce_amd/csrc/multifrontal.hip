@@ -228,7 +228,21 @@ __global__ void __launch_bounds__(256) k_mf_assemble(MfDev d, const int4* ent, c
       if (pr >= r1) break;
       const T* crow = Fc + (long long)(npc + a) * mc + npc;
       T* prow = Fs + (long long)pr * ms;
-      for (int b = lane; b < nu; b += 64) prow[em[b]] += crow[b];
+      for (int b0 = lane; b0 < nu; b0 += 256) {
+        int eb[4];
+        T cv[4], pv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int b = b0 + 64 * q;
+          eb[q] = b < nu ? em[b] : 0;
+          cv[q] = b < nu ? crow[b] : (T)0;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pv[q] = b0 + 64 * q < nu ? prow[eb[q]] : (T)0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (b0 + 64 * q < nu) prow[eb[q]] = pv[q] + cv[q];
+      }
     }
     __syncthreads();
   }
@@ -252,8 +266,14 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
   const int ld = ms + 1;
   T* Fs = F + d.fo[s];
   const int t = threadIdx.x;
-  for (int i = 0; i < ms; ++i)
-    if (t < ms) A[i * ld + t] = Fs[(long long)i * ms + t];
+  for (int i0 = 0; i0 < ms; i0 += 8) {
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = (t < ms && i0 + q < ms) ? Fs[(long long)(i0 + q) * ms + t] : (T)0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (t < ms && i0 + q < ms) A[(i0 + q) * ld + t] = v[q];
+  }
   __syncthreads();
   // the children's Schur complements, one child after another (a child's
   // update has at most ms rows and columns: lane t adds column t)
@@ -373,9 +393,16 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
   if (rest > 0 && blk < nrb) {
     // L21 rows [r0, r0 + 64), staged through LDS for coalesced access
     const int r0 = base + blk * 64;
-    for (int it = 0; it < kMfPanel; ++it) {
-      const int rr = (t >> 5) + 2 * it, q = t & 31;
-      Tl[rr][q] = (r0 + rr < ms && q < pw) ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
+    {
+      // all loads issued before the LDS stores (one wait, not one per row)
+      T v[kMfPanel];
+#pragma unroll
+      for (int it = 0; it < kMfPanel; ++it) {
+        const int rr = (t >> 5) + 2 * it, q = t & 31;
+        v[it] = (r0 + rr < ms && q < pw) ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
+      }
+#pragma unroll
+      for (int it = 0; it < kMfPanel; ++it) Tl[(t >> 5) + 2 * it][t & 31] = v[it];
     }
     __syncthreads();
     T a[kMfPanel];
@@ -395,6 +422,7 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
 #pragma unroll
     for (int q = 0; q < kMfPanel; ++q) Tl[t][q] = a[q];
     __syncthreads();
+#pragma unroll
     for (int it = 0; it < kMfPanel; ++it) {
       const int rr = (t >> 5) + 2 * it, q = t & 31;
       if (r0 + rr < ms && q < pw) Fs[(long long)(r0 + rr) * ms + p0 + q] = Tl[rr][q];
@@ -450,13 +478,25 @@ __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int
   T acc[4][4];
   for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
   for (int q0 = 0; q0 < pw; q0 += KB) {
-    for (int x = threadIdx.x; x < TB * KB; x += 256) {
-      const int r = x / KB, q = x % KB;
-      const int gi = i0 + r, gq = p0 + q0 + q;
-      Ls[r][q] = (gi < ms && q0 + q < pw) ? Fs[(long long)gi * ms + gq] : (T)0;
-      const int qq = x / TB, c = x % TB;
-      const int gj = j0 + c, gq2 = p0 + q0 + qq;
-      Us[qq][c] = (gj < ms && q0 + qq < pw) ? Fs[(long long)gq2 * ms + gj] : (T)0;
+    {
+      // all loads issued before the LDS stores
+      T lv[TB * KB / 256], uv[TB * KB / 256];
+#pragma unroll
+      for (int it = 0; it < TB * KB / 256; ++it) {
+        const int x = threadIdx.x + it * 256;
+        const int r = x / KB, q = x % KB;
+        const int gi = i0 + r, gq = p0 + q0 + q;
+        lv[it] = (gi < ms && q0 + q < pw) ? Fs[(long long)gi * ms + gq] : (T)0;
+        const int qq = x / TB, c = x % TB;
+        const int gj = j0 + c, gq2 = p0 + q0 + qq;
+        uv[it] = (gj < ms && q0 + qq < pw) ? Fs[(long long)gq2 * ms + gj] : (T)0;
+      }
+#pragma unroll
+      for (int it = 0; it < TB * KB / 256; ++it) {
+        const int x = threadIdx.x + it * 256;
+        Ls[x / KB][x % KB] = lv[it];
+        Us[x / TB][x % TB] = uv[it];
+      }
     }
     __syncthreads();
     for (int q = 0; q < KB; ++q) {
@@ -557,9 +597,18 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int
         if (lane == 0) Vl[b0 + rr] -= acc;
       }
     }
-    for (int x = t; x < bw * 64; x += blockDim.x) {
-      const int i = x >> 6, j = x & 63;
-      if (j < i) Bl[i * 65 + j] = Fs[(long long)(b0 + i) * ms + b0 + j];
+    {
+      T v[64 * 64 / kMfSolveBlock];
+#pragma unroll
+      for (int it = 0; it < 64 * 64 / kMfSolveBlock; ++it) {
+        const int x = t + it * kMfSolveBlock, i = x >> 6, j = x & 63;
+        v[it] = (i < bw && j < i) ? Fs[(long long)(b0 + i) * ms + b0 + j] : (T)0;
+      }
+#pragma unroll
+      for (int it = 0; it < 64 * 64 / kMfSolveBlock; ++it) {
+        const int x = t + it * kMfSolveBlock;
+        Bl[(x >> 6) * 65 + (x & 63)] = v[it];
+      }
     }
     __syncthreads();
     if (wave == 0) {
@@ -594,8 +643,18 @@ __global__ void __launch_bounds__(64) k_mf_bwd_tiny(MfDev d, const int* list, co
   if (t < ms) Xl[t] = x[ix[t]];           // y on the pivot rows, solved x below
   __syncthreads();
   T v = Xl[t];
-  if (t < np)
-    for (int u = np; u < ms; ++u) v -= Fs[(long long)t * ms + u] * Xl[u];
+  if (t < np) {
+    const T* row = Fs + (long long)t * ms;
+    int u = np;
+    for (; u + 4 <= ms; u += 4) {
+      const T f0 = row[u], f1 = row[u + 1], f2 = row[u + 2], f3 = row[u + 3];
+      v -= f0 * Xl[u];
+      v -= f1 * Xl[u + 1];
+      v -= f2 * Xl[u + 2];
+      v -= f3 * Xl[u + 3];
+    }
+    for (; u < ms; ++u) v -= row[u] * Xl[u];
+  }
   for (int k = np - 1; k >= 0; --k) {
     const T xk = mf_bcast(v, k) / Fs[(long long)k * ms + k];
     if (t == k) v = xk;
@@ -632,9 +691,18 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int
         if (lane == 0) Xl[b0 + rr] -= acc;
       }
     }
-    for (int q = t; q < bw * 64; q += blockDim.x) {
-      const int i = q >> 6, j = q & 63;
-      if (j >= i && j < bw) Bl[i * 65 + j] = Fs[(long long)(b0 + i) * ms + b0 + j];
+    {
+      T v[64 * 64 / kMfSolveBlock];
+#pragma unroll
+      for (int it = 0; it < 64 * 64 / kMfSolveBlock; ++it) {
+        const int x = t + it * kMfSolveBlock, i = x >> 6, j = x & 63;
+        v[it] = (i < bw && j >= i && j < bw) ? Fs[(long long)(b0 + i) * ms + b0 + j] : (T)0;
+      }
+#pragma unroll
+      for (int it = 0; it < 64 * 64 / kMfSolveBlock; ++it) {
+        const int x = t + it * kMfSolveBlock;
+        Bl[(x >> 6) * 65 + (x & 63)] = v[it];
+      }
     }
     __syncthreads();
     if (wave == 0) {
