@@ -44,6 +44,10 @@ struct SolverState {
   int order = 1;
   double h = 0.0;        // current time step
   double time = 0.0;     // t_{n+1}
+  // integration method: 0 = OneStep trapezoid, 1 = Gear12 (BDF1/BDF2), whose
+  // residual weights the charges by alpha0..alpha2 (N_TIA_Gear12.C:1050-1111)
+  int method = 0;
+  double a0 = 1.0, a1 = -1.0, a2 = 0.0;
 };
 
 struct Options {

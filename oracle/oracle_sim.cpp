@@ -189,6 +189,7 @@ struct Work {
   // else probe_k rows of probe_w (row-major, n each): the products w_r . x
   int probe_k = 0;
   std::vector<double> probe_w;
+  int method = 0;   // 0 = OneStep trapezoid, 1 = Gear12 (.OPTIONS TIMEINT METHOD=GEAR)
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
     x.assign(n + 1, 0.0); F.assign(n + 1, 0.0); Q.assign(n + 1, 0.0); B.assign(n + 1, 0.0);
@@ -279,8 +280,25 @@ static void load_jacobian(Work& w, const SolverState& ss) {
 
 // OneStep::obtainResidual N_TIA_OneStep.C:219-289 and NoTimeIntegration::obtainResidual
 // (N_TIA_NoTimeIntegration.C:161).  Epetra Update(a,A,b,B,c) is c*this + a*A + b*B.
+// Gear12::obtainResidual (N_TIA_Gear12.C:208-263): q0 = qHistory[0] (Q_n),
+// q2 = qHistory[1] (Q_{n-1}).
+static void combine_residual_gear(Work& w, const SolverState& ss, const double* q0, const double* q1) {
+  const int n = w.n;
+  const double h = ss.h;
+  for (int i = 0; i < n; ++i) {
+    double r = ss.a0 * w.Q[i] + ss.a1 * q0[i];
+    if (ss.order == 2) r = r + ss.a2 * q1[i];
+    r = (1.0 / h) * r + 1.0 * w.F[i] + -1.0 * w.B[i];
+    r *= -1.0;
+    r = r + 1.0 * ((ss.a0 / h) * w.dQdxdVp[i]);
+    r = r + 1.0 * w.dFdxdVp[i];
+    w.RHS[i] = r;
+  }
+}
+
 static void combine_residual(Work& w, const SolverState& ss, const double* q0, const double* q2) {
   const int n = w.n;
+  if (!ss.dcop && ss.method == 1) { combine_residual_gear(w, ss, q0, q2); return; }
   if (ss.dcop) {
     for (int i = 0; i < n; ++i) {
       double r = 1.0 * w.F[i] + -1.0 * w.B[i];
@@ -313,6 +331,7 @@ static void combine_residual(Work& w, const SolverState& ss, const double* q0, c
 static void combine_jacobian(Work& w, const SolverState& ss) {
   double a, b;
   if (ss.dcop) { a = 1.0e-20; b = 1.0; }
+  else if (ss.method == 1) { a = ss.a0 / ss.h; b = 1.0; }    // Gear12::obtainJacobian :477-505
   else { a = -(-1.0) / ss.h; b = (ss.order == 2) ? 1.0 / 2.0 : 1.0; }
   for (int k = 0; k < w.nnz; ++k) w.J[k] = a * w.dQdx[k] + b * w.dFdx[k];
 }
@@ -560,7 +579,12 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
     hh = std::max(hh, hmin);
     h = hh;
     lastTimeStep = h; currentTimeStepSum = 2.0 * h;
-    for (int i = 0; i < n; ++i) { xH1[i] = 0.0; qH1[i] = -(F0[i] - B0[i]) * h; }
+    if (w.method == 1) {
+      // Gear12::initialize (N_TIA_Gear12.C:1272-1285): both history slots hold the current point
+      for (int i = 0; i < n; ++i) { xH1[i] = xH0[i]; qH1[i] = qH0[i]; }
+    } else {
+      for (int i = 0; i < n; ++i) { xH1[i] = 0.0; qH1[i] = -(F0[i] - B0[i]) * h; }
+    }
     numberOfSteps = 0; order = 1; usedOrder = 1; psi[0] = h;
     beginning = true;
   };
@@ -576,8 +600,28 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
     double temp1 = h;
     if (order == 2) psi[2] = psi[1];
     psi[1] = psi[0]; psi[0] = temp1;
-    double beta1, beta2 = 0.0, ck;
-    if (order == 2) {
+    double beta0 = 1.0, beta1, beta2 = 0.0, ck;
+    double a0 = 1.0, a1 = -1.0, a2 = 0.0;
+    if (w.method == 1) {
+      // Gear12::updateCoeffs (N_TIA_Gear12.C:1050-1111): BDF2 predictor and
+      // corrector weights for variable steps, or BDF1
+      if (order == 2) {
+        beta2 = temp1 / psi[2] * (temp1 + psi[1]) / (psi[1] + psi[2]);
+        beta1 = -temp1 / psi[1] - beta2 * (psi[1] + psi[2]) / psi[1];
+        beta0 = 1.0 - beta2 - beta1;
+        a2 = -temp1 / psi[1] * temp1 / (2 * temp1 + psi[1]);
+        a1 = 1 - a2;
+        a0 = -a1 - a2 * (1 + psi[1] / temp1);
+        a2 = a2 / a0;
+        a1 = a1 / a0;
+        a0 = -1 / a0;
+        ck = h / (temp1 + psi[1] + psi[2]);
+      } else {
+        beta0 = 1.0 + temp1 / psi[1];
+        beta1 = -temp1 / psi[1];
+        ck = h / (temp1 + psi[1]);
+      }
+    } else if (order == 2) {
       double temp2 = psi[1];
       beta1 = temp1 / temp2 + (temp1 / temp2) * (temp1 / temp2) / 2;
       beta2 = -1.0 * temp1 * temp1 / temp2 / psi[2] / 2;
@@ -586,11 +630,19 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
       beta1 = temp1 / psi[1];
       ck = h / currentTimeStepSum;
     }
-    // predictor
+    // predictor: OneStep x_n + sum beta_i (difference history); Gear12
+    // sum beta_i x_{n-i} (N_TIA_Gear12.C:131-141)
     for (int i = 0; i < n; ++i) {
-      double v = xH0[i];
-      v += beta1 * xH1[i];
-      if (order == 2) v += beta2 * xH2[i];
+      double v;
+      if (w.method == 1) {
+        v = beta0 * xH0[i];
+        v = beta1 * xH1[i] + v;
+        if (order == 2) v = beta2 * xH2[i] + v;
+      } else {
+        v = xH0[i];
+        v += beta1 * xH1[i];
+        if (order == 2) v += beta2 * xH2[i];
+      }
       xn0[i] = v;
       xcurr[i] = xH0[i];
     }
@@ -598,7 +650,8 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
     for (int i = 0; i < n; ++i) errwt[i] = reltol * mx + abstol;
     for (int i = 0; i < n; ++i) w.x[i] = xn0[i];
     ss.order = order; ss.h = h; ss.time = t + h;
-    NewtonStats ns = newton(w, ss, p, xcurr, qH0.data(), qH2.data(), &R.loads);
+    ss.method = w.method; ss.a0 = a0; ss.a1 = a1; ss.a2 = a2;
+    NewtonStats ns = newton(w, ss, p, xcurr, qH0.data(), w.method == 1 ? qH1.data() : qH2.data(), &R.loads);
     R.newton_iters += ns.iters;
     bool ok = ns.status > 0;
     if (ok && !beginning) {
@@ -624,14 +677,22 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
       if (rr >= r_hincr_test) { rr = r_hincr; newh = rr * h; }
       else if (rr <= 1) { rr = std::max(r_min, std::min(r_max, rr)); newh = rr * h; }
       // updateHistory
-      if (order == 2) {
-        xH2 = xH1;
-        for (int i = 0; i < n; ++i) qH2[i] = 1.0 * w.F[i] + -1.0 * w.B[i];
-      }
-      for (int i = 0; i < n; ++i) {
-        xH1[i] = 1.0 * w.x[i] + -1.0 * xH0[i];
-        qH1[i] = 1.0 * w.Q[i] + -1.0 * qH0[i];
-        xH0[i] = w.x[i]; qH0[i] = w.Q[i];
+      if (w.method == 1) {
+        // Gear12::updateHistory (N_TIA_Gear12.C:822-857): shift the values
+        if (order == 2) xH2 = xH1;
+        qH1 = qH0;
+        xH1 = xH0;
+        for (int i = 0; i < n; ++i) { xH0[i] = w.x[i]; qH0[i] = w.Q[i]; }
+      } else {
+        if (order == 2) {
+          xH2 = xH1;
+          for (int i = 0; i < n; ++i) qH2[i] = 1.0 * w.F[i] + -1.0 * w.B[i];
+        }
+        for (int i = 0; i < n; ++i) {
+          xH1[i] = 1.0 * w.x[i] + -1.0 * xH0[i];
+          qH1[i] = 1.0 * w.Q[i] + -1.0 * qH0[i];
+          xH0[i] = w.x[i]; qH0[i] = w.Q[i];
+        }
       }
       w.staCur = w.staNext; w.stoCur = w.stoNext;
       t = tnew;
@@ -658,7 +719,9 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
       double newh;
       if (ns.status <= 0) { newh = h / 8; order = 1; }
       else if (nef == 1) {
-        order = 1;
+        // OneStep restarts at order 1; Gear12 keeps its order for the first
+        // local-error failure (N_TIA_Gear12.C:1502-1516)
+        if (w.method != 1) order = 1;
         double rr = tolAimFac / (estOverTol + 0.0001);
         rr = std::pow(rr, 1.0 / (order + 1.0));
         rr = std::max(r_min, std::min(r_max, rr));
@@ -902,6 +965,14 @@ void orc_tran_orders(OrcHandle* h, int* ord) {
 
 // Record k products w_r . x per accepted point instead of the solution
 // (k = 0 restores whole solutions); w is k x n, row-major.
+// Integration method of later orc_tran calls: 0 = OneStep trapezoid (the
+// default), 1 = Gear12.
+int orc_set_method(OrcHandle* h, int method) {
+  if (method != 0 && method != 1) return -1;
+  h->w->method = method;
+  return 0;
+}
+
 int orc_set_probe(OrcHandle* h, int k, const double* w) {
   if (k < 0 || (k > 0 && !w)) return -1;
   h->w->probe_k = k;

@@ -41,6 +41,8 @@ def lib():
         L.orc_tran_orders.argtypes = [P, P]
         L.orc_set_probe.argtypes = [P, C.c_int, P]
         L.orc_set_probe.restype = C.c_int
+        L.orc_set_method.argtypes = [P, C.c_int]
+        L.orc_set_method.restype = C.c_int
         for nm in ("orc_limvds", "orc_fetlim", "orc_qmeyer", "orc_pnjlim"):
             getattr(L, nm).restype = C.c_int
         _LIB = L
@@ -148,6 +150,12 @@ class Oracle:
     def set_tolerances(self, tia_reltol=1e-3, tia_abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
         lib().orc_set_tolerances(self.h, C.c_double(tia_reltol), C.c_double(tia_abstol), C.c_double(nl_reltol),
                                  C.c_double(nl_abstol))
+
+    def set_method(self, method):
+        """Integration method of later transients: 'trap' (OneStep, the
+        default) or 'gear' (Gear12)."""
+        m = {"trap": 0, "gear": 1}[method]
+        assert lib().orc_set_method(self.h, m) == 0
 
     def set_probe(self, w=None):
         """Record, per accepted point, the products of the rows of ``w``

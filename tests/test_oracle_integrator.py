@@ -203,3 +203,65 @@ def test_power_grid_charge_balance_and_planted_mistake():
     flipped["order"] = np.where(r["order"] == 2, 1, 2)
     flipped["order"][0] = 0
     assert grid_charge_balance(c, o, flipped, 1e-12, 0.4e-9) > 100.0
+
+
+def _bdf_defects(r, col_a, col_b, R, cap):
+    """Per accepted step of an RC run (source node a, capacitor node b), the
+    defect of the backward-differentiation condition that defines the step:
+    the derivative at t_k of the polynomial through the last order+1 points
+    of C v_b (numpy.polyfit) must equal the capacitor current (v_a - v_b)/R at
+    t_k.  Variable-step BDF2 is exactly this interpolation condition."""
+    t, x, order = r["t"], r["x"], r["order"]
+    out = []
+    for k in range(1, len(t)):
+        o = int(order[k])
+        if o == 2 and k < 2:
+            continue
+        ts = t[k - o:k + 1] - t[k]
+        qs = cap * x[k - o:k + 1, col_b]
+        coef = np.polyfit(ts / ts[0], qs, o)            # scaled abscissa for conditioning
+        dq = coef[-2] / ts[0]                           # derivative at t_k (ts = 0)
+        cur = (x[k, col_a] - x[k, col_b]) / R
+        out.append(abs(dq - cur) / (abs(cur) + 1e-6 / R))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("method", ["gear", "trap"])
+def test_gear12_steps_are_variable_step_bdf(method):
+    """Gear12 (.OPTIONS TIMEINT METHOD=GEAR): every accepted step of an RC
+    circuit satisfies the variable-step BDF1/BDF2 interpolation condition to
+    round-off; the trapezoid run does not (the check tells them apart)."""
+    R, cap = 1e3, 1e-12
+    c, a, b = _rc([0.0, 0.0, 0.2e-9, 0.0, 0.25e-9, 1.0, 1.5e-9, 1.0, 1.7e-9, 0.2], r=R, cap=cap)
+    o = Oracle(c)
+    o.set_method(method)
+    r = o.tran(4e-9, 1e-12, 0.0)
+    assert r["status"] == 0
+    assert set(np.unique(r["order"][1:])) == {1, 2}
+    d = _bdf_defects(r, _node_col(o, c, a), _node_col(o, c, b), R, cap)
+    if method == "gear":
+        assert np.max(d) <= 1e-10, np.max(d)
+    else:
+        assert np.max(d) > 1e-3
+
+
+def test_gear12_rc_closed_forms():
+    """Gear12 on the RC ramp and discharge: the closed-form responses within
+    the local-error tolerance, breakpoints hit exactly."""
+    t0, tr, v = 0.2e-9, 50e-12, 1.0
+    c, a, b = _rc([0.0, 0.0, t0, 0.0, t0 + tr, v])
+    o = Oracle(c)
+    o.set_method("gear")
+    r = o.tran(5e-9, 1e-12, 0.0)
+    assert r["status"] == 0
+    ref = _ramp_response(r["t"], t0, tr, v, 1e-9)
+    assert np.max(np.abs(r["x"][:, _node_col(o, c, b)] - ref)) <= 2e-3 * v
+    for bp in (t0, t0 + tr):
+        assert np.min(np.abs(r["t"] - bp)) <= 1e-21
+    c, a, b = _rc([0.0, 1.0, 0.1e-9, 1.0, 0.101e-9, 0.0], r=2e3, cap=0.5e-12)
+    o = Oracle(c)
+    o.set_method("gear")
+    r = o.tran(4e-9, 1e-12, 0.0)
+    m = r["t"] >= 0.106e-9
+    ref = np.exp(-(r["t"][m] - 0.1005e-9) / 1e-9)
+    assert np.max(np.abs(r["x"][m, _node_col(o, c, b)] - ref)) <= 3e-3
