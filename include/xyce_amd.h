@@ -195,6 +195,12 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
  * instances' LDS-sized groups run side by side on streams of their own, 0:
  * one after another, -1 = keep. */
 int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups);
+/* Time-integration method of every instance: 0 = trapezoid (OneStep, the
+ * default), 1 = Gear12 (variable-step BDF1/BDF2), as .OPTIONS TIMEINT
+ * METHOD=TRAP|GEAR selects in the reference (src/AnalysisPKG/N_ANP_Transient.C:384-387;
+ * src/TimeIntegrationPKG/N_TIA_Gear12.C replaces N_TIA_OneStep.C).  Only before
+ * the first launch (xg_batch_advance / xg_batch_autotune): XG_E_ARG after. */
+int xg_batch_set_method(xg_batch* b, int method);
 /* Per-phase device timers of the first-scheduled instance (the counterpart of
  * the reference's Residual / Jacobian / Linear-solve StatTop timers,
  * src/UtilityPKG/N_UTL_Stats.h:481-511): enable = 1 resets and starts,

@@ -132,3 +132,16 @@ def test_prn_round_trip(tmp_path):
     np.testing.assert_array_equal(data[:, 0], np.arange(11))
     np.testing.assert_allclose(data[:, 1], t, rtol=1e-8, atol=1e-30)
     np.testing.assert_allclose(data[:, 2], cols["V(OUT)"], rtol=1e-8, atol=1e-12)
+
+
+@pytest.mark.parametrize("line,method", [(".OPTIONS TIMEINT METHOD=GEAR", "gear"), (".options timeint method=8", "gear"),
+                                         (".OPTIONS TIMEINT METHOD=TRAP", "trap"), (".OPTIONS TIMEINT RELTOL=1e-3", "trap"),
+                                         (".OPTIONS DEVICE GMIN=1e-12", "trap")])
+def test_integration_method_option(line, method):
+    _, an = parse_netlist("t\nR1 a 0 1k\nV1 a 0 DC 1\n" + line + "\n.TRAN 1p 1n\n.END\n")
+    assert an["method"] == method
+
+
+def test_unknown_integration_method_is_rejected():
+    with pytest.raises(NetlistError):
+        parse_netlist("t\nR1 a 0 1k\n.OPTIONS TIMEINT METHOD=BDF5\n.END\n")

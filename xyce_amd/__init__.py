@@ -41,7 +41,7 @@ EXPORTS = [
     "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
-    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
+    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
 ]
@@ -288,7 +288,9 @@ class DirectSolver:
 class Batch:
     """Batched independent circuits (one workgroup each) on the GPU."""
 
-    def __init__(self, circuits_list, tstop, tstep, hmax, record_max=0, gmin=1e-12, temp=300.15):
+    def __init__(self, circuits_list, tstop, tstep, hmax, record_max=0, gmin=1e-12, temp=300.15, method="trap"):
+        """``method``: 'trap' (OneStep trapezoid, the default) or 'gear'
+        (Gear12, .OPTIONS TIMEINT METHOD=GEAR)."""
         self._keep = []
         descs = (CircuitDesc * len(circuits_list))()
         for i, c in enumerate(circuits_list):
@@ -302,6 +304,9 @@ class Batch:
         self.h = h
         self.n_inst = len(circuits_list)
         self.record_max = record_max
+        m = {"trap": 0, "gear": 1}[method]
+        if m:
+            _check(lib().xg_batch_set_method(self.h, C.c_int(m)), "xg_batch_set_method")
 
     def close(self):
         if getattr(self, "h", None):

@@ -87,8 +87,13 @@ struct InstState {
   // polynomial predictor
   double it_learned, st_learned, it_poly, st_poly;
   long long n_choice;
-  double pad[22];
+  // Gear12: predictor weight of x_n and the corrector weights of Q_{n+1},
+  // Q_n, Q_{n-1} (N_TIA_Gear12.C:1050-1111)
+  double beta0, a0, a1, a2;
+  double pad[18];
 };
+
+static_assert(sizeof(InstState) == 512, "InstState is one 512-byte record");
 
 struct InstDesc {
   int cls, dev_off, vec_off, nnz_off, src_off, rec_off;
@@ -123,6 +128,7 @@ struct BatchArgs {
   const double* bp;
   // transient parameters
   double tstop, tstep, hmax;
+  int method;                  // 0 = OneStep trapezoid, 1 = Gear12 (BDF1/BDF2)
   // recording
   int rec_max; double* rec_t; double* rec_x;
   int max_n, max_nlu, max_blob;   // over all instances (max_n: stride of the recorded points)
@@ -204,6 +210,17 @@ __device__ __forceinline__ double residual_combine(bool dc, int order, double h,
     res = res + 1.0 * ((1.0 / h) * ql);
     res = res + (order == 2 ? 0.5 : 1.0) * fl;
   }
+  return res;
+}
+// Gear12::obtainResidual (N_TIA_Gear12.C:208-263) rounding: q0 = Q_n, q1 = Q_{n-1}
+__device__ __forceinline__ double residual_gear(int order, double h, double a0, double a1, double a2, double f,
+                                                double q, double b, double fl, double ql, double q0, double q1) {
+  double res = a0 * q + a1 * q0;
+  if (order == 2) res = res + a2 * q1;
+  res = (1.0 / h) * res + 1.0 * f + -1.0 * b;
+  res *= -1.0;
+  res = res + 1.0 * ((a0 / h) * ql);
+  res = res + 1.0 * fl;
   return res;
 }
 #pragma clang fp contract(fast)
@@ -357,7 +374,25 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
     const double t1 = S.h;
     if (S.order == 2) S.psi2 = S.psi1;
     S.psi1 = S.psi0; S.psi0 = t1;
-    if (S.order == 2) {
+    S.beta0 = 1.0; S.a0 = 1.0; S.a1 = -1.0; S.a2 = 0.0;
+    if (A.method == 1) {
+      // Gear12::updateCoeffs: variable-step BDF2 predictor/corrector, or BDF1
+      if (S.order == 2) {
+        S.beta2 = t1 / S.psi2 * (t1 + S.psi1) / (S.psi1 + S.psi2);
+        S.beta1 = -t1 / S.psi1 - S.beta2 * (S.psi1 + S.psi2) / S.psi1;
+        S.beta0 = 1.0 - S.beta2 - S.beta1;
+        double a2 = -t1 / S.psi1 * t1 / (2 * t1 + S.psi1);
+        double a1 = 1 - a2;
+        double a0 = -a1 - a2 * (1 + S.psi1 / t1);
+        S.a2 = a2 / a0; S.a1 = a1 / a0; S.a0 = -1 / a0;
+        S.ck = S.h / (t1 + S.psi1 + S.psi2);
+      } else {
+        S.beta0 = 1.0 + t1 / S.psi1;
+        S.beta1 = -t1 / S.psi1;
+        S.beta2 = 0.0;
+        S.ck = S.h / (t1 + S.psi1);
+      }
+    } else if (S.order == 2) {
       const double t2 = S.psi1;
       S.beta1 = t1 / t2 + (t1 / t2) * (t1 / t2) / 2;
       S.beta2 = -1.0 * t1 * t1 / t2 / S.psi2 / 2;
@@ -368,17 +403,26 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
       S.ck = S.h / S.h_sum;
     }
     S.newton_iter = 0;
-    sh[0] = S.beta1; sh[1] = S.beta2; sh[2] = (double)S.order;
+    sh[0] = S.beta1; sh[1] = S.beta2; sh[2] = (double)S.order; sh[3] = S.beta0;
   }
   __syncthreads();
-  const double b1 = sh[0], b2 = sh[1];
+  const double b1 = sh[0], b2 = sh[1], b0 = sh[3];
   const int order = (int)sh[2];
+  const bool gear = A.method == 1;
   double mx = 0.0;
   const int off = I.vec_off;
   for (int i = threadIdx.x; i < C.n; i += blockDim.x) {
-    double v = A.xH0[off + i];
-    v += b1 * A.xH1[off + i];
-    if (order == 2) v += b2 * A.xH2[off + i];
+    double v;
+    if (gear) {
+      // Gear12::obtainPredictor: sum of beta_i x_{n-i}
+      v = b0 * A.xH0[off + i];
+      v = b1 * A.xH1[off + i] + v;
+      if (order == 2) v = b2 * A.xH2[off + i] + v;
+    } else {
+      v = A.xH0[off + i];
+      v += b1 * A.xH1[off + i];
+      if (order == 2) v += b2 * A.xH2[off + i];
+    }
     A.xn0[off + i] = v;
     A.x[off + i] = v;
     mx = fmax(mx, fabs(A.xH0[off + i]));
@@ -608,7 +652,9 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
         if (ent & 1) b -= v; else b += v;
       }
       A.F[off + r] = f; A.Q[off + r] = q; A.B[off + r] = b;
-      const double res = residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
+      const double res = (!dc && A.method == 1)
+                             ? residual_gear(S.order, h, S.a0, S.a1, S.a2, f, q, b, fl, ql, A.qH0[off + r], A.qH1[off + r])
+                             : residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
       rhs[r] = res;
       n2 += res * res;
       ninf = fmax(ninf, fabs(res));
@@ -667,8 +713,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       for (int i = tid; i < n; i += nth)
         A.wt[off + i] = kNlRel * fmax(fabs(A.xH0[off + i]), fabs(xs[i])) + kNlAbs;
     }
-    const double ja = dc ? 1.0e-20 : 1.0 / h;
-    const double jb = dc ? 1.0 : (S.order == 2 ? 0.5 : 1.0);
+    const double ja = dc ? 1.0e-20 : (A.method == 1 ? S.a0 / h : 1.0 / h);
+    const double jb = dc ? 1.0 : (A.method == 1 ? 1.0 : (S.order == 2 ? 0.5 : 1.0));
     bool use_dense = dc;
     if (!dc) {
       for (int p = tid; p < nlu; p += nth) lu[p] = 0.0;
@@ -864,7 +910,12 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       sh_d[0] = S.h;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += nth) A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[0];
+    if (A.method == 1) {
+      // Gear12::initialize: both history slots hold the operating point
+      for (int i = tid; i < n; i += nth) { A.xH1[off + i] = xs[i]; A.qH1[off + i] = A.Q[off + i]; }
+    } else {
+      for (int i = tid; i < n; i += nth) A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[0];
+    }
     if (A.rec_max > 0 && tid == 0 && S.n_rec < A.rec_max) A.rec_t[(size_t)I.rec_off + S.n_rec] = 0.0;
     if (A.rec_max > 0 && S.n_rec < A.rec_max)
       for (int i = tid; i < n; i += nth) A.rec_x[((size_t)I.rec_off + S.n_rec) * A.max_n + i] = xs[i];
@@ -909,16 +960,27 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }
     __syncthreads();
     const int new_order = sh_int[4];
-    for (int i = tid; i < n; i += nth) {
-      if (new_order == 2) {
-        A.xH2[off + i] = A.xH1[off + i];
-        A.qH2[off + i] = 1.0 * A.F[off + i] + -1.0 * A.B[off + i];
+    if (A.method == 1) {
+      // Gear12::updateHistory (N_TIA_Gear12.C:822-857): shift the values
+      for (int i = tid; i < n; i += nth) {
+        if (new_order == 2) A.xH2[off + i] = A.xH1[off + i];
+        A.qH1[off + i] = A.qH0[off + i];
+        A.xH1[off + i] = A.xH0[off + i];
+        A.xH0[off + i] = xs[i];
+        A.qH0[off + i] = A.Q[off + i];
       }
-      const double xv = xs[i], qv = A.Q[off + i];
-      A.xH1[off + i] = 1.0 * xv + -1.0 * A.xH0[off + i];
-      A.qH1[off + i] = 1.0 * qv + -1.0 * A.qH0[off + i];
-      A.xH0[off + i] = xv;
-      A.qH0[off + i] = qv;
+    } else {
+      for (int i = tid; i < n; i += nth) {
+        if (new_order == 2) {
+          A.xH2[off + i] = A.xH1[off + i];
+          A.qH2[off + i] = 1.0 * A.F[off + i] + -1.0 * A.B[off + i];
+        }
+        const double xv = xs[i], qv = A.Q[off + i];
+        A.xH1[off + i] = 1.0 * xv + -1.0 * A.xH0[off + i];
+        A.qH1[off + i] = 1.0 * qv + -1.0 * A.qH0[off + i];
+        A.xH0[off + i] = xv;
+        A.qH0[off + i] = qv;
+      }
     }
     for (int d = tid; d < C.n_mos; d += nth) {
       const int g = I.dev_off + d;
@@ -962,8 +1024,13 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     if (S.mode == 2) { return; }
     if (sh_int[5]) {
       for (int i = tid; i < n; i += nth) {
-        A.xH1[off + i] = 0.0;
-        A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[2];
+        if (A.method == 1) {
+          A.xH1[off + i] = A.xH0[off + i];
+          A.qH1[off + i] = A.qH0[off + i];
+        } else {
+          A.xH1[off + i] = 0.0;
+          A.qH1[off + i] = -(A.F[off + i] - A.B[off + i]) * sh_d[2];
+        }
       }
       __syncthreads();
     }
@@ -988,7 +1055,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     double newh;
     if (status <= 0) { newh = S.h / 8; S.order = 1; }
     else if (S.nef == 1) {
-      S.order = 1;
+      // OneStep restarts at order 1; Gear12 keeps its order (N_TIA_Gear12.C:1502-1516)
+      if (A.method != 1) S.order = 1;
       const double rr = fmax(0.25, fmin(0.9, pow(0.5 / (S.est + 0.0001), 1.0 / (S.order + 1.0))));
       newh = rr * S.h;
     } else { newh = 0.25 * S.h; S.order = 1; }
@@ -1173,6 +1241,7 @@ struct DArr {
 
 struct xg_batch {
   int n_inst = 0;
+  bool started = false;           // a launch has run (the integration method is fixed from then on)
   std::vector<ClassDesc> classes;
   std::vector<InstDesc> insts;
   std::vector<int> cls_n;         // per instance n
@@ -1637,6 +1706,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   }
   HIPCHK(hipDeviceSynchronize());
   BatchArgs& A = b->args;
+  A.method = 0;
   A.n_inst = n_inst; A.order = b->d_order.p; A.inst = b->d_inst.p; A.st = b->d_st.p; A.cls = b->d_cls.p; A.ci = b->d_ci.p;
   A.nt_mos = (int)NT; A.kcls = b->d_kcls.p; A.mcls = b->d_mcls.p; A.tvbi = b->d_tvbi.p; A.beta = b->d_beta.p;
   A.tvto = b->d_tvto.p; A.mask = b->d_mask.p; A.sto_cur = b->d_sto_cur.p; A.sto_nxt = b->d_sto_nxt.p;
@@ -1670,6 +1740,7 @@ int xg_batch_destroy(xg_batch* b) {
 
 int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
   if (!b || launches < 0) return XG_E_ARG;
+  if (launches > 0) b->started = true;
   HIPCHK(hipEventRecord(b->ev0, b->stream));
   int left = launches;
   if (graph_chunk > 0 && launches >= graph_chunk) {
@@ -1935,6 +2006,13 @@ int xg_batch_autotune(xg_batch* b, double* result) {
       result[5 + j] = j < (int)b->buckets.size() ? (b->buckets[j].block ? b->buckets[j].block : b->block) : 0;
     result[9] = b->concurrent_groups ? 1 : 0;
   }
+  return XG_OK;
+}
+
+int xg_batch_set_method(xg_batch* b, int method) {
+  if (!b || (method != 0 && method != 1) || b->started) return XG_E_ARG;
+  b->args.method = method;
+  b->drop_graph();
   return XG_OK;
 }
 

@@ -9,7 +9,9 @@ importer) and builds a ``Circuit``:
   M);
 * ``.MODEL name NMOS|PMOS (LEVEL=1 ...)`` with the level-1 parameters;
 * ``.TRAN tstep tstop [tstart [tmax]]``, ``.PRINT TRAN V(node) ...``,
-  ``.END``; ``.OPTIONS`` lines are accepted and ignored;
+  ``.END``; ``.OPTIONS TIMEINT METHOD=TRAP|GEAR`` (also 7|8, the reference's
+  numbering, src/TimeIntegrationPKG/N_TIA_fwd.h:52-58) selects the
+  integrator, other ``.OPTIONS`` lines are accepted and ignored;
 * ``*`` comment lines, ``;`` end-of-line comments, ``+`` continuation lines,
   case-insensitive names, SPICE scale suffixes (T G MEG K MIL M U N P F).
 
@@ -125,7 +127,8 @@ def _source(tokens, no):
 
 def parse_netlist(text: str):
     """Returns (Circuit, analysis) with analysis = {'tran': (tstep, tstop,
-    tstart, tmax) or None, 'print': [node names], 'title': str}."""
+    tstart, tmax) or None, 'print': [node names], 'title': str, 'method':
+    'trap' | 'gear', 'nodes': {name: index}}."""
     c = Circuit()
     lines = text.splitlines()
     title = lines[0].strip() if lines else ""
@@ -133,7 +136,7 @@ def parse_netlist(text: str):
     nodes = {"0": 0, "GND": 0}
     models = {}
     pending = []        # MOSFETs waiting for their .model (it may follow them)
-    analysis = dict(tran=None, print=[], title=title.lstrip("*").strip())
+    analysis = dict(tran=None, print=[], title=title.lstrip("*").strip(), method="trap")
 
     def node(name):
         key = name.upper()
@@ -173,9 +176,17 @@ def parse_netlist(text: str):
                         i += 4
                     else:
                         i += 1
-            elif head in (".END", ".OPTIONS", ".OPTION"):
-                if head == ".END":
-                    break
+            elif head in (".OPTIONS", ".OPTION"):
+                if len(tok) > 1 and tok[1].upper() == "TIMEINT":
+                    for i in range(2, len(tok) - 2):
+                        if tok[i].upper() == "METHOD" and tok[i + 1] == "=":
+                            m = tok[i + 2].upper()
+                            methods = {"TRAP": "trap", "TRAPEZOIDAL": "trap", "7": "trap", "GEAR": "gear", "8": "gear"}
+                            if m not in methods:
+                                raise NetlistError(f"line {no}: integration method {tok[i + 2]!r} is not supported")
+                            analysis["method"] = methods[m]
+            elif head == ".END":
+                break
             else:
                 raise NetlistError(f"line {no}: unsupported control line {tok[0]}")
             continue
@@ -252,7 +263,7 @@ def run_transient(text: str, prn_path=None, record_max: int = 200_000):
     if an["tran"] is None:
         raise NetlistError("no .TRAN line")
     tstep, tstop, tstart, tmax = an["tran"]
-    b = Batch([c], tstop, tstep, tmax, record_max=record_max)
+    b = Batch([c], tstop, tstep, tmax, record_max=record_max, method=an["method"])
     cnt = b.run_to_end(chunk=512)
     if cnt["failed"]:
         raise RuntimeError("transient failed (DC operating point or time-step control)")
