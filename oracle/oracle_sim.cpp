@@ -190,6 +190,15 @@ struct Work {
   int probe_k = 0;
   std::vector<double> probe_w;
   int method = 0;   // 0 = OneStep trapezoid, 1 = Gear12 (.OPTIONS TIMEINT METHOD=GEAR)
+  // DC continuation (Interface::spiceStrategy, N_NLS_NOX_Interface.C:321-437):
+  // conductance added to every node-voltage row (GStepping::augmentResidual /
+  // augmentJacobian, N_NLS_NOX_AugmentLinSys_GStepping.C), the scale of every
+  // independent source (VSRCSCALE), and whether the first Newton iteration
+  // applies initJct (only the first solve of a strategy does)
+  double gcond = 0.0, src_scale = 1.0;
+  bool dc_initjct = true;
+  int dc_strategy = 0;               // 0 plain Newton, 1 gmin stepping, 2 source stepping (the last DC solve)
+  std::vector<int> vrow, vdiag;      // node-voltage rows and their diagonal CSR slots
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
     x.assign(n + 1, 0.0); F.assign(n + 1, 0.0); Q.assign(n + 1, 0.0); B.assign(n + 1, 0.0);
@@ -264,6 +273,12 @@ static void load_rhs(Work& w, const SolverState& ss) {
   }
 }
 
+// B holds only the independent sources' values: source stepping scales it
+static void scale_sources(Work& w) {
+  if (w.src_scale == 1.0) return;
+  for (int i = 0; i < w.n; ++i) w.B[i] *= w.src_scale;
+}
+
 // NonlinearEquationLoader::loadJacobian (:514-558): zero, nonlinear device
 // matrices, then FilteredMatrix::addToMatrix of the linear part (:632).
 static void load_jacobian(Work& w, const SolverState& ss) {
@@ -306,6 +321,8 @@ static void combine_residual(Work& w, const SolverState& ss, const double* q0, c
       r = r + 1.0 * w.dFdxdVp[i];
       w.RHS[i] = r;
     }
+    if (w.gcond != 0.0)
+      for (int r : w.vrow) w.RHS[r] -= w.gcond * w.x[r];
     return;
   }
   const double h = ss.h;
@@ -334,6 +351,9 @@ static void combine_jacobian(Work& w, const SolverState& ss) {
   else if (ss.method == 1) { a = ss.a0 / ss.h; b = 1.0; }    // Gear12::obtainJacobian :477-505
   else { a = -(-1.0) / ss.h; b = (ss.order == 2) ? 1.0 / 2.0 : 1.0; }
   for (int k = 0; k < w.nnz; ++k) w.J[k] = a * w.dQdx[k] + b * w.dFdx[k];
+  if (ss.dcop && w.gcond != 0.0)
+    for (int k : w.vdiag)
+      if (k >= 0) w.J[k] += w.gcond;
 }
 
 // KLU-semantics direct solve (AmesosSolver::doSolve N_LAS_AmesosSolver.C:216-478:
@@ -401,8 +421,9 @@ static NewtonStats newton(Work& w, SolverState ss, const NLP& p, const std::vect
   std::vector<double> dx(n), wt(n);
   auto rhs = [&](int it) {
     ss.newtonIter = it;
-    ss.initJct = ss.dcop && it == 0;   // N_DEV_SolverState.C:387-409
+    ss.initJct = ss.dcop && it == 0 && w.dc_initjct;   // N_DEV_SolverState.C:387-409
     load_rhs(w, ss);
+    if (ss.dcop) scale_sources(w);
     combine_residual(w, ss, q0, q2);
     if (n_loads) ++*n_loads;
   };
@@ -489,15 +510,89 @@ static double wrms(const double* v, const double* wt, int n) {
 // DC operating point.  Xyce uses NOX for DCOP (N_NLS_Manager.C:368-376), which
 // is not on the path; the oracle runs DampedNewton with the DC defaults and an
 // initJct first iteration (N_DEV_SolverState.C:387-409).
+// One continuation run (LOCA natural continuation with a constant
+// predictor and adaptive steps, as set up by Interface::gminSteppingSolve
+// :1116-1185 and sourceSteppingSolve :1284-1340): the parameter moves from
+// p0 to p1; each step is a Newton solve (at most 20 iterations) from the last
+// converged point; a failed step is retried at half the step, a converged one
+// lets the step grow by (1 + a (1 - (iters/20)^2)) up to max_step.  LOCA is
+// third-party (Trilinos); this restates its published natural-continuation
+// scheme with the reference's settings.
+template <class SetParam>
+static int dc_continuation(Work& w, long* iters, double p0, double p1, double step, double min_step, double max_step,
+                           SetParam set_param) {
+  const int n = w.n;
+  SolverState base; base.dcop = 1; base.tran = 0; base.time = 0.0;
+  NLP p{1e-3, 1e-12, 1.0, 1e-6, 1e-6, 20, false};
+  std::vector<double> xc(n, 0.0);
+  std::fill(w.x.begin(), w.x.end(), 0.0);
+  std::fill(w.staCur.begin(), w.staCur.end(), 0.0); std::fill(w.stoCur.begin(), w.stoCur.end(), 0.0);
+  std::vector<double> x_ok(w.x), sta_ok(w.staCur), sto_ok(w.stoCur);
+  double par = p0, par_ok = p0;
+  bool first = true, done = false;
+  for (int k = 0; k < 400 && !done; ++k) {
+    set_param(par);
+    w.dc_initjct = first;
+    NewtonStats ns = newton(w, base, p, xc, nullptr, nullptr, nullptr);
+    if (iters) *iters += ns.iters;
+    if (ns.status > 0) {
+      first = false;
+      par_ok = par;
+      x_ok = w.x; sta_ok = w.staNext; sto_ok = w.stoNext;
+      w.staCur = w.staNext; w.stoCur = w.stoNext;
+      if (par == p1) { done = true; break; }
+      const double r = ns.iters / 20.0;
+      step *= 1.0 + 0.01 * (1.0 - r * r);
+      if (std::fabs(step) > max_step) step = step > 0 ? max_step : -max_step;
+    } else {
+      std::copy(x_ok.begin(), x_ok.end(), w.x.begin());
+      w.staNext = sta_ok; w.stoNext = sto_ok;
+      step *= 0.5;
+      if (std::fabs(step) < min_step) break;
+    }
+    par = par_ok + step;
+    if ((p1 - par) * (p1 - p0) <= 0.0) par = p1;   // do not step past the end value
+  }
+  w.dc_initjct = true;
+  set_param(p1);
+  return done ? 1 : -1;
+}
+
+// DC operating point: Interface::spiceStrategy (N_NLS_NOX_Interface.C:321-437):
+// Newton from initJct; if that fails, gmin stepping (10^p - 10^-4 siemens on
+// every node-voltage row, p from 4 to -4); if that fails, source stepping
+// (all independent sources scaled from 0 to 1).
 static int dcop(Work& w, long* iters) {
   const int n = w.n;
   SolverState base; base.dcop = 1; base.tran = 0; base.time = 0.0;
   NLP p{1e-3, 1e-12, 1.0, 1e-6, 1e-6, 200, false};
   std::vector<double> xc(n, 0.0);
   std::fill(w.x.begin(), w.x.end(), 0.0);
+  w.gcond = 0.0; w.src_scale = 1.0; w.dc_initjct = true;
+  w.dc_strategy = 0;
   NewtonStats ns = newton(w, base, p, xc, nullptr, nullptr, nullptr);
   if (iters) *iters += ns.iters;
-  return ns.status;
+  if (ns.status > 0) return ns.status;
+  if (w.vrow.empty()) {
+    const Circuit& c = *w.c;
+    for (size_t k = 1; k < c.node_gid.size(); ++k) {
+      const int r = c.node_gid[k];
+      if (r < 0) continue;
+      int slot = -1;
+      for (int q = c.rowptr[r]; q < c.rowptr[r + 1]; ++q) if (c.colind[q] == r) slot = q;
+      w.vrow.push_back(r);
+      w.vdiag.push_back(slot);
+    }
+  }
+  w.dc_strategy = 1;
+  int st = dc_continuation(w, iters, 4.0, -4.0, -2.0, 1e-6, 1e12,
+                           [&](double q) { w.gcond = std::pow(10.0, q) - std::pow(10.0, -4.0); });
+  w.gcond = 0.0;
+  if (st > 0) return st;
+  w.dc_strategy = 2;
+  st = dc_continuation(w, iters, 0.0, 1.0, 0.2, 1e-4, 0.2, [&](double q) { w.src_scale = q; });
+  w.src_scale = 1.0;
+  return st;
 }
 
 // Transient loop: OneStep::initialize (N_TIA_OneStep.C:1684), updateCoeffs
@@ -965,6 +1060,10 @@ void orc_tran_orders(OrcHandle* h, int* ord) {
 
 // Record k products w_r . x per accepted point instead of the solution
 // (k = 0 restores whole solutions); w is k x n, row-major.
+// Which DC strategy produced the last operating point: 0 Newton, 1 gmin
+// stepping, 2 source stepping.
+int orc_dc_strategy(OrcHandle* h) { return h->w->dc_strategy; }
+
 // Integration method of later orc_tran calls: 0 = OneStep trapezoid (the
 // default), 1 = Gear12.
 int orc_set_method(OrcHandle* h, int method) {

@@ -41,6 +41,8 @@ def lib():
         L.orc_tran_orders.argtypes = [P, P]
         L.orc_set_probe.argtypes = [P, C.c_int, P]
         L.orc_set_probe.restype = C.c_int
+        L.orc_dc_strategy.argtypes = [P]
+        L.orc_dc_strategy.restype = C.c_int
         L.orc_set_method.argtypes = [P, C.c_int]
         L.orc_set_method.restype = C.c_int
         for nm in ("orc_limvds", "orc_fetlim", "orc_qmeyer", "orc_pnjlim"):
@@ -150,6 +152,11 @@ class Oracle:
     def set_tolerances(self, tia_reltol=1e-3, tia_abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
         lib().orc_set_tolerances(self.h, C.c_double(tia_reltol), C.c_double(tia_abstol), C.c_double(nl_reltol),
                                  C.c_double(nl_abstol))
+
+    def dc_strategy(self):
+        """Strategy of the last DC operating point: 'newton', 'gmin' or
+        'source' (stepping)."""
+        return ("newton", "gmin", "source")[lib().orc_dc_strategy(self.h)]
 
     def set_method(self, method):
         """Integration method of later transients: 'trap' (OneStep, the

@@ -301,3 +301,42 @@ def test_pwl_waveform_closed_form():
     o = Oracle(c)
     for t, v in [(0.0, 0.0), (0.5e-9, 0.5), (1e-9, 1.0), (2e-9, 0.0), (3e-9, -1.0), (5e-9, -1.0)]:
         assert o.wave(w, t) == pytest.approx(v, rel=1e-12, abs=1e-15), t
+
+
+def _gate_logic(c):
+    """Expected DC level (0 or VDD) of every gate output of a gate_dag
+    circuit, from its MOSFET list: the PMOS devices whose drain is a gate's
+    output carry its inputs (one: inverter, two: NAND2); primary inputs start
+    at their PULSE's initial value (0 V)."""
+    import re
+    level = {k: 0 for k, nm in c.names.items() if nm.startswith("pi")}
+    vdd_node = [k for k, nm in c.names.items() if nm == "vdd"][0]
+    level[vdd_node] = 1
+    ins = {}
+    for i, (d, g, s, b) in enumerate(c.mos_nodes):
+        if c.models[c.mos_model[i]][0] < 0:           # PMOS: drain is the gate output
+            ins.setdefault(d, []).append(g)
+    out = {}
+    for k in sorted((k for k, nm in c.names.items() if re.fullmatch(r"g\d+", nm)), key=lambda k: int(c.names[k][1:])):
+        vals = [level[g] for g in ins[k]]
+        level[k] = 0 if all(vals) else 1               # INV / NAND2
+        out[k] = level[k]
+    return out
+
+
+@pytest.mark.parametrize("n,seed", [(100, 1), (400, 1), (1000, 4)])
+def test_dc_operating_point_strategy(n, seed):
+    """DC operating point with the reference's strategy (Newton from initJct,
+    then gmin stepping, then source stepping): every gate output sits at the
+    logic level its inputs give (within 1 mV of 0 or VDD) and the DC residual
+    is at the Newton tolerance.  Plain Newton alone fails on the larger gate
+    arrays (they need the continuation)."""
+    c = C.gate_dag(n, seed=seed)
+    o = Oracle(c)
+    st, x, sta, sto, Q, F, B = o.dcop()
+    assert st > 0
+    assert o.dc_strategy() == ("newton" if n <= 100 else "gmin")
+    assert np.max(np.abs(F - B)) <= 1e-8
+    gid = o.pattern()["node_gid"]
+    for k, lv in _gate_logic(c).items():
+        assert abs(x[gid[k]] - lv * 1.8) <= 1e-3, (c.names[k], x[gid[k]], lv)
