@@ -201,6 +201,16 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
  * src/TimeIntegrationPKG/N_TIA_Gear12.C replaces N_TIA_OneStep.C).  Only before
  * the first launch (xg_batch_advance / xg_batch_autotune): XG_E_ARG after. */
 int xg_batch_set_method(xg_batch* b, int method);
+/* DC operating point strategy (Interface::spiceStrategy,
+ * src/NonlinearSolverPKG/N_NLS_NOX_Interface.C:321-437): every instance runs
+ * Newton from initJct, then gmin stepping (10^p - 10^-4 S on every
+ * node-voltage row, p from 4 to -4), then source stepping (independent
+ * sources scaled from 0 to 1).  first_stage 1 or 2 starts at gmin or source
+ * stepping, as .OPTIONS NONLIN CONTINUATION=GMIN|SOURCE does; only before the
+ * first launch.  xg_batch_dc_stage reports the stage that produced an
+ * instance's operating point (0 Newton, 1 gmin, 2 source stepping). */
+int xg_batch_dc_strategy(xg_batch* b, int first_stage);
+int xg_batch_dc_stage(xg_batch* b, int inst, int* stage);
 /* Per-phase device timers of the first-scheduled instance (the counterpart of
  * the reference's Residual / Jacobian / Linear-solve StatTop timers,
  * src/UtilityPKG/N_UTL_Stats.h:481-511): enable = 1 resets and starts,

@@ -198,6 +198,7 @@ struct Work {
   double gcond = 0.0, src_scale = 1.0;
   bool dc_initjct = true;
   int dc_strategy = 0;               // 0 plain Newton, 1 gmin stepping, 2 source stepping (the last DC solve)
+  int dc_first_stage = 0;            // tests: start the strategy at this stage (0 = the reference's order)
   std::vector<int> vrow, vdiag;      // node-voltage rows and their diagonal CSR slots
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
@@ -512,15 +513,17 @@ static double wrms(const double* v, const double* wt, int n) {
 // initJct first iteration (N_DEV_SolverState.C:387-409).
 // One continuation run (LOCA natural continuation with a constant
 // predictor and adaptive steps, as set up by Interface::gminSteppingSolve
-// :1116-1185 and sourceSteppingSolve :1284-1340): the parameter moves from
+// :1116-1185 and sourceSteppingSolve :1284-1345): the parameter moves from
 // p0 to p1; each step is a Newton solve (at most 20 iterations) from the last
 // converged point; a failed step is retried at half the step, a converged one
-// lets the step grow by (1 + a (1 - (iters/20)^2)) up to max_step.  LOCA is
+// lets the step grow by (1 + a (1 - (iters/20)^2)) up to max_step, with the
+// aggressiveness a = 0.01 for gmin and 1 for source stepping; source stepping
+// disables initJct (setDisableInitJctFlags).  LOCA is
 // third-party (Trilinos); this restates its published natural-continuation
 // scheme with the reference's settings.
 template <class SetParam>
 static int dc_continuation(Work& w, long* iters, double p0, double p1, double step, double min_step, double max_step,
-                           SetParam set_param) {
+                           double aggressiveness, bool init_jct, SetParam set_param) {
   const int n = w.n;
   SolverState base; base.dcop = 1; base.tran = 0; base.time = 0.0;
   NLP p{1e-3, 1e-12, 1.0, 1e-6, 1e-6, 20, false};
@@ -532,7 +535,7 @@ static int dc_continuation(Work& w, long* iters, double p0, double p1, double st
   bool first = true, done = false;
   for (int k = 0; k < 400 && !done; ++k) {
     set_param(par);
-    w.dc_initjct = first;
+    w.dc_initjct = first && init_jct;
     NewtonStats ns = newton(w, base, p, xc, nullptr, nullptr, nullptr);
     if (iters) *iters += ns.iters;
     if (ns.status > 0) {
@@ -542,7 +545,7 @@ static int dc_continuation(Work& w, long* iters, double p0, double p1, double st
       w.staCur = w.staNext; w.stoCur = w.stoNext;
       if (par == p1) { done = true; break; }
       const double r = ns.iters / 20.0;
-      step *= 1.0 + 0.01 * (1.0 - r * r);
+      step *= 1.0 + aggressiveness * (1.0 - r * r);
       if (std::fabs(step) > max_step) step = step > 0 ? max_step : -max_step;
     } else {
       std::copy(x_ok.begin(), x_ok.end(), w.x.begin());
@@ -570,9 +573,11 @@ static int dcop(Work& w, long* iters) {
   std::fill(w.x.begin(), w.x.end(), 0.0);
   w.gcond = 0.0; w.src_scale = 1.0; w.dc_initjct = true;
   w.dc_strategy = 0;
-  NewtonStats ns = newton(w, base, p, xc, nullptr, nullptr, nullptr);
-  if (iters) *iters += ns.iters;
-  if (ns.status > 0) return ns.status;
+  if (w.dc_first_stage == 0) {
+    NewtonStats ns = newton(w, base, p, xc, nullptr, nullptr, nullptr);
+    if (iters) *iters += ns.iters;
+    if (ns.status > 0) return ns.status;
+  }
   if (w.vrow.empty()) {
     const Circuit& c = *w.c;
     for (size_t k = 1; k < c.node_gid.size(); ++k) {
@@ -584,13 +589,16 @@ static int dcop(Work& w, long* iters) {
       w.vdiag.push_back(slot);
     }
   }
-  w.dc_strategy = 1;
-  int st = dc_continuation(w, iters, 4.0, -4.0, -2.0, 1e-6, 1e12,
-                           [&](double q) { w.gcond = std::pow(10.0, q) - std::pow(10.0, -4.0); });
-  w.gcond = 0.0;
-  if (st > 0) return st;
+  int st = -1;
+  if (w.dc_first_stage <= 1) {
+    w.dc_strategy = 1;
+    st = dc_continuation(w, iters, 4.0, -4.0, -2.0, 1e-6, 1e12, 0.01, true,
+                         [&](double q) { w.gcond = std::pow(10.0, q) - std::pow(10.0, -4.0); });
+    w.gcond = 0.0;
+    if (st > 0) return st;
+  }
   w.dc_strategy = 2;
-  st = dc_continuation(w, iters, 0.0, 1.0, 0.2, 1e-4, 0.2, [&](double q) { w.src_scale = q; });
+  st = dc_continuation(w, iters, 0.0, 1.0, 0.2, 1e-4, 0.2, 1.0, false, [&](double q) { w.src_scale = q; });
   w.src_scale = 1.0;
   return st;
 }
@@ -1063,6 +1071,14 @@ void orc_tran_orders(OrcHandle* h, int* ord) {
 // Which DC strategy produced the last operating point: 0 Newton, 1 gmin
 // stepping, 2 source stepping.
 int orc_dc_strategy(OrcHandle* h) { return h->w->dc_strategy; }
+
+// Start later DC solves at this stage of the strategy (0 Newton, 1 gmin
+// stepping, 2 source stepping): lets the tests reach each stage.
+int orc_set_dc_first_stage(OrcHandle* h, int stage) {
+  if (stage < 0 || stage > 2) return -1;
+  h->w->dc_first_stage = stage;
+  return 0;
+}
 
 // Integration method of later orc_tran calls: 0 = OneStep trapezoid (the
 // default), 1 = Gear12.

@@ -41,6 +41,8 @@ def lib():
         L.orc_tran_orders.argtypes = [P, P]
         L.orc_set_probe.argtypes = [P, C.c_int, P]
         L.orc_set_probe.restype = C.c_int
+        L.orc_set_dc_first_stage.argtypes = [P, C.c_int]
+        L.orc_set_dc_first_stage.restype = C.c_int
         L.orc_dc_strategy.argtypes = [P]
         L.orc_dc_strategy.restype = C.c_int
         L.orc_set_method.argtypes = [P, C.c_int]
@@ -157,6 +159,11 @@ class Oracle:
         """Strategy of the last DC operating point: 'newton', 'gmin' or
         'source' (stepping)."""
         return ("newton", "gmin", "source")[lib().orc_dc_strategy(self.h)]
+
+    def set_dc_first_stage(self, stage):
+        """Start the DC strategy at 'newton' (the reference's order), 'gmin'
+        or 'source' stepping."""
+        assert lib().orc_set_dc_first_stage(self.h, {"newton": 0, "gmin": 1, "source": 2}[stage]) == 0
 
     def set_method(self, method):
         """Integration method of later transients: 'trap' (OneStep, the

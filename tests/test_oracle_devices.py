@@ -340,3 +340,23 @@ def test_dc_operating_point_strategy(n, seed):
     gid = o.pattern()["node_gid"]
     for k, lv in _gate_logic(c).items():
         assert abs(x[gid[k]] - lv * 1.8) <= 1e-3, (c.names[k], x[gid[k]], lv)
+
+
+@pytest.mark.parametrize("stage,make", [("gmin", lambda: C.gate_dag(150, seed=2)),
+                                        ("gmin", lambda: C.ring(11, seed=3)),
+                                        ("source", lambda: C.ring(11, seed=3)),
+                                        ("source", lambda: C.inverter_chain(3, TOX=2e-8, CJ=2e-4))])
+def test_dc_continuation_stages_reach_the_newton_point(stage, make):
+    """Each continuation stage started on its own reaches the operating point
+    plain Newton finds where it converges (the DC point is unique here), to
+    the DC Newton tolerance (1e-3 |V| + 1 uV)."""
+    c = make()
+    ref = Oracle(c)
+    st0, x0, *_ = ref.dcop()
+    assert st0 > 0 and ref.dc_strategy() == "newton"
+    o = Oracle(c)
+    o.set_dc_first_stage(stage)
+    st, x, sta, sto, Q, F, B = o.dcop()
+    assert st > 0 and o.dc_strategy() == stage
+    assert np.max(np.abs(F - B)) <= 1e-8
+    assert np.all(np.abs(x - x0) <= 1e-3 * np.abs(x0) + 1e-6)

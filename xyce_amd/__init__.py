@@ -41,7 +41,7 @@ EXPORTS = [
     "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
-    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
+    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
 ]
@@ -334,6 +334,19 @@ class Batch:
         _check(lib().xg_batch_instance(self.h, C.c_int(i), _p(ts), _p(x)), "xg_batch_instance")
         keys = ("t", "h", "order", "mode", "newton_iters", "steps", "rejects", "loads")
         return dict(zip(keys, ts.tolist())), x
+
+    def dc_strategy(self, first_stage):
+        """Start every instance's DC operating point at 'newton' (the
+        reference's order: Newton, gmin stepping, source stepping), 'gmin' or
+        'source' stepping; before the first launch."""
+        st = {"newton": 0, "gmin": 1, "source": 2}[first_stage]
+        _check(lib().xg_batch_dc_strategy(self.h, C.c_int(st)), "xg_batch_dc_strategy")
+
+    def dc_stage(self, i):
+        """The DC strategy stage that produced instance i's operating point."""
+        v = C.c_int(0)
+        _check(lib().xg_batch_dc_stage(self.h, C.c_int(i), C.byref(v)), "xg_batch_dc_stage")
+        return ("newton", "gmin", "source")[v.value]
 
     def points(self, i, n):
         t = np.zeros(self.record_max)

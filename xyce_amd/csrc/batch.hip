@@ -58,6 +58,7 @@ struct ClassDesc {
   // long LU targets / solve rows (>= kLongOps terms, one wavefront each) come
   // last in their level: first long index per level
   int b_fls, b_fss, b_bss;
+  int b_vrow;        // n flags: 1 = node-voltage row (gmin stepping adds its conductance there)
   // term ranges, linear entries (CSR slots with a nonzero linear value, per
   // row) and per-row source lists, all uint16
   int b_vptr, b_mptr, b_lrp, b_lk, b_bgp, b_bge;
@@ -90,7 +91,12 @@ struct InstState {
   // Gear12: predictor weight of x_n and the corrector weights of Q_{n+1},
   // Q_n, Q_{n-1} (N_TIA_Gear12.C:1050-1111)
   double beta0, a0, a1, a2;
-  double pad[18];
+  // DC operating point strategy (Interface::spiceStrategy): stage 0 Newton,
+  // 1 gmin stepping, 2 source stepping; the continuation parameter, its last
+  // converged value and step, the conductance and source scale in force,
+  // whether the next solve starts from initJct, and the steps taken
+  double dc_stage, cont_par, cont_ok, cont_step, gcond, src_scale, cont_first, cont_k;
+  double pad[10];
 };
 
 static_assert(sizeof(InstState) == 512, "InstState is one 512-byte record");
@@ -159,6 +165,7 @@ constexpr double kNlRel = 1e-2, kNlAbs = 1e-6, kDxTol = 0.33, kRhsTol = 1e-2;  /
 constexpr int kMaxNewton = 20;
 constexpr double kDcRel = 1e-3, kDcAbs = 1e-12, kDcDxTol = 1.0, kDcRhsTol = 1e-6;
 constexpr int kDcMaxNewton = 200;
+constexpr int kContMaxNewton = 20;       // per continuation step (LOCA "Max Nonlinear Iterations")
 constexpr double kHmin = 1e-18;
 constexpr int kLongSeg = 32;     // term groups this long are reduced by a wavefront
 constexpr int kMaxLong = 16;     // per class
@@ -364,6 +371,28 @@ __device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDe
   }
 }
 
+// Gear12::updateCoeffs (N_TIA_Gear12.C:1050-1111): variable-step BDF2
+// predictor and corrector weights, or BDF1 (psi already shifted; out of line
+// to keep the iteration kernel's registers)
+__device__ __noinline__ void gear_coeffs(InstState& S) {
+  const double t1 = S.h;
+  if (S.order == 2) {
+    S.beta2 = t1 / S.psi2 * (t1 + S.psi1) / (S.psi1 + S.psi2);
+    S.beta1 = -t1 / S.psi1 - S.beta2 * (S.psi1 + S.psi2) / S.psi1;
+    S.beta0 = 1.0 - S.beta2 - S.beta1;
+    double a2 = -t1 / S.psi1 * t1 / (2 * t1 + S.psi1);
+    double a1 = 1 - a2;
+    double a0 = -a1 - a2 * (1 + S.psi1 / t1);
+    S.a2 = a2 / a0; S.a1 = a1 / a0; S.a0 = -1 / a0;
+    S.ck = S.h / (t1 + S.psi1 + S.psi2);
+  } else {
+    S.beta0 = 1.0 + t1 / S.psi1;
+    S.beta1 = -t1 / S.psi1;
+    S.beta2 = 0.0;
+    S.ck = S.h / (t1 + S.psi1);
+  }
+}
+
 // Top of a time step: clamp to the next breakpoint, OneStep::updateCoeffs
 // (N_TIA_OneStep.C:1584), predictor (:135), error weights (NEWLTE=1).
 __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C, const InstDesc& I, double* red) {
@@ -376,22 +405,7 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
     S.psi1 = S.psi0; S.psi0 = t1;
     S.beta0 = 1.0; S.a0 = 1.0; S.a1 = -1.0; S.a2 = 0.0;
     if (A.method == 1) {
-      // Gear12::updateCoeffs: variable-step BDF2 predictor/corrector, or BDF1
-      if (S.order == 2) {
-        S.beta2 = t1 / S.psi2 * (t1 + S.psi1) / (S.psi1 + S.psi2);
-        S.beta1 = -t1 / S.psi1 - S.beta2 * (S.psi1 + S.psi2) / S.psi1;
-        S.beta0 = 1.0 - S.beta2 - S.beta1;
-        double a2 = -t1 / S.psi1 * t1 / (2 * t1 + S.psi1);
-        double a1 = 1 - a2;
-        double a0 = -a1 - a2 * (1 + S.psi1 / t1);
-        S.a2 = a2 / a0; S.a1 = a1 / a0; S.a0 = -1 / a0;
-        S.ck = S.h / (t1 + S.psi1 + S.psi2);
-      } else {
-        S.beta0 = 1.0 + t1 / S.psi1;
-        S.beta1 = -t1 / S.psi1;
-        S.beta2 = 0.0;
-        S.ck = S.h / (t1 + S.psi1);
-      }
+      gear_coeffs(S);
     } else if (S.order == 2) {
       const double t2 = S.psi1;
       S.beta1 = t1 / t2 + (t1 / t2) * (t1 / t2) / 2;
@@ -451,7 +465,8 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
 // (N_LAS_AmesosSolver.C:318) would pick different pivots.  Returns nonzero if
 // a zero pivot is met (singular).
 __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const InstDesc& I, const int* ci,
-                                 const unsigned short* prog, double ja, double jb, double* rhs, double* red, int* sh_i) {
+                                 const unsigned short* prog, double ja, double jb, double gdiag, double* rhs,
+                                 double* red, int* sh_i) {
   const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
   double* D = A.dense + I.dense_off;
   const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
@@ -470,6 +485,14 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
       D[(size_t)r * n + cc[s]] = jac_combine(ja, dq, jb, df);
     }
   __syncthreads();
+  if (gdiag != 0.0) {
+    // gmin stepping: the continuation conductance on every node-voltage row
+    // (GStepping::augmentJacobian)
+    const unsigned short* vr = prog + C.b_vrow;
+    for (int r = tid; r < n; r += nth)
+      if (vr[r]) D[(size_t)r * n + r] += gdiag;
+    __syncthreads();
+  }
   int singular = 0;
   for (int k = 0; k < n; ++k) {
     // pivot: largest |D(i,k)|, i >= k (lowest index on ties)
@@ -523,6 +546,59 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
 
 // One Newton iteration of one instance (the body of every launch; a launch
 // runs A.iters of them back to back with the instance state kept in LDS).
+// One decision of the DC strategy after a Newton solve ended with `status`
+// (scalar code, kept out of line so that the iteration kernel's registers are
+// not spent on it).  Returns 0 done (to transient), 1 next continuation step
+// from the converged point, 2 failed, 3 a strategy starts over from zero,
+// 4 retry from the last converged point.
+__device__ __noinline__ int dc_strategy_step(InstState& S, int status) {
+  int act = 0;       // 0 done (to transient), 1 new solve, 2 failed
+  const int stage = (int)S.dc_stage;
+  if (status > 0) {
+    if (stage == 0) act = 0;
+    else {
+      S.cont_first = 0.0;
+      S.cont_ok = S.cont_par;
+      const double p1 = stage == 1 ? -4.0 : 1.0;
+      if (S.cont_par == p1) act = 0;
+      else {
+        const double r = S.newton_iter / 20.0;
+        S.cont_step *= 1.0 + (stage == 2 ? 1.0 : 0.01) * (1.0 - r * r);   // LOCA aggressiveness
+        if (stage == 2 && fabs(S.cont_step) > 0.2) S.cont_step = 0.2;
+        act = 1;
+      }
+    }
+  } else if (stage == 0) {
+    S.dc_stage = 1; S.cont_par = 4.0; S.cont_ok = 4.0; S.cont_step = -2.0; S.cont_first = 1.0; S.cont_k = 0.0;
+    act = 3;         // restart from zero
+  } else {
+    S.cont_step *= 0.5;
+    const double min_step = stage == 1 ? 1e-6 : 1e-4;
+    if (fabs(S.cont_step) < min_step || S.cont_k >= 400.0) {
+      if (stage == 1) {
+        // source stepping disables initJct (setDisableInitJctFlags)
+        S.dc_stage = 2; S.cont_par = 0.0; S.cont_ok = 0.0; S.cont_step = 0.2; S.cont_first = 0.0; S.cont_k = 0.0;
+        act = 3;
+      } else act = 2;
+    } else act = 4;  // retry from the last converged point
+  }
+  if (act == 1 || act == 4) {
+    const double p0 = stage == 1 ? 4.0 : 0.0, p1 = stage == 1 ? -4.0 : 1.0;
+    S.cont_par = S.cont_ok + S.cont_step;
+    if ((p1 - S.cont_par) * (p1 - p0) <= 0.0) S.cont_par = p1;
+    S.cont_k += 1.0;
+  }
+  if (act == 1 || act == 3 || act == 4) {
+    const int st2 = (int)S.dc_stage;
+    S.gcond = st2 == 1 ? pow(10.0, S.cont_par) - pow(10.0, -4.0) : 0.0;
+    S.src_scale = st2 == 2 ? S.cont_par : 1.0;
+    S.newton_iter = 0;
+  }
+  if (act == 0) { S.gcond = 0.0; S.src_scale = 1.0; }
+  if (act == 2) S.mode = -1;
+  return act;
+}
+
 template <bool FUSED>
 __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I, const ClassDesc& C, InstState& S,
                                 double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
@@ -559,10 +635,12 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     // source values at this iteration's time, staged in LDS (read after the
     // barrier that ends the device evaluation)
     const double ts = dc ? 0.0 : S.t + S.h;
+    const double sscale = dc ? S.src_scale : 1.0;     // source stepping (VSRCSCALE)
     for (int k = tid; k < C.n_src; k += nth) {
       const int w = A.src_wave[I.src_off + k];
       const int wo = A.wave_off[w];
       srcv[k] = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, ts, A.tstep, A.tstop);
+      if (sscale != 1.0) srcv[k] *= sscale;
     }
   }
   __syncthreads();
@@ -571,7 +649,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   //         slot-ordered position in the instance's term array
   if (FUSED) {
     MosFlags fl;
-    fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0; fl.newton_iter = S.newton_iter;
+    fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0 && S.cont_first != 0.0;
+    fl.newton_iter = S.newton_iter;
     const int* lid = ci + C.o_lid;
     const int* tpos = ci + C.o_tpos;
     int lim_any = 0;
@@ -614,6 +693,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     if (nl) __syncthreads();
   }
   const unsigned short* cc = prog + C.b_cc;
+  const unsigned short* vrow = prog + C.b_vrow;
   // ---- 3. vectors per row (contiguous term ranges) and the residual
   const double h = S.h;
   double n2 = 0.0, ninf = 0.0;
@@ -655,7 +735,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const double res = (!dc && A.method == 1)
                              ? residual_gear(S.order, h, S.a0, S.a1, S.a2, f, q, b, fl, ql, A.qH0[off + r], A.qH1[off + r])
                              : residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
-      rhs[r] = res;
+      rhs[r] = (dc && S.gcond != 0.0 && vrow[r]) ? res - S.gcond * xs[r] : res;
       n2 += res * res;
       ninf = fmax(ninf, fabs(res));
     }
@@ -673,7 +753,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       S.norm_init = normRHS; S.norm_old = normRHS; S.stall_count = 0; S.tmp_rate = 1.0;
       status = isfinite(normRHS) ? 0 : -6;
     } else {
-      const int maxstep = dc ? kDcMaxNewton : kMaxNewton;
+      const int maxstep = dc ? (S.dc_stage != 0.0 ? kContMaxNewton : kDcMaxNewton) : kMaxNewton;
       const double rhstol = dc ? kDcRhsTol : kRhsTol, dxtol = dc ? kDcDxTol : kDxTol;
       const int nl = S.newton_iter;
       if (S.last_status < 0) status = S.last_status;          // linear failure / NaN from the solve
@@ -845,7 +925,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }
     if (use_dense) {
       if (tid == 0) S.n_dense++;
-      const int sing = dense_pivot_solve(A, C, I, ci, prog, ja, jb, rhs, red, red_i);
+      const int sing = dense_pivot_solve(A, C, I, ci, prog, ja, jb, dc ? S.gcond : 0.0, rhs, red, red_i);
       if (sing && tid == 0) sh_int[2] |= 1;
       __syncthreads();
     }
@@ -876,8 +956,47 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
 
   // ---- 5b. Newton finished for this step
   if (dc) {
-    if (status < 0) {
-      if (tid == 0) S.mode = -1;
+    // DC strategy (Interface::spiceStrategy, N_NLS_NOX_Interface.C:321-437):
+    // Newton from initJct; on failure gmin stepping (10^p - 10^-4 S on every
+    // node-voltage row, p from 4 to -4, step -2); on its failure source
+    // stepping (sources scaled from 0 to 1, step 0.2, no initJct); natural
+    // continuation with a constant predictor: a failed step is retried at half
+    // the step from the last converged point, a converged one grows the step
+    // by (1 + a (1 - (iters/20)^2)), a = 0.01 (gmin) or 1 (source).
+    if (tid == 0) {
+      const int act = dc_strategy_step(S, status);
+      sh_int[7] = act;
+    }
+    __syncthreads();
+    const int act = sh_int[7];
+    if (act == 2) return;
+    if (act == 1) {
+      // converged continuation step: it becomes the restart point (xH1 is free during DC)
+      for (int i = tid; i < n; i += nth) A.xH1[off + i] = xs[i];
+      for (int d = tid; d < C.n_mos; d += nth) {
+        const int g = I.dev_off + d;
+        for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
+        for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
+      }
+      return;
+    }
+    if (act == 3) {
+      // a strategy starts over from zero with fresh device history
+      for (int i = tid; i < n; i += nth) { A.x[off + i] = 0.0; A.xH1[off + i] = 0.0; }
+      for (int d = tid; d < C.n_mos; d += nth) {
+        const int g = I.dev_off + d;
+        for (int j = 0; j < 6; ++j) { A.sto_cur[(size_t)j * NT + g] = 0.0; A.sto_nxt[(size_t)j * NT + g] = 0.0; }
+        for (int j = 0; j < 8; ++j) { A.sta_cur[(size_t)j * NT + g] = 0.0; A.sta_nxt[(size_t)j * NT + g] = 0.0; }
+      }
+      return;
+    }
+    if (act == 4) {
+      for (int i = tid; i < n; i += nth) A.x[off + i] = A.xH1[off + i];
+      for (int d = tid; d < C.n_mos; d += nth) {
+        const int g = I.dev_off + d;
+        for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = A.sto_cur[(size_t)j * NT + g];
+        for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = A.sta_cur[(size_t)j * NT + g];
+      }
       return;
     }
     // tranop -> transient: OneStep::initialize (N_TIA_OneStep.C:1684)
@@ -1131,6 +1250,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const int mode = A.st[inst].mode;
   if (mode != 0 && mode != 1) return;
   const int newton_iter = A.st[inst].newton_iter;
+  const bool first_solve = A.st[inst].cont_first != 0.0;
   const InstDesc& I = A.inst[inst];
   const ClassDesc& C = A.cls[I.cls];
   const int d = g - I.dev_off;
@@ -1138,7 +1258,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const int NT = A.nt_mos;
   const bool dc = mode == 0;
   MosFlags fl;
-  fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0; fl.newton_iter = newton_iter;
+  fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0 && first_solve; fl.newton_iter = newton_iter;
   const int* lid = A.ci + C.o_lid;
   const int* tpos = A.ci + C.o_tpos;
   double* T = A.terms + I.term_off;
@@ -1211,7 +1331,27 @@ __global__ void k_init_state(int n_inst, InstState* st) {
   InstState s;
   memset(&s, 0, sizeof(s));
   s.mode = 0; s.order = 1; s.h = 1e-12;
+  s.src_scale = 1.0; s.cont_first = 1.0;
   st[i] = s;
+}
+
+// DC strategy starting at a later stage for every instance (the
+// counterpart of .OPTIONS NONLIN CONTINUATION=GMIN / SOURCE)
+__global__ void k_dc_first_stage(int n_inst, InstState* st, int stage) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_inst) return;
+  InstState& S = st[i];
+  S.dc_stage = stage;
+  S.cont_k = 0.0;
+  if (stage == 1) {
+    S.cont_par = 4.0; S.cont_ok = 4.0; S.cont_step = -2.0; S.cont_first = 1.0;
+    S.gcond = pow(10.0, 4.0) - pow(10.0, -4.0); S.src_scale = 1.0;
+  } else if (stage == 2) {
+    S.cont_par = 0.0; S.cont_ok = 0.0; S.cont_step = 0.2; S.cont_first = 0.0;
+    S.gcond = 0.0; S.src_scale = 0.0;
+  } else {
+    S.cont_first = 1.0; S.gcond = 0.0; S.src_scale = 1.0;
+  }
 }
 
 }  // namespace xg
@@ -1591,6 +1731,11 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         C.b_pcol = put(sym.perm_col); C.b_rp = put(hc.rowptr); C.b_cc = put(hc.colind); C.b_a2lu = put(sym.a2lu);
         C.b_vptr = put(vptr); C.b_mptr = put(mptr); C.b_lrp = put(lrp); C.b_lk = put(lk);
         C.b_bgp = put(bgp_h); C.b_bge = put(bge_h);
+        {
+          std::vector<int> vflag(hc.n);
+          for (int r = 0; r < hc.n; ++r) vflag[r] = hc.var_type[r] == 'V' ? 1 : 0;
+          C.b_vrow = put(vflag);
+        }
         blob_ok = (int)ftg.size() == C.nlu && (int)fls.size() == C.nflev && (int)fsr.size() == n &&
                   (int)bsr.size() == n && (int)fss.size() == C.nfs && (int)bss.size() == C.nbs &&
                   fop.back() == C.nops && (int)fops.size() == 2 * C.nops;
@@ -2013,6 +2158,22 @@ int xg_batch_set_method(xg_batch* b, int method) {
   if (!b || (method != 0 && method != 1) || b->started) return XG_E_ARG;
   b->args.method = method;
   b->drop_graph();
+  return XG_OK;
+}
+
+int xg_batch_dc_strategy(xg_batch* b, int first_stage) {
+  if (!b || first_stage < 0 || first_stage > 2 || b->started) return XG_E_ARG;
+  k_dc_first_stage<<<(b->n_inst + 255) / 256, 256, 0, b->stream>>>(b->n_inst, b->d_st.p, first_stage);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XG_OK;
+}
+
+int xg_batch_dc_stage(xg_batch* b, int inst, int* stage) {
+  if (!b || inst < 0 || inst >= b->n_inst || !stage) return XG_E_ARG;
+  InstState s;
+  HIPCHK(hipMemcpy(&s, b->d_st.p + inst, sizeof(s), hipMemcpyDeviceToHost));
+  *stage = (int)s.dc_stage;
   return XG_OK;
 }
 
