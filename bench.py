@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512)
-    ap.add_argument("--cpu-tstop", type=float, default=3e-9)
+    ap.add_argument("--cpu-tstop", type=float, default=20e-9)
     ap.add_argument("--form", choices=("auto", "split", "fused"), default="auto",
                     help="launch form: autotuned, separate evaluation kernel, or fused kernel")
     ap.add_argument("--iters", type=int, default=8, help="Newton iterations per launch of the fused form")
