@@ -7,7 +7,7 @@
 * a hand-enumerated 3-stage ring pins both against the reference's stamp
   rules (jacStamp with RD=RS=0 merged, Vsrc branch row, ground dropped);
 * the symbolic LU: valid permutations, zero-free diagonal, fill that equals
-  a brute-force boolean elimination.
+  a brute-force boolean elimination; a malformed pattern is rejected.
 """
 import os
 import re
@@ -133,3 +133,23 @@ def test_value_steered_transversal_avoids_tiny_diagonal():
     A = _dense_of(n, rp, ci, vals)
     B = A[np.ix_(s["perm_row"], s["perm_col"])]
     assert np.all(np.abs(np.diag(B)) >= 1e-4)
+
+
+@pytest.mark.parametrize("case", ["col_high", "col_negative", "rowptr_decreasing", "rowptr_start"])
+def test_malformed_pattern_is_rejected(case):
+    """The symbolic analysis (what xg_lu_symbolic runs before any device
+    work) rejects a malformed CSR pattern with XG_E_ARG instead of reading
+    out of bounds."""
+    rp = np.array([0, 2, 4, 6], np.int32)
+    ci = np.array([0, 1, 0, 1, 1, 2], np.int32)
+    if case == "col_high":
+        ci[3] = 3
+    elif case == "col_negative":
+        ci[0] = -1
+    elif case == "rowptr_decreasing":
+        rp[2] = 1
+    else:
+        rp = rp + 1
+    with pytest.raises(X.XgError, match="bad argument"):
+        X.host_lu_analyse(3, rp, ci)
+    X.host_lu_analyse(3, np.array([0, 2, 4, 6], np.int32), np.array([0, 1, 0, 1, 1, 2], np.int32))

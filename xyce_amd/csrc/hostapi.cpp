@@ -27,7 +27,7 @@ int xg_host_pattern(const xg_circuit_desc* d, int* sizes, int* rowptr, int* coli
 
 int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double* values, double* info, int* perm_row,
                        int* perm_col, int* lu_rowptr, int* lu_colind, int max_lu) {
-  if (n <= 0 || !rowptr || !colind || !info) return XG_E_ARG;
+  if (n <= 0 || !rowptr || !colind || !info || !xg::valid_csr(n, rowptr, colind)) return XG_E_ARG;
   xg::LuSymbolic s;
   try {
     const int st = xg::lu_analyse(n, rowptr, colind, values, s, true);
@@ -47,7 +47,7 @@ int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double
 }
 
 int xg_host_mf_plan(int n, const int* rowptr, const int* colind, const double* values, double* info) {
-  if (n <= 0 || !rowptr || !colind || !info) return XG_E_ARG;
+  if (n <= 0 || !rowptr || !colind || !info || !xg::valid_csr(n, rowptr, colind)) return XG_E_ARG;
   xg::LuSymbolic s;
   xg::MfPlan p;
   try {

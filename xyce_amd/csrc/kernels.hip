@@ -785,6 +785,7 @@ int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, d
 // ---------------- LU ----------------
 int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double* values, int method, xg_lu** out) {
   if (n <= 0 || !rowptr || !colind || !out || method < 0 || method > 2) return XG_E_ARG;
+  if (!xg::valid_csr(n, rowptr, colind)) return XG_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
   auto lu = std::make_unique<xg_lu>();

@@ -102,6 +102,17 @@ struct LuSymbolic {
 
 // want_schedule: build the entry-level factor / solve schedules;
 // keep_tree: keep the elimination tree and the L column structure
+// A square CSR pattern the symbolic analysis can read: rowptr[0] = 0,
+// non-decreasing, every column index in [0, n).
+inline bool valid_csr(int n, const int* rowptr, const int* colind) {
+  if (n <= 0 || !rowptr || !colind || rowptr[0] != 0) return false;
+  for (int r = 0; r < n; ++r)
+    if (rowptr[r + 1] < rowptr[r]) return false;
+  for (int k = 0; k < rowptr[n]; ++k)
+    if (colind[k] < 0 || colind[k] >= n) return false;
+  return true;
+}
+
 int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s,
                bool want_schedule = true, bool keep_tree = false);
 
