@@ -148,9 +148,17 @@ def test_gate_array_full_size_fp32_refined(gpu):
     vec = dm.loadDAEVectors()
     dm.loadDAEMatrices()
     rhs, J = dm.combine(ref["Q"], ref["F"] - ref["B"])
-    scale = np.maximum(np.abs(J_o), 1e-300)
-    assert np.max(np.abs(J - J_o) / np.maximum(scale, 1e-12 * np.max(np.abs(J_o)))) <= 1e-12
     p = o.pattern()
+    # entries within 1e-12 of their magnitude; an entry summed from n_t device
+    # terms (the supply rows collect one from every PMOS: n_t ~ 375k) within
+    # the rounding bound 2 n_t eps of the sum, which the reference's own
+    # sequential sum meets no better (the GPU sums long entries by a
+    # fixed-order workgroup reduction)
+    fs, qs = p["mos_fslot"].ravel(), p["mos_qslot"].ravel()
+    n_t = np.bincount(fs[fs >= 0], minlength=len(J)) + np.bincount(qs[qs >= 0], minlength=len(J))
+    tol = np.maximum(1e-12, 2 * n_t * np.finfo(float).eps)
+    scale = np.maximum(np.abs(J_o), 1e-12 * np.max(np.abs(J_o)))
+    assert np.all(np.abs(J - J_o) <= tol * scale), np.max(np.abs(J - J_o) / (tol * scale))
     s = X.DirectSolver(o.n, p["rowptr"], p["colind"], values=J)
     assert s.info()["method"] == 2
     s.set_precision(1, ir_tol=1e-12, ir_max=10)

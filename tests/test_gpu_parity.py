@@ -350,3 +350,43 @@ def test_reference_driver_drives_the_dropin(gpu):
     assert abs(r["steps"] - ref["steps"]) <= 0.02 * ref["steps"] + 2
     err = waveform_error(r["t"], r["x"], ref["t"], ref["x"])
     assert err <= 1e-3 * 1.8 + 1e-6, err
+
+
+def test_gate_array_loads_full_size(gpu):
+    """Device loads of the 250k-gate INV/NAND2 array (749,642 level-1
+    MOSFETs, 375,323 unknowns) against the oracle's, entry by entry: within
+    1e-12 of each entry's magnitude, or, for an entry summed from n_t device
+    terms (a supply row collects one from every PMOS), within 2 n_t eps of
+    it (the reference's own sequential sum meets no better); three repeated
+    loads are bit-identical (fixed summation order)."""
+    c = C.gate_dag(250_000, seed=4)
+    o = Oracle(c)
+    rng = np.random.default_rng(11)
+    x = 0.9 + rng.normal(0.0, 0.05, size=o.n)
+    sta0, sto0 = np.zeros(max(o.nsta, 1)), np.zeros(max(o.nsto, 1))
+    kw = dict(dcop=0, tran=1, init_jct=0, newton_iter=1, order=2, h=2e-12, time=0.0, tstep=1e-12, tstop=1e-9)
+    ref = o.load(x, sta0, sto0, **kw)
+    p = o.pattern()
+    eps = np.finfo(float).eps
+    fs, qs = p["mos_fslot"].ravel(), p["mos_qslot"].ravel()
+    nt_m = np.bincount(fs[fs >= 0], minlength=o.nnz) + np.bincount(qs[qs >= 0], minlength=o.nnz)
+    nt_v = np.bincount(p["mos_lid"].ravel()[p["mos_lid"].ravel() < o.n], minlength=o.n) + np.diff(p["rowptr"])
+    d = X.DeviceMgr(c)
+    d.set_solver_state(**kw)
+    runs = []
+    for _ in range(3):
+        d.updateState(x, sta0, sto0)
+        runs.append((d.loadDAEVectors(), d.loadDAEMatrices()))
+    vec, (dF, dQ) = runs[0]
+    for k in ("F", "Q", "dFdxdVp", "dQdxdVp"):
+        a, b = vec[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        tol = np.maximum(1e-12, 2 * nt_v * eps) * scale + 1e-30
+        assert np.all(np.abs(a - b) <= tol), (k, np.max(np.abs(a - b) / tol))
+    for a, b in ((dF, ref["dFdx"]), (dQ, ref["dQdx"])):
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        tol = np.maximum(1e-12, 2 * nt_m * eps) * scale + 1e-30
+        assert np.all(np.abs(a - b) <= tol), np.max(np.abs(a - b) / tol)
+    for v2, (dF2, dQ2) in runs[1:]:
+        assert all(np.array_equal(vec[k], v2[k]) for k in vec)
+        assert np.array_equal(dF, dF2) and np.array_equal(dQ, dQ2)

@@ -91,9 +91,28 @@ __device__ __forceinline__ double src_val(const SrcArgs& s, int src) {
   return wave_value(s.wave_type[w], s.wave_data + off, s.wave_off[w + 1] - off, s.time, s.tstep, s.tstop);
 }
 
+// Rows and CSR slots that sum more than this many terms (a supply node
+// collects a term from every PMOS) are summed by a whole workgroup.
+constexpr int kGatherLong = 64;
+
+// sum over the workgroup in a fixed order (lane-strided partial sums, a
+// fixed shuffle tree per wavefront, wavefronts in index order): the same
+// bits on every call
+__device__ __forceinline__ double wg_sum_fixed(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < nw; ++w) t += sh[w];
+  return t;
+}
+
 // Per row: F/Q/dFdxdVp/dQdxdVp from the record terms (device order), then the
 // linear part's row sums (FilteredMatrix::axpy, N_LAS_FilteredMatrix.C:511-536),
-// then B from the sources (DeviceMgr::loadBVectorsforSources).
+// then B from the sources (DeviceMgr::loadBVectorsforSources).  Long rows are
+// left to k_gather_vectors_long.
 __global__ void k_gather_vectors(int n, int n_mos, const int* __restrict__ vg_ptr, const int* __restrict__ vg_ent,
                                  const double* __restrict__ rec, const int* __restrict__ rowptr,
                                  const int* __restrict__ colind, const double* __restrict__ linF,
@@ -101,6 +120,7 @@ __global__ void k_gather_vectors(int n, int n_mos, const int* __restrict__ vg_pt
                                  double* F, double* Q, double* B, double* FL, double* QL) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
+  if (vg_ptr[r + 1] - vg_ptr[r] + rowptr[r + 1] - rowptr[r] > kGatherLong) return;
   double f = 0.0, q = 0.0, fl = 0.0, ql = 0.0;
   for (int e = vg_ptr[r]; e < vg_ptr[r + 1]; ++e) {
     const int ent = vg_ent[e];
@@ -125,6 +145,72 @@ __global__ void k_gather_vectors(int n, int n_mos, const int* __restrict__ vg_pt
   F[r] = f; Q[r] = q; B[r] = b; FL[r] = fl; QL[r] = ql;
 }
 
+// Long rows and slots are summed in two passes: chunks of at most
+// kGatherChunk terms, one workgroup each, into partial sums (fixed-order
+// workgroup reduction), then one thread per long entry adds its chunks'
+// partials in chunk order.  Chunk c covers terms [cb[c], ce[c]) of entry
+// ci[c]; kind 0 = record terms, 1 = linear CSR entries (vectors only).
+constexpr int kGatherChunk = 2048;
+
+__global__ void __launch_bounds__(256) k_gather_vectors_part(
+    int n_mos, const int* __restrict__ ck, const int* __restrict__ cb, const int* __restrict__ ce,
+    const int* __restrict__ vg_ent, const double* __restrict__ rec, const int* __restrict__ colind,
+    const double* __restrict__ linF, const double* __restrict__ linQ, const double* __restrict__ x, double* part) {
+  __shared__ double sh[16];
+  const int c = blockIdx.x;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (ck[c] == 0) {
+    for (int e = cb[c] + (int)threadIdx.x; e < ce[c]; e += blockDim.x) {
+      const int ent = vg_ent[e];
+      const int dev = ent >> 6, plane = ent & 63;
+      const double v = rec[(size_t)plane * n_mos + dev];
+      const int vid = d_rec_vec_id(plane);
+      if (vid == 0) a0 += v; else if (vid == 1) a1 += v; else if (vid == 2) a2 += v; else a3 += v;
+    }
+  } else {
+    for (int k = cb[c] + (int)threadIdx.x; k < ce[c]; k += blockDim.x) {
+      const double xv = x[colind[k]];
+      if (linF[k] != 0.0) a0 += linF[k] * xv;
+      if (linQ[k] != 0.0) a1 += linQ[k] * xv;
+    }
+  }
+  a0 = wg_sum_fixed(a0, sh); a1 = wg_sum_fixed(a1, sh); a2 = wg_sum_fixed(a2, sh); a3 = wg_sum_fixed(a3, sh);
+  if (threadIdx.x == 0) {
+    part[4 * c] = a0; part[4 * c + 1] = a1; part[4 * c + 2] = a2; part[4 * c + 3] = a3;
+  }
+}
+
+// one wavefront per long row: record-term chunks [rc[3l], rc[3l+1]), CSR
+// chunks [rc[3l+1], rc[3l+2]) of the partial sums (lane-strided, fixed
+// shuffle tree), then the sources
+__device__ __forceinline__ double wave_sum_fixed(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64) k_gather_vectors_fin(const int* __restrict__ rows, const int* __restrict__ rc,
+                                                          const double* __restrict__ part, SrcArgs s, double* F,
+                                                          double* Q, double* B, double* FL, double* QL) {
+  const int l = blockIdx.x, lane = threadIdx.x;
+  const int r = rows[l];
+  double f = 0.0, q = 0.0, fl = 0.0, ql = 0.0, sf = 0.0, sq = 0.0;
+  for (int c = rc[3 * l] + lane; c < rc[3 * l + 1]; c += 64) {
+    f += part[4 * c]; q += part[4 * c + 1]; fl += part[4 * c + 2]; ql += part[4 * c + 3];
+  }
+  for (int c = rc[3 * l + 1] + lane; c < rc[3 * l + 2]; c += 64) { sf += part[4 * c]; sq += part[4 * c + 1]; }
+  f = wave_sum_fixed(f); q = wave_sum_fixed(q); fl = wave_sum_fixed(fl); ql = wave_sum_fixed(ql);
+  sf = wave_sum_fixed(sf); sq = wave_sum_fixed(sq);
+  if (lane != 0) return;
+  f += sf; q += sq;
+  double b = 0.0;
+  for (int e = s.bg_ptr[r]; e < s.bg_ptr[r + 1]; ++e) {
+    const int ent = s.bg_ent[e];
+    const double v = src_val(s, ent >> 1);
+    if (ent & 1) b -= v; else b += v;
+  }
+  F[r] = f; Q[r] = q; B[r] = b; FL[r] = fl; QL[r] = ql;
+}
+
 #pragma clang fp contract(off)
 __device__ __forceinline__ double jac_combine_k(double a, double dq, double b, double df) { return a * dq + b * df; }
 #pragma clang fp contract(fast)
@@ -138,6 +224,7 @@ __global__ void k_gather_matrices(int nnz, int n_mos, const int* __restrict__ mg
                                   double* J, double* lu, const int* __restrict__ a2lu) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nnz) return;
+  if (mg_ptr[s + 1] - mg_ptr[s] > kGatherLong) return;   // k_gather_matrices_long
   double df = 0.0, dq = 0.0;
   for (int e = mg_ptr[s]; e < mg_ptr[s + 1]; ++e) {
     const int ent = mg_ent[e];
@@ -145,6 +232,48 @@ __global__ void k_gather_matrices(int nnz, int n_mos, const int* __restrict__ mg
     const double v = rec[(size_t)plane * n_mos + dev];
     if (plane < R_JQ0) df += v; else dq += v;
   }
+  if (linF[s] != 0.0) df += linF[s];
+  if (linQ[s] != 0.0) dq += linQ[s];
+  if (dFdx) { dFdx[s] = df; dQdx[s] = dq; }
+  if (J || lu) {
+    const double j = jac_combine_k(a, dq, b, df);
+    if (J) J[s] = j;
+    if (lu) lu[a2lu[s]] = j;
+  }
+}
+
+// Long CSR slots: chunk partial sums of dFdx and dQdx terms ...
+__global__ void __launch_bounds__(256) k_gather_matrices_part(int n_mos, const int* __restrict__ cb,
+                                                              const int* __restrict__ ce, const int* __restrict__ mg_ent,
+                                                              const double* __restrict__ rec, double* part) {
+  __shared__ double sh[16];
+  const int c = blockIdx.x;
+  double df = 0.0, dq = 0.0;
+  for (int e = cb[c] + (int)threadIdx.x; e < ce[c]; e += blockDim.x) {
+    const int ent = mg_ent[e];
+    const int dev = ent >> 6, plane = ent & 63;
+    const double v = rec[(size_t)plane * n_mos + dev];
+    if (plane < R_JQ0) df += v; else dq += v;
+  }
+  df = wg_sum_fixed(df, sh);
+  dq = wg_sum_fixed(dq, sh);
+  if (threadIdx.x == 0) { part[2 * c] = df; part[2 * c + 1] = dq; }
+}
+
+// ... then one wavefront per long slot over its chunks [sc[l], sc[l + 1])
+__global__ void __launch_bounds__(64) k_gather_matrices_fin(const int* __restrict__ slots, const int* __restrict__ sc,
+                                                           const double* __restrict__ part,
+                                                           const double* __restrict__ linF,
+                                                           const double* __restrict__ linQ, double* dFdx, double* dQdx,
+                                                           double a, double b, double* J, double* lu,
+                                                           const int* __restrict__ a2lu) {
+  const int l = blockIdx.x, lane = threadIdx.x;
+  const int s = slots[l];
+  double df = 0.0, dq = 0.0;
+  for (int c = sc[l] + lane; c < sc[l + 1]; c += 64) { df += part[2 * c]; dq += part[2 * c + 1]; }
+  df = wave_sum_fixed(df);
+  dq = wave_sum_fixed(dq);
+  if (lane != 0) return;
   if (linF[s] != 0.0) df += linF[s];
   if (linQ[s] != 0.0) dq += linQ[s];
   if (dFdx) { dFdx[s] = df; dQdx[s] = dq; }
@@ -576,6 +705,10 @@ struct xg_circuit {
   DBuf<double> sto_cur, sto_nxt, sta_cur, sta_nxt, rec;
   DBuf<double> x, F, Q, B, FL, QL, rhs, q0, q2, dFdx, dQdx, J, linF, linQ;
   DBuf<int> rowptr, colind, vg_ptr, vg_ent, mg_ptr, mg_ent, bg_ptr, bg_ent, src_wave, wave_type, wave_off, limited;
+  // long rows / slots: chunk lists (kind, begin, end), per-entry chunk ranges, partial sums
+  DBuf<int> long_rows, long_slots, vck, vcb, vce, vrc, mcb, mce, msc;
+  DBuf<double> vpart, mpart;
+  int n_long_rows = 0, n_long_slots = 0, n_vchunks = 0, n_mchunks = 0;
   DBuf<double> wave_data, xy_sta, xy_sto;
   xg_solver_state ss{};
 };
@@ -600,6 +733,41 @@ static int circuit_upload(xg_circuit* c) {
   HIPCHK(c->rowptr.upload(h.rowptr, s)); HIPCHK(c->colind.upload(h.colind, s));
   HIPCHK(c->vg_ptr.upload(h.vg_ptr, s)); HIPCHK(c->vg_ent.upload(h.vg_ent, s));
   HIPCHK(c->mg_ptr.upload(h.mg_ptr, s)); HIPCHK(c->mg_ent.upload(h.mg_ent, s));
+  {
+    std::vector<int> lr, ls;
+    for (int r = 0; r < n; ++r)
+      if (h.vg_ptr[r + 1] - h.vg_ptr[r] + h.rowptr[r + 1] - h.rowptr[r] > kGatherLong) lr.push_back(r);
+    for (int q = 0; q < nnz; ++q)
+      if (h.mg_ptr[q + 1] - h.mg_ptr[q] > kGatherLong) ls.push_back(q);
+    c->n_long_rows = (int)lr.size();
+    c->n_long_slots = (int)ls.size();
+    std::vector<int> ck, cb, ce, rc, mb, me, sc;
+    auto split = [&](int kind, int b0, int b1) {
+      for (int a = b0; a < b1; a += kGatherChunk) { ck.push_back(kind); cb.push_back(a); ce.push_back(std::min(b1, a + kGatherChunk)); }
+    };
+    for (int r : lr) {
+      rc.push_back((int)cb.size());
+      split(0, h.vg_ptr[r], h.vg_ptr[r + 1]);
+      rc.push_back((int)cb.size());
+      split(1, h.rowptr[r], h.rowptr[r + 1]);
+      rc.push_back((int)cb.size());
+    }
+    sc.push_back(0);
+    for (int q : ls) {
+      for (int a = h.mg_ptr[q]; a < h.mg_ptr[q + 1]; a += kGatherChunk) { mb.push_back(a); me.push_back(std::min(h.mg_ptr[q + 1], a + kGatherChunk)); }
+      sc.push_back((int)mb.size());
+    }
+    c->n_vchunks = (int)cb.size();
+    c->n_mchunks = (int)mb.size();
+    if (!lr.empty()) {
+      HIPCHK(c->long_rows.upload(lr, s)); HIPCHK(c->vck.upload(ck, s)); HIPCHK(c->vcb.upload(cb, s));
+      HIPCHK(c->vce.upload(ce, s)); HIPCHK(c->vrc.upload(rc, s)); HIPCHK(c->vpart.alloc(4 * (size_t)cb.size()));
+    }
+    if (!ls.empty()) {
+      HIPCHK(c->long_slots.upload(ls, s)); HIPCHK(c->mcb.upload(mb, s)); HIPCHK(c->mce.upload(me, s));
+      HIPCHK(c->msc.upload(sc, s)); HIPCHK(c->mpart.alloc(2 * (size_t)mb.size()));
+    }
+  }
   // per-row source gather lists (source order)
   std::vector<int> bcnt(n + 1, 0);
   for (auto& e : h.srcs) {
@@ -736,6 +904,12 @@ int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdx
   const int n = h.n;
   k_gather_vectors<<<nblk(n), 256, 0, s>>>(n, h.n_mos, c->vg_ptr.p, c->vg_ent.p, c->rec.p, c->rowptr.p, c->colind.p,
                                            c->linF.p, c->linQ.p, c->x.p, src_args(c), c->F.p, c->Q.p, c->B.p, c->FL.p, c->QL.p);
+  if (c->n_long_rows) {
+    k_gather_vectors_part<<<c->n_vchunks, 256, 0, s>>>(h.n_mos, c->vck.p, c->vcb.p, c->vce.p, c->vg_ent.p, c->rec.p,
+                                                       c->colind.p, c->linF.p, c->linQ.p, c->x.p, c->vpart.p);
+    k_gather_vectors_fin<<<c->n_long_rows, 64, 0, s>>>(c->long_rows.p, c->vrc.p, c->vpart.p, src_args(c), c->F.p,
+                                                       c->Q.p, c->B.p, c->FL.p, c->QL.p);
+  }
   HIPCHK(hipGetLastError());
   if (F) HIPCHK(hipMemcpyAsync(F, c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   if (Q) HIPCHK(hipMemcpyAsync(Q, c->Q.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
@@ -752,6 +926,11 @@ int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
   hipStream_t s = c->stream;
   k_gather_matrices<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, h.n_mos, c->mg_ptr.p, c->mg_ent.p, c->rec.p, c->linF.p,
                                                c->linQ.p, c->dFdx.p, c->dQdx.p, 0.0, 0.0, nullptr, nullptr, nullptr);
+  if (c->n_long_slots) {
+    k_gather_matrices_part<<<c->n_mchunks, 256, 0, s>>>(h.n_mos, c->mcb.p, c->mce.p, c->mg_ent.p, c->rec.p, c->mpart.p);
+    k_gather_matrices_fin<<<c->n_long_slots, 64, 0, s>>>(c->long_slots.p, c->msc.p, c->mpart.p, c->linF.p, c->linQ.p,
+                                                         c->dFdx.p, c->dQdx.p, 0.0, 0.0, nullptr, nullptr, nullptr);
+  }
   HIPCHK(hipGetLastError());
   if (dFdx_vals) HIPCHK(hipMemcpyAsync(dFdx_vals, c->dFdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
   if (dQdx_vals) HIPCHK(hipMemcpyAsync(dQdx_vals, c->dQdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
