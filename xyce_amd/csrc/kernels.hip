@@ -462,7 +462,7 @@ __global__ void k_residual_f64(int n, const int* __restrict__ rp, const int* __r
                                unsigned long long* red) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double ar = 0.0, ab = 0.0;
-  if (i < n) {
+  if (i < n && rp[i + 1] - rp[i] <= kGatherLong) {   // longer rows: k_spmv_part + k_residual_fin
     double s = 0.0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) s += a[k] * x[ci[k]];
     const double v = b[i] - s;
@@ -480,6 +480,37 @@ __global__ void k_residual_f64(int n, const int* __restrict__ rp, const int* __r
     atomicMax(&red[1], (unsigned long long)__double_as_longlong(ab));
   }
 }
+// rows of A with more than kGatherLong entries: chunk partial products
+// (fixed-order workgroup reduction) ...
+__global__ void __launch_bounds__(256) k_spmv_part(const int* __restrict__ cb, const int* __restrict__ ce,
+                                                   const int* __restrict__ ci, const double* __restrict__ a,
+                                                   const double* __restrict__ x, double* part) {
+  __shared__ double sh[16];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int k = cb[c] + (int)threadIdx.x; k < ce[c]; k += blockDim.x) acc += a[k] * x[ci[k]];
+  acc = wg_sum_fixed(acc, sh);
+  if (threadIdx.x == 0) part[c] = acc;
+}
+
+// ... then one wavefront per long row: its chunks in a fixed order, r = b - Ax
+__global__ void __launch_bounds__(64) k_residual_fin(const int* __restrict__ rows, const int* __restrict__ rc,
+                                                     const double* __restrict__ part, const double* __restrict__ b,
+                                                     double* __restrict__ r, unsigned long long* red) {
+  const int l = blockIdx.x, lane = threadIdx.x;
+  double acc = 0.0;
+  for (int c = rc[l] + lane; c < rc[l + 1]; c += 64) acc += part[c];
+  acc = wave_sum_fixed(acc);
+  if (lane != 0) return;
+  const int i = rows[l];
+  const double v = b[i] - acc;
+  r[i] = v;
+  double ar = fabs(v);
+  if (!isfinite(ar)) ar = INFINITY;
+  atomicMax(&red[0], (unsigned long long)__double_as_longlong(ar));
+  atomicMax(&red[1], (unsigned long long)__double_as_longlong(fabs(b[i])));
+}
+
 __global__ void k_axpy_f64(int n, const double* d, double* x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += d[i];
@@ -512,6 +543,9 @@ struct xg_lu {
   bool own_stream = false;
   DBuf<int> a2lu, lu_rowptr, lu_colind, lu_diag, f_lvl_ptr, f_tgt, f_op_ptr, f_ops, f_div;
   DBuf<int> fs_ptr, fs_rows, bs_ptr, bs_rows, prow, pcol, flag, arp, aci;
+  DBuf<int> long_rows, lcb, lce, lrc;   // rows of A with many entries: chunks for the residual
+  DBuf<double> lpart;
+  int n_long = 0, n_lchunks = 0;
   DBuf<double> lu, avals, y, b, x, r, d;
   DBuf<float> lu32, y32;
   DBuf<unsigned long long> red;
@@ -545,8 +579,34 @@ struct xg_lu {
     return XG_OK;
   }
 
+  // rows of A longer than kGatherLong (a supply node's) are summed in chunks
+  int setup_long_rows() {
+    std::vector<int> lr, cb, ce, rc{0};
+    const int n = (int)a_rowptr.size() - 1;
+    for (int i = 0; i < n; ++i) {
+      if (a_rowptr[i + 1] - a_rowptr[i] <= kGatherLong) continue;
+      lr.push_back(i);
+      for (int k = a_rowptr[i]; k < a_rowptr[i + 1]; k += kGatherChunk) {
+        cb.push_back(k);
+        ce.push_back(std::min(a_rowptr[i + 1], k + kGatherChunk));
+      }
+      rc.push_back((int)cb.size());
+    }
+    n_long = (int)lr.size();
+    n_lchunks = (int)cb.size();
+    if (n_long) {
+      HIPCHK(long_rows.upload(lr, stream)); HIPCHK(lcb.upload(cb, stream)); HIPCHK(lce.upload(ce, stream));
+      HIPCHK(lrc.upload(rc, stream)); HIPCHK(lpart.alloc(cb.size()));
+    }
+    return XG_OK;
+  }
+
   int upload() {
     hipStream_t st = stream;
+    {
+      const int rc = setup_long_rows();
+      if (rc != XG_OK) return rc;
+    }
     if (method == 2) {
       HIPCHK(prow.upload(s.perm_row, st)); HIPCHK(pcol.upload(s.perm_col, st));
       HIPCHK(arp.upload(a_rowptr, st)); HIPCHK(aci.upload(a_colind, st));
@@ -645,6 +705,10 @@ struct xg_lu {
   int residual(const double* db, const double* dx, double* rel) {
     HIPCHK(hipMemsetAsync(red.p, 0, 2 * sizeof(unsigned long long), stream));
     k_residual_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, arp.p, aci.p, avals.p, dx, db, r.p, red.p);
+    if (n_long) {
+      k_spmv_part<<<n_lchunks, 256, 0, stream>>>(lcb.p, lce.p, aci.p, avals.p, dx, lpart.p);
+      k_residual_fin<<<n_long, 64, 0, stream>>>(long_rows.p, lrc.p, lpart.p, db, r.p, red.p);
+    }
     unsigned long long h[2];
     HIPCHK(hipMemcpyAsync(h, red.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));

@@ -256,6 +256,107 @@ void amd_order(const Graph& g0, const std::vector<int>& nodes, std::vector<int>&
   }
 }
 
+// Height of the elimination tree of the symmetric pattern g restricted to
+// `order` (eliminated in that sequence; Liu's algorithm with path compression).
+int etree_height(const Graph& g, const std::vector<int>& order) {
+  const int m = (int)order.size();
+  std::vector<int> pos(g.size(), -1);
+  for (int i = 0; i < m; ++i) pos[order[i]] = i;
+  std::vector<int> parent(m, -1), anc(m, -1);
+  for (int i = 0; i < m; ++i) {
+    for (int a : g[order[i]]) {
+      int k = pos[a];
+      if (k < 0 || k >= i) continue;
+      while (k != -1 && k < i) {
+        const int nx = anc[k];
+        anc[k] = i;
+        if (nx == -1) { parent[k] = i; break; }
+        k = nx;
+      }
+    }
+  }
+  std::vector<int> h(m, 0);
+  int best = 0;
+  for (int i = 0; i < m; ++i) {
+    if (parent[i] >= 0) h[parent[i]] = std::max(h[parent[i]], h[i] + 1);
+    best = std::max(best, h[i]);
+  }
+  return best + 1;
+}
+
+// Nested dissection by breadth-first level structures (George & Liu): from a
+// pseudo-peripheral node, the level of smallest size near the half-way count
+// separates the levels before it from the levels after it; the two parts are
+// ordered recursively, then the separator.  Parts of at most `leaf` nodes get
+// approximate minimum degree.  Gives a shallow elimination tree (about
+// log2(n / leaf) separator levels) where minimum degree leaves long chains.
+void nd_order(const Graph& g, std::vector<int> nodes, std::vector<int>& order, int leaf, std::vector<int>& tag,
+              int& stamp, std::vector<int>& lvl) {
+  if ((int)nodes.size() <= leaf) { amd_order(g, nodes, order); return; }
+  const int id = ++stamp;
+  for (int v : nodes) tag[v] = id;
+  auto bfs = [&](int root, std::vector<int>& seq, std::vector<int>& lptr) {
+    seq.clear(); lptr.assign(1, 0);
+    seq.push_back(root);
+    lvl[root] = 0;
+    tag[root] = -id;
+    size_t head = 0;
+    while (head < seq.size()) {
+      const size_t end = seq.size();
+      for (; head < end; ++head)
+        for (int a : g[seq[head]])
+          if (tag[a] == id) { tag[a] = -id; lvl[a] = lvl[seq[head]] + 1; seq.push_back(a); }
+      lptr.push_back((int)end);
+    }
+    if (lptr.back() != (int)seq.size()) lptr.push_back((int)seq.size());
+    for (int v : seq) tag[v] = id;
+  };
+  std::vector<int> seq, lptr;
+  int root = nodes[0];
+  for (int it = 0; it < 3; ++it) {      // pseudo-peripheral node: deepest level, smallest degree
+    bfs(root, seq, lptr);
+    const int last = lptr[lptr.size() - 2];
+    int best = seq[last];
+    for (int q = last; q < (int)seq.size(); ++q)
+      if (g[seq[q]].size() < g[best].size()) best = seq[q];
+    if (best == root) break;
+    root = best;
+  }
+  bfs(root, seq, lptr);
+  if (seq.size() < nodes.size()) {
+    // disconnected: the component and the rest are independent
+    std::vector<int> rest;
+    for (int v : seq) tag[v] = -id;
+    for (int v : nodes) if (tag[v] == id) rest.push_back(v);
+    nd_order(g, seq, order, leaf, tag, stamp, lvl);
+    nd_order(g, rest, order, leaf, tag, stamp, lvl);
+    return;
+  }
+  const int nl = (int)lptr.size() - 1;
+  if (nl < 3) { amd_order(g, nodes, order); return; }
+  const int half = (int)seq.size() / 2;
+  int mid = 1;
+  while (mid < nl - 1 && lptr[mid + 1] <= half) ++mid;
+  // smallest level within a window around the middle that keeps both sides
+  // at least a third of the nodes
+  int sep = mid;
+  for (int L = 1; L < nl - 1; ++L) {
+    if (lptr[L] < (int)seq.size() / 3 || (int)seq.size() - lptr[L + 1] < (int)seq.size() / 3) continue;
+    if (lptr[L + 1] - lptr[L] < lptr[sep + 1] - lptr[sep]) sep = L;
+  }
+  std::vector<int> A(seq.begin(), seq.begin() + lptr[sep]), B(seq.begin() + lptr[sep + 1], seq.end()), S;
+  // separator nodes with no neighbour beyond the separator join the first part
+  for (int q = lptr[sep]; q < lptr[sep + 1]; ++q) {
+    const int v = seq[q];
+    bool beyond = false;
+    for (int a : g[v]) if (tag[a] == id && lvl[a] == sep + 1) { beyond = true; break; }
+    (beyond ? S : A).push_back(v);
+  }
+  nd_order(g, std::move(A), order, leaf, tag, stamp, lvl);
+  nd_order(g, std::move(B), order, leaf, tag, stamp, lvl);
+  amd_order(g, S, order);
+}
+
 }  // namespace
 
 int lu_analyse(int n, const int* rowptr, const int* colind, const double* values, LuSymbolic& s, bool want_schedule,
@@ -282,7 +383,21 @@ int lu_analyse(int n, const int* rowptr, const int* colind, const double* values
   // small systems (the batched engine's classes): multiple minimum degree,
   // whose independent sets keep the level schedule shallow; larger: AMD
   if ((int)sparse_nodes.size() <= 256) mmd_order(g, sparse_nodes, order);
-  else amd_order(g, sparse_nodes, order);
+  else {
+    amd_order(g, sparse_nodes, order);
+    // a large system whose minimum-degree elimination tree is a long chain
+    // (gate arrays wired with locality) is dissected instead: the GPU
+    // factorisation proceeds one tree level after another
+    if ((int)sparse_nodes.size() > 65535) {
+      const int h_amd = etree_height(g, order);
+      if (h_amd > 2000) {
+        std::vector<int> nd, tag(n, 0), lvl(n, 0);
+        int stamp = 0;
+        nd_order(g, sparse_nodes, nd, 4096, tag, stamp, lvl);
+        if ((int)nd.size() == (int)sparse_nodes.size() && etree_height(g, nd) * 4 < h_amd) order.swap(nd);
+      }
+    }
+  }
   order.insert(order.end(), dense_nodes.begin(), dense_nodes.end());
   s.dense_tail = (int)dense_nodes.size();
   // B(i,j) = A'(order[i], order[j]) = A(match[order[i]], order[j])
