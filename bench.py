@@ -127,6 +127,7 @@ def main():
         if args.form != "auto":
             b.config(iters_per_launch=args.iters, split_eval=1 if args.form == "split" else 0)
     log(f"launch form: {tune} (form {args.form})")
+    log(f"launch groups: {b.groups()}")
     if args.form == "auto":
         k_iter = tune["iters_per_launch"]
     else:

@@ -191,7 +191,9 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
  * the per-instance kernel, one Newton iteration per launch; 0: the
  * evaluation is fused into the per-instance kernel, which then runs
  * iters_per_launch iterations back to back with the instance's state and
- * program kept in LDS (0 = keep); -1 = keep.  concurrent_groups 1: the
+ * program kept in LDS (0 = keep); 2: fused, and each group's term arrays
+ * (the slot-ordered device contributions) kept in LDS too where they fit;
+ * -1 = keep.  concurrent_groups 1: the
  * instances' LDS-sized groups run side by side on streams of their own, 0:
  * one after another, -1 = keep. */
 int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups);
@@ -210,6 +212,12 @@ int xg_batch_set_method(xg_batch* b, int method);
  * first launch.  xg_batch_dc_stage reports the stage that produced an
  * instance's operating point (0 Newton, 1 gmin, 2 source stepping). */
 int xg_batch_dc_strategy(xg_batch* b, int first_stage);
+/* Launch group `group` (instances grouped by LDS need): info[0] instances,
+ * [1] workgroup size, [2] 1 if the fused form keeps their term arrays in LDS
+ * (chosen by xg_batch_autotune), [3] dynamic LDS bytes per workgroup,
+ * [4] the bytes it would take with the term arrays in LDS.  Returns the
+ * number of groups (info left untouched if group >= it). */
+int xg_batch_group_info(xg_batch* b, int group, int* info);
 int xg_batch_dc_stage(xg_batch* b, int inst, int* stage);
 /* Per-phase device timers of the first-scheduled instance (the counterpart of
  * the reference's Residual / Jacobian / Linear-solve StatTop timers,

@@ -206,13 +206,14 @@ def test_batch_uneven_instances_match_individual_runs(gpu):
 
 def test_launch_forms_do_not_change_results(gpu):
     """The separate evaluation kernel (one iteration per launch) and the
-    fused kernel with 1 or K Newton iterations per launch (state kept in LDS),
+    fused kernel with 1 or K Newton iterations per launch (state kept in LDS,
+    the term arrays in HBM or in LDS),
     with the LDS-sized instance groups one after another or side by side on
     their own streams, give bit-identical trajectories."""
     cs = [C.ring(s, seed=7 + s, mc_seed=7 + s) for s in (9, 23, 15, 31, 11)]
     out = []
     ns = [X.host_pattern(c)["n"] for c in cs]
-    for split, k, conc in ((1, 1, 0), (0, 1, 0), (0, 4, 0), (0, 4, 1), (1, 1, 1)):
+    for split, k, conc in ((1, 1, 0), (0, 1, 0), (0, 4, 0), (0, 4, 1), (1, 1, 1), (2, 4, 0), (2, 1, 1)):
         b = X.Batch(cs, 0.5e-9, 1e-12, 1e-11, record_max=2000)
         b.config(iters_per_launch=k, split_eval=split, concurrent_groups=conc)
         b.run_to_end(chunk=64)

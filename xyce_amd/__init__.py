@@ -41,7 +41,7 @@ EXPORTS = [
     "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
-    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
+    "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_group_info", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
 ]
@@ -416,7 +416,22 @@ class Batch:
         _check(lib().xg_batch_autotune(self.h, _p(r)), "xg_batch_autotune")
         return dict(block=int(r[0]), order_mode=int(r[1]), us_per_iteration=float(r[2]), split_eval=int(r[3]),
                     iters_per_launch=int(r[4]), group_blocks=[int(v) for v in r[5:9] if v > 0],
-                    concurrent_groups=int(r[9]))
+                    concurrent_groups=int(r[9]), group_terms_in_lds=[g["terms_in_lds"] for g in self.groups()])
+
+    def groups(self):
+        """Launch groups: instances, workgroup size, term arrays in LDS, LDS
+        bytes, LDS bytes with the term arrays in LDS."""
+        out = []
+        info = np.zeros(5, np.int32)
+        g = 0
+        while True:
+            cnt = lib().xg_batch_group_info(self.h, C.c_int(g), _p(info))
+            _check(min(cnt, 0), "xg_batch_group_info")
+            if g >= cnt:
+                return out
+            out.append(dict(instances=int(info[0]), block=int(info[1]), terms_in_lds=int(info[2]), lds_bytes=int(info[3]),
+                            lds_bytes_with_terms=int(info[4])))
+            g += 1
 
 
 class MLP:

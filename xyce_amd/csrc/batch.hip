@@ -139,6 +139,7 @@ struct BatchArgs {
   int rec_max; double* rec_t; double* rec_x;
   int max_n, max_nlu, max_blob;   // over all instances (max_n: stride of the recorded points)
   int lds_n, lds_nlu;              // LDS layout of this launch's instances (their largest class)
+  int lds_terms, lds_blob;         // fused form: the term array in LDS after the program (lds_blob uint16)
   int n_kcls;                  // entries of the MOSFET class table
   int iters;                   // Newton iterations per launch
   unsigned long long* prof;    // per-phase wall-clock ticks of one instance (null = off)
@@ -465,13 +466,12 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
 // (N_LAS_AmesosSolver.C:318) would pick different pivots.  Returns nonzero if
 // a zero pivot is met (singular).
 __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const InstDesc& I, const int* ci,
-                                 const unsigned short* prog, double ja, double jb, double gdiag, double* rhs,
-                                 double* red, int* sh_i) {
+                                 const unsigned short* prog, const double* T, double ja, double jb, double gdiag,
+                                 double* rhs, double* red, int* sh_i) {
   const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
   double* D = A.dense + I.dense_off;
   const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
   const unsigned short* mptr = prog + C.b_mptr;
-  const double* T = A.terms + I.term_off;
   const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
   for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
   __syncthreads();
@@ -603,13 +603,13 @@ template <bool FUSED>
 __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I, const ClassDesc& C, InstState& S,
                                 double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
                                 double* y, double* lu, const unsigned short* prog, const double* kcls,
-                                double* srcv) {
+                                double* srcv, double* T) {
   const int tid = threadIdx.x, nth = blockDim.x;
   const int n = C.n, off = I.vec_off;
   const int* ci = A.ci;
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
-  double* T = A.terms + I.term_off;
+  // T: the instance's slot-ordered term array (HBM, or LDS in the fused form)
   // phase timers (one instance, thread 0, after each phase's barrier)
   const bool prof_on = A.prof != nullptr && blockIdx.x == 0 && tid == 0;
   unsigned long long prof_t = prof_on ? wall_clock64() : 0;
@@ -925,7 +925,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }
     if (use_dense) {
       if (tid == 0) S.n_dense++;
-      const int sing = dense_pivot_solve(A, C, I, ci, prog, ja, jb, dc ? S.gcond : 0.0, rhs, red, red_i);
+      const int sing = dense_pivot_solve(A, C, I, ci, prog, T, ja, jb, dc ? S.gcond : 0.0, rhs, red, red_i);
       if (sing && tid == 0) sh_int[2] |= 1;
       __syncthreads();
     }
@@ -1229,9 +1229,13 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     kcls = kc_s;
   }
   __syncthreads();
+  // fused form with lds_terms: the term array lives in LDS after the program
+  // (8-byte aligned); it is written and read within each Newton iteration
+  double* T = (FUSED && A.lds_terms) ? reinterpret_cast<double*>(prog + ((A.lds_blob + 3) & ~3))
+                                      : A.terms + I.term_off;
   const int iters = FUSED ? A.iters : 1;
   for (int it = 0; it < iters; ++it) {
-    batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv);
+    batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv, T);
     __syncthreads();
     if (S.mode >= 2 || S.mode < 0) break;
   }
@@ -1379,6 +1383,8 @@ struct DArr {
   }
 };
 
+constexpr size_t kMaxLdsBytes = 150 * 1024;   // dynamic LDS per workgroup this engine uses at most
+
 struct xg_batch {
   int n_inst = 0;
   bool started = false;           // a launch has run (the integration method is fixed from then on)
@@ -1416,7 +1422,10 @@ struct xg_batch {
   size_t shmem = 0;
   // launch buckets: instances grouped by the LDS their class needs, one
   // per-instance launch each with its own dynamic LDS size
-  struct Bucket { int start, count, n, nlu; size_t shmem; int block; };   // block 0: the batch's
+  // block 0: the batch's; terms: fused form keeps the term arrays in LDS
+  // (shmem_terms bytes, chosen by the autotuner where it fits)
+  struct Bucket { int start, count, n, nlu; size_t shmem; int block; int blob = 0, nterm = 0; bool terms = false;
+                  size_t shmem_terms = 0; };
   std::vector<Bucket> buckets;
   std::vector<size_t> cls_lds;    // per class: dynamic LDS bytes
   std::vector<int> inst_cls;
@@ -1459,12 +1468,15 @@ struct xg_batch {
         const int s0 = (int)((long long)n_inst * k / nb), s1 = (int)((long long)n_inst * (k + 1) / nb);
         if (s1 <= s0) continue;
         Bucket bk{s0, s1 - s0, 0, 0, 0, 0};
-        int blob = 0;
+        int blob = 0, nterm = 0;
         for (int j = s0; j < s1; ++j) {
           const ClassDesc& C = classes[inst_cls[ord[j]]];
           bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu); blob = std::max(blob, C.blob_len);
+          nterm = std::max(nterm, C.nterm);
         }
         bk.shmem = lds_bytes(bk.n, bk.nlu, blob);
+        bk.blob = blob; bk.nterm = nterm;
+        bk.shmem_terms = lds_bytes(bk.n, bk.nlu, ((blob + 3) & ~3)) + sizeof(double) * (size_t)nterm;
         buckets.push_back(bk);
       }
     } else {
@@ -1490,6 +1502,9 @@ struct xg_batch {
       const Bucket& bk = buckets[j];
       BatchArgs a = args;
       a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
+      const bool lt = !split && bk.terms;
+      a.lds_terms = lt ? 1 : 0; a.lds_blob = bk.blob;
+      const size_t shm = lt ? bk.shmem_terms : bk.shmem;
       const int bs = bk.block ? bk.block : block;
       hipStream_t s = stream;
       if (side) {
@@ -1497,7 +1512,7 @@ struct xg_batch {
         if (hipStreamWaitEvent(s, fork_ev, 0) != hipSuccess) return XG_E_HIP;
       }
       if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(bs), bk.shmem, s, a);
-      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), shm, s, a);
       if (side) {
         if (hipEventRecord(join_ev[j], s) != hipSuccess) return XG_E_HIP;
         if (hipStreamWaitEvent(stream, join_ev[j], 0) != hipSuccess) return XG_E_HIP;
@@ -1871,7 +1886,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
   for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu, C.blob_len));
   for (const InstDesc& D : b->insts) b->inst_cls.push_back(D.cls);
-  if (b->shmem > 150 * 1024) return XG_E_ARG;
+  if (b->shmem > kMaxLdsBytes) return XG_E_ARG;
   b->set_order(1);
   *out = b.release();
   return XG_OK;
@@ -2069,7 +2084,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     *us = b->last_seconds / timed * 1e6;
     return XG_OK;
   };
-  struct Cand { double us; int block, order, k; bool conc; std::vector<int> gblocks; };
+  struct Cand { double us; int block, order, k; bool conc; std::vector<int> gblocks, gterms; };
   Cand best_split{1e30, b->block, 1, 1, false, {}}, best_fused{1e30, b->block, 1, 8, false, {}};
   b->concurrent_groups = false;
   for (int sp = 1; sp >= 0; --sp) {
@@ -2097,30 +2112,39 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   {
     double best_total = 1e30;
     int best_kk = 8;
-    std::vector<int> best_blocks(b->buckets.size(), best_fused.block);
+    std::vector<int> best_blocks(b->buckets.size(), best_fused.block), best_terms(b->buckets.size(), 0);
     for (int kk : {4, 8, 16}) {
       b->args.iters = kk;
       double total = 0;
-      std::vector<int> bb(b->buckets.size(), best_fused.block);
+      std::vector<int> bb(b->buckets.size(), best_fused.block), bt(b->buckets.size(), 0);
       for (size_t j = 0; j < b->buckets.size(); ++j) {
         double bj = 1e30;
         for (int bs : {128, 256}) {
-          b->buckets[j].block = bs;
-          b->only_bucket = (int)j;
-          double us = 0;
-          const int st = time_launches(1, 4, &us);
-          b->only_bucket = -1;
-          if (st != XG_OK) return st;
-          if (us < bj) { bj = us; bb[j] = bs; }
+          for (int lt = 0; lt < 2; ++lt) {
+            // the term arrays in LDS where they fit a workgroup's LDS
+            if (lt && b->buckets[j].shmem_terms > kMaxLdsBytes) continue;
+            b->buckets[j].block = bs;
+            b->buckets[j].terms = lt != 0;
+            b->only_bucket = (int)j;
+            double us = 0;
+            const int st = time_launches(1, 4, &us);
+            b->only_bucket = -1;
+            if (st != XG_OK) return st;
+            if (us < bj) { bj = us; bb[j] = bs; bt[j] = lt; }
+          }
         }
         b->buckets[j].block = bb[j];
+        b->buckets[j].terms = bt[j] != 0;
         total += bj;
       }
       total /= kk;
-      if (total < best_total) { best_total = total; best_kk = kk; best_blocks = bb; }
+      if (total < best_total) { best_total = total; best_kk = kk; best_blocks = bb; best_terms = bt; }
     }
     b->args.iters = best_kk;
-    for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = best_blocks[j];
+    for (size_t j = 0; j < b->buckets.size(); ++j) {
+      b->buckets[j].block = best_blocks[j];
+      b->buckets[j].terms = best_terms[j] != 0;
+    }
     double seq = 0, con = 0;
     b->concurrent_groups = false;
     int st = time_launches(1, 4, &seq);
@@ -2130,7 +2154,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     if (st != XG_OK) return st;
     const double grouped = std::min(seq, con) / best_kk;
     if (grouped < best_fused.us) {
-      best_fused = Cand{grouped, best_fused.block, 1, best_kk, con < seq, best_blocks};
+      best_fused = Cand{grouped, best_fused.block, 1, best_kk, con < seq, best_blocks, best_terms};
     }
   }
   // the winner
@@ -2143,6 +2167,8 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   b->concurrent_groups = w.conc;
   if (!w.gblocks.empty() && w.gblocks.size() == b->buckets.size())
     for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].block = w.gblocks[j];
+  if (!w.gterms.empty() && w.gterms.size() == b->buckets.size())
+    for (size_t j = 0; j < b->buckets.size(); ++j) b->buckets[j].terms = w.gterms[j] != 0;
   b->drop_graph();
   b->tune_us = w.us;
   if (result) {
@@ -2159,6 +2185,18 @@ int xg_batch_set_method(xg_batch* b, int method) {
   b->args.method = method;
   b->drop_graph();
   return XG_OK;
+}
+
+int xg_batch_group_info(xg_batch* b, int group, int* info) {
+  if (!b || !info || group < 0) return XG_E_ARG;
+  if (group >= (int)b->buckets.size()) return 0;
+  const auto& bk = b->buckets[group];
+  info[0] = bk.count;
+  info[1] = bk.block ? bk.block : b->block;
+  info[2] = !b->split && bk.terms ? 1 : 0;
+  info[3] = (int)(!b->split && bk.terms ? bk.shmem_terms : bk.shmem);
+  info[4] = (int)bk.shmem_terms;
+  return (int)b->buckets.size();
 }
 
 int xg_batch_dc_strategy(xg_batch* b, int first_stage) {
@@ -2178,7 +2216,7 @@ int xg_batch_dc_stage(xg_batch* b, int inst, int* stage) {
 }
 
 int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups) {
-  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 1 || concurrent_groups < -1 ||
+  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 2 || concurrent_groups < -1 ||
       concurrent_groups > 1)
     return XG_E_ARG;
   if (concurrent_groups >= 0) b->concurrent_groups = concurrent_groups == 1;
@@ -2187,7 +2225,11 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
     b->block = block;
     for (auto& bk : b->buckets) bk.block = 0;
   }
-  if (split_eval >= 0) b->split = split_eval;
+  if (split_eval >= 0) {
+    b->split = split_eval == 1 ? 1 : 0;
+    // 2: fused, each group's term arrays in LDS where they fit; 0: in HBM
+    for (auto& bk : b->buckets) bk.terms = split_eval == 2 && bk.shmem_terms <= kMaxLdsBytes;
+  }
   if (iters_per_launch) b->args.iters = iters_per_launch;
   if (b->split) b->args.iters = 1;
   b->drop_graph();
