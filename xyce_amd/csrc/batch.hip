@@ -2119,7 +2119,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
       std::vector<int> bb(b->buckets.size(), best_fused.block), bt(b->buckets.size(), 0);
       for (size_t j = 0; j < b->buckets.size(); ++j) {
         double bj = 1e30;
-        for (int bs : {128, 256}) {
+        for (int bs : blocks) {
           for (int lt = 0; lt < 2; ++lt) {
             // the term arrays in LDS where they fit a workgroup's LDS
             if (lt && b->buckets[j].shmem_terms > kMaxLdsBytes) continue;
