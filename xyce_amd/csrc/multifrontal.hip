@@ -517,20 +517,36 @@ __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int
   }
 }
 
-// sum over c in [c0, c1) of row[c] v[c] by one wavefront (lanes over c, four
-// loads in flight per lane), returned in every lane
+// sums over c in [c0, c1) of F[r + k][c] v[c] for the rows r .. r + nr - 1
+// (nr <= 4, wave-uniform) of a row-major front, by one wavefront: every lane
+// keeps eight loads in flight (four rows, two column blocks), so the four rows
+// share their memory round trips; row k's sum is returned in out[k] in every
+// lane
 template <class T>
-__device__ __forceinline__ T mf_row_dot(const T* row, const T* v, int c0, int c1, int lane) {
-  T a0 = (T)0, a1 = (T)0, a2 = (T)0, a3 = (T)0;
+__device__ __forceinline__ void mf_rows_dot4(const T* Fs, long long ms, int r, int nr, const T* v, int c0, int c1,
+                                             int lane, T out[4]) {
+  const T* row[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) row[k] = Fs + (long long)(r + (k < nr ? k : 0)) * ms;
+  T a[4], b[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) { a[k] = (T)0; b[k] = (T)0; }
   int c = c0 + lane;
-  for (; c + 192 < c1; c += 256) {
-    a0 += row[c] * v[c];
-    a1 += row[c + 64] * v[c + 64];
-    a2 += row[c + 128] * v[c + 128];
-    a3 += row[c + 192] * v[c + 192];
+  for (; c + 64 < c1; c += 128) {
+    const T v0 = v[c], v1 = v[c + 64];
+    T f0[4], f1[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { f0[k] = row[k][c]; f1[k] = row[k][c + 64]; }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { a[k] += f0[k] * v0; b[k] += f1[k] * v1; }
   }
-  for (; c < c1; c += 64) a0 += row[c] * v[c];
-  return mf_wave_sum((a0 + a1) + (a2 + a3));
+  if (c < c1) {
+    const T v0 = v[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] += row[k][c] * v0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out[k] = mf_wave_sum(a[k] + b[k]);
 }
 
 // ---- solves.  V holds, per front, the vector of its index list: y on the
@@ -592,9 +608,14 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int
   for (int b0 = 0; b0 < np; b0 += 64) {
     const int bw = min(64, np - b0);
     if (b0 > 0) {
-      for (int rr = wave; rr < bw; rr += nw) {
-        const T acc = mf_row_dot(Fs + (long long)(b0 + rr) * ms, Vl, 0, b0, lane);
-        if (lane == 0) Vl[b0 + rr] -= acc;
+      for (int rr = 4 * wave; rr < bw; rr += 4 * nw) {
+        const int nr = min(4, bw - rr);
+        T acc[4];
+        mf_rows_dot4(Fs, ms, b0 + rr, nr, Vl, 0, b0, lane, acc);
+        if (lane == 0)
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k < nr) Vl[b0 + rr + k] -= acc[k];
       }
     }
     {
@@ -621,9 +642,14 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_fwd_big(MfDev d, const int
     }
     __syncthreads();
   }
-  for (int rr = np + wave; rr < ms; rr += nw) {
-    const T acc = mf_row_dot(Fs + (long long)rr * ms, Vl, 0, np, lane);
-    if (lane == 0) Vl[rr] -= acc;
+  for (int rr = np + 4 * wave; rr < ms; rr += 4 * nw) {
+    const int nr = min(4, ms - rr);
+    T acc[4];
+    mf_rows_dot4(Fs, ms, rr, nr, Vl, 0, np, lane, acc);
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < nr) Vl[rr + k] -= acc[k];
   }
   __syncthreads();
   if (ldsv)
@@ -655,8 +681,11 @@ __global__ void __launch_bounds__(64) k_mf_bwd_tiny(MfDev d, const int* list, co
     }
     for (; u < ms; ++u) v -= row[u] * Xl[u];
   }
+  // each lane's pivot reciprocal first (in parallel), so that the serial
+  // sweep multiplies instead of dividing
+  const T rd = t < np ? (T)1 / Fs[(long long)t * ms + t] : (T)0;
   for (int k = np - 1; k >= 0; --k) {
-    const T xk = mf_bcast(v, k) / Fs[(long long)k * ms + k];
+    const T xk = mf_bcast(v, k) * mf_bcast(rd, k);
     if (t == k) v = xk;
     else if (t < k) v -= Fs[(long long)t * ms + k] * xk;
   }
@@ -678,17 +707,27 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nw = blockDim.x >> 6;
   for (int i = t; i < ms; i += blockDim.x) Xl[i] = x[ix[i]];
   __syncthreads();
-  for (int rr = wave; rr < np; rr += nw) {
-    const T acc = mf_row_dot(Fs + (long long)rr * ms, Xl, np, ms, lane);
-    if (lane == 0) Xl[rr] -= acc;
+  for (int rr = 4 * wave; rr < np; rr += 4 * nw) {
+    const int nr = min(4, np - rr);
+    T acc[4];
+    mf_rows_dot4(Fs, ms, rr, nr, Xl, np, ms, lane, acc);
+    if (lane == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (k < nr) Xl[rr + k] -= acc[k];
   }
   __syncthreads();
   for (int b1 = np; b1 > 0; b1 -= 64) {
     const int b0 = max(0, b1 - 64), bw = b1 - b0;
     if (b1 < np) {
-      for (int rr = wave; rr < bw; rr += nw) {
-        const T acc = mf_row_dot(Fs + (long long)(b0 + rr) * ms, Xl, b1, np, lane);
-        if (lane == 0) Xl[b0 + rr] -= acc;
+      for (int rr = 4 * wave; rr < bw; rr += 4 * nw) {
+        const int nr = min(4, bw - rr);
+        T acc[4];
+        mf_rows_dot4(Fs, ms, b0 + rr, nr, Xl, b1, np, lane, acc);
+        if (lane == 0)
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k < nr) Xl[b0 + rr + k] -= acc[k];
       }
     }
     {
@@ -707,8 +746,9 @@ __global__ void __launch_bounds__(kMfSolveBlock) k_mf_bwd_big(MfDev d, const int
     __syncthreads();
     if (wave == 0) {
       T v = lane < bw ? Xl[b0 + lane] : (T)0;
+      const T rd = lane < bw ? (T)1 / Bl[lane * 65 + lane] : (T)0;
       for (int k = bw - 1; k >= 0; --k) {
-        const T xk = mf_bcast(v, k) / Bl[k * 65 + k];
+        const T xk = mf_bcast(v, k) * mf_bcast(rd, k);
         if (lane == k) v = xk;
         else if (lane < k) v -= Bl[lane * 65 + k] * xk;
       }
