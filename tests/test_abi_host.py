@@ -7,7 +7,10 @@
 * a hand-enumerated 3-stage ring pins both against the reference's stamp
   rules (jacStamp with RD=RS=0 merged, Vsrc branch row, ground dropped);
 * the symbolic LU: valid permutations, zero-free diagonal, fill that equals
-  a brute-force boolean elimination; a malformed pattern is rejected.
+  a brute-force boolean elimination; a malformed pattern is rejected;
+* the patterns are one strongly connected block apart from each voltage
+  source's two 1x1 blocks, so the block split of KLU's BTF would not change
+  the ordering.
 """
 import os
 import re
@@ -153,3 +156,21 @@ def test_malformed_pattern_is_rejected(case):
     with pytest.raises(X.XgError, match="bad argument"):
         X.host_lu_analyse(3, rp, ci)
     X.host_lu_analyse(3, np.array([0, 2, 4, 6], np.int32), np.array([0, 1, 0, 1, 1, 2], np.int32))
+
+
+@pytest.mark.parametrize("name", ["ring", "grid", "gates"])
+def test_patterns_are_one_strongly_connected_block(name):
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components, maximum_bipartite_matching
+    c = {"ring": lambda: C.ring(101),
+         "grid": lambda: C.power_grid(40, 40, seed=7, pad_pitch=16, n_sinks=48),
+         "gates": lambda: C.gate_dag(2000, seed=4)}[name]()
+    p = X.host_pattern(c)
+    n = p["n"]
+    A = sp.csr_matrix((np.ones(len(p["colind"])), p["colind"], p["rowptr"]), shape=(n, n))
+    match = maximum_bipartite_matching(A, perm_type="column")
+    assert (match >= 0).all()
+    k, lab = connected_components(A[:, match], directed=True, connection="strong")
+    sizes = np.bincount(lab)
+    n_vsrc = len(c.vsrc)
+    assert k == 1 + 2 * n_vsrc and sizes.max() == n - 2 * n_vsrc
