@@ -118,3 +118,63 @@ def grid_charge_balance(c, o, r, tstep, tstop):
         allow += 1e-9 * abs(sq[k] - sq[k - 1]) + 4 * eps * abs(sq[k]) + 1e-300
         worst = max(worst, abs(d) / allow)
     return worst
+
+
+# Level-1 model cards of the device zoo: capacitances on (Meyer, overlap,
+# bottom and sidewall junctions), each depletion-charge form of
+# updateIntermediateVars: MJ = MJSW = 0.5 (the square-root path), MJ = MJSW
+# (one power), MJ != MJSW (two powers; the default MJSW 0.33)
+ZOO_CAPS = dict(TOX=2e-8, CJ=2e-4, CJSW=1e-9, CGSO=3e-10, CGDO=3e-10, CGBO=2e-10, GAMMA=0.4, LAMBDA=0.03)
+ZOO_CARDS = ({}, dict(MJ=0.5, MJSW=0.5), dict(MJ=0.4, MJSW=0.4), dict(MJ=0.45, MJSW=0.3))
+
+
+def device_zoo(n_per=25):
+    """NMOS and PMOS devices of every ZOO_CARDS card, each with its own four
+    terminal nodes (1 kOhm to ground each), so that a load at arbitrary node
+    voltages puts devices in every operating branch."""
+    from xyce_amd import circuits as C
+    c = C.Circuit()
+    cards = []
+    for extra in ZOO_CARDS:
+        for t, base in ((+1, C.NMOS_RING), (-1, C.PMOS_RING)):
+            cards.append(c.add_model(C.model_card(t, **base, **{**ZOO_CAPS, **extra})))
+    for m in cards:
+        for _ in range(n_per):
+            k = len(c.mos_model)
+            nd = [c.node(f"m{k}_{j}") for j in "dgsb"]
+            c.add_mos(nd[0], nd[1], nd[2], nd[3], m,
+                      C.inst_card(W=2e-6, L=0.5e-6, AD=4e-12, AS=4e-12, PD=8e-6, PS=8e-6))
+            for j in nd:
+                c.add_res(j, 0, 1e3)
+    return c
+
+
+def zoo_states(o, seed):
+    """A DC load at random node voltages supplies the store/state history;
+    returns (x, sta, sto) with x that history's voltages plus noise."""
+    rng = np.random.default_rng(seed)
+    xp = rng.uniform(-0.5, 2.3, size=o.n)
+    r0 = o.load(xp, np.zeros(o.nsta), np.zeros(o.nsto), dcop=1, tran=0, init_jct=0, newton_iter=1, order=1)
+    return xp + rng.normal(0.0, 0.3, size=o.n), r0["sta_next"].copy(), r0["sto_next"].copy()
+
+
+def branch_classes(c, mos_stasto, sto, tdep=0.4):
+    """Operating branches the devices of a load went through, read from the
+    store values it wrote (limited vbd, vbs, vgs, vds and von, as
+    updateIntermediateVars uses them): (type, mode, cutoff|sat|lin) and
+    (bs|bd, type, reverse|forward below FC*PB|forward beyond FC*PB)."""
+    out = set()
+    for d in range(len(c.mos_model)):
+        s0 = mos_stasto[d, 1]
+        vbd, vbs, vgs, vds, von_s = sto[s0:s0 + 5]
+        t = int(c.models[c.mos_model[d]][0])
+        mode = 1 if vds >= 0 else -1
+        vgst = (vgs if mode == 1 else vgs - vds) - t * von_s
+        out.add(("m", t, mode, "cutoff" if vgst <= 0 else ("sat" if vgst <= vds * mode else "lin")))
+        for nm, v in (("bs", vbs), ("bd", vbd)):
+            out.add((nm, t, "rev" if v <= 0 else ("fwd<fc" if v < tdep else "fwd>=fc")))
+    return out
+
+
+ALL_BRANCHES = ({("m", t, m, r) for t in (1, -1) for m in (1, -1) for r in ("cutoff", "sat", "lin")}
+                | {(j, t, b) for t in (1, -1) for j in ("bs", "bd") for b in ("rev", "fwd<fc", "fwd>=fc")})

@@ -105,6 +105,40 @@ def test_load_parity_with_caps(gpu, newton_iter, order):
         assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+@pytest.mark.parametrize("dcop,init_jct,newton_iter", [(1, 1, 0), (1, 0, 1), (0, 0, 0), (0, 0, 2)])
+def test_device_zoo_load_parity(gpu, seed, dcop, init_jct, newton_iter):
+    """200 independent devices over four model cards (every depletion-charge
+    form) at random terminal voltages: every operating branch of
+    updateIntermediateVars occurs (checked on the oracle's output, as in
+    test_oracle_devices.py::test_device_zoo_reaches_every_branch), and every
+    state, store, vector and matrix entry meets the 1e-12 bar."""
+    from tests.helpers import ALL_BRANCHES, branch_classes, device_zoo, zoo_states
+    c = device_zoo()
+    d = X.DeviceMgr(c)
+    o = Oracle(c)
+    x, sta0, sto0 = zoo_states(o, seed)
+    kw = dict(dcop=dcop, tran=1 - dcop, init_jct=init_jct, newton_iter=newton_iter, order=2, h=3e-12,
+              time=0.4e-9, tstep=1e-12, tstop=5e-9)
+    ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    if not init_jct:
+        assert ALL_BRANCHES <= branch_classes(c, o.pattern()["mos_stasto"], ref["sto_next"])
+    d.set_solver_state(**kw)
+    sta_n, sto_n, lim = d.updateState(x, sta0, sto0, sto_next=sto0)
+    assert lim == ref["limited"]
+    assert entry_tol(sta_n, ref["sta_next"], np.abs(ref["sta_next"]) + 1e-300) <= 1
+    assert entry_tol(sto_n, ref["sto_next"], np.abs(ref["sto_next"]) + 1e-300) <= 1
+    vec = d.loadDAEVectors()
+    for k in ("F", "Q", "B", "dFdxdVp", "dQdxdVp"):
+        a, b = vec[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1, k
+    dF, dQ = d.loadDAEMatrices()
+    for a, b in ((dF, ref["dFdx"]), (dQ, ref["dQdx"])):
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
+
+
 def test_dc_load_parity(gpu):
     c = C.ring(7, seed=9, mc_seed=9)
     d = X.DeviceMgr(c)

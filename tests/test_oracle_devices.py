@@ -360,3 +360,19 @@ def test_dc_continuation_stages_reach_the_newton_point(stage, make):
     assert st > 0 and o.dc_strategy() == stage
     assert np.max(np.abs(F - B)) <= 1e-8
     assert np.all(np.abs(x - x0) <= 1e-3 * np.abs(x0) + 1e-6)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_device_zoo_reaches_every_branch(seed):
+    """The inputs of the GPU's device-zoo load parity test put devices of
+    both types in both modes and all three regions, and both junctions
+    reverse, forward below FC*PB and forward beyond it, with limiting on."""
+    from tests.helpers import ALL_BRANCHES, branch_classes, device_zoo, zoo_states
+    c = device_zoo()
+    o = Oracle(c)
+    x, sta0, sto0 = zoo_states(o, seed)
+    for ni in (0, 1):
+        ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, dcop=0, tran=1, init_jct=0, newton_iter=ni,
+                     order=2, h=3e-12, time=0.4e-9, tstep=1e-12, tstop=5e-9)
+        assert ref["limited"] == 1
+        assert ALL_BRANCHES <= branch_classes(c, o.pattern()["mos_stasto"], ref["sto_next"])
