@@ -472,3 +472,50 @@ def test_config1_ring101_50ns_matches_oracle(gpu):
     assert abs(st["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2
     assert early <= 1e-3 * 1.8 + 1e-6
     assert ng == no and abs(pg - po) <= 1e-4 * po
+
+
+def test_mc4096_batch_runs_to_the_end_and_matches_oracle(gpu):
+    """The headline workload at full size: 4096 Monte Carlo rings (stages odd
+    U[11,201], per-device VTO and KP), .TRAN 0-20 ns, hmax 10 ps, in the
+    autotuned launch form that the benchmark times.  Every instance finishes.
+    Six of them (the smallest, the largest, the median and three more) run
+    alone give the same step and Newton counts and the same final solution
+    bit for bit (instances are independent), and against the oracle: step and
+    Newton counts within 2 %, node voltages within RELTOL |V| + ABSTOL over
+    the first 2 ns, the oscillation period within 1e-4 after 5 ns."""
+    cs = C.mc_ring_batch(4096, seed=3)
+    tstop, tstep, hmax = 20e-9, 1e-12, 10e-12
+    b = X.Batch(cs, tstop, tstep, hmax)
+    tune = b.autotune()
+    cnt = b.run_to_end(chunk=256)
+    print(f"4096 rings, {tune}: {cnt}")
+    assert cnt["done"] == 4096 and cnt["failed"] == 0, cnt
+    stages = np.array([(len(c.mos_model)) // 2 for c in cs])
+    order = np.argsort(stages, kind="stable")
+    sample = [int(order[0]), int(order[-1]), int(order[len(order) // 2]), 7, 1234, 4095]
+    for i in sample:
+        c = cs[i]
+        o = Oracle(c)
+        st_b, x_b = b.instance(i, o.n)
+        b1 = X.Batch([c], tstop, tstep, hmax, record_max=20000)
+        c1 = b1.run_to_end(chunk=1024)
+        assert c1["done"] == 1, c1
+        st_1, x_1 = b1.instance(0, o.n)
+        assert st_1["steps"] == st_b["steps"] and st_1["newton_iters"] == st_b["newton_iters"]
+        assert np.array_equal(x_1, x_b)
+        r = o.tran(tstop, tstep, hmax)
+        assert r["status"] == 0
+        t_g, x_g = b1.points(0, o.n)
+        assert t_g[-1] == pytest.approx(tstop, rel=1e-12)
+        eg, eo = t_g <= 2e-9, r["t"] <= 2e-9
+        early = waveform_error(t_g[eg], x_g[eg], r["t"][eo], r["x"][eo])
+        col = int(o.pattern()["node_gid"][2])
+        pg, ng = _period(t_g, x_g[:, col])
+        po, no = _period(r["t"], r["x"][:, col])
+        print(f"instance {i} ({stages[i]} stages): steps gpu {st_1['steps']:.0f} oracle {r['steps']}, Newton "
+              f"{st_1['newton_iters']:.0f} / {r['newton_iters']}; 0-2 ns max |dV| {early:.2e} V; period "
+              f"{pg * 1e9:.6f} / {po * 1e9:.6f} ns ({ng} / {no} crossings)")
+        assert abs(st_1["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
+        assert abs(st_1["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2
+        assert early <= 1e-3 * 1.8 + 1e-6
+        assert abs(ng - no) <= 1 and abs(pg - po) <= 1e-4 * po
