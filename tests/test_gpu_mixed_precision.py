@@ -130,12 +130,14 @@ def test_refinement_that_stalls_falls_back(gpu):
     np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-6)
 
 
-def test_gate_array_full_size_fp32_refined(gpu):
-    """250k INV/NAND2 gates (749,642 level-1 MOSFETs, 375,323 unknowns): the
+@pytest.mark.parametrize("frac_nand", [0.5, 1.0])
+def test_gate_array_full_size_fp32_refined(gpu, frac_nand):
+    """250k gates, half INV and half NAND2 (749,642 level-1 MOSFETs, 375,323
+    unknowns) or all NAND2 (1,000,000 MOSFETs, 500,502 unknowns): the
     transient Jacobian from the GPU device loads (checked entrywise against
     the oracle's), factored by the multifrontal path in FP32 with FP64
     refinement; error per node against SuperLU's FP64 solution."""
-    c = C.gate_dag(250_000, seed=4)
+    c = C.gate_dag(250_000, seed=4, frac_nand=frac_nand)
     o = Oracle(c)
     rng = np.random.default_rng(11)
     x = 0.9 + rng.normal(0.0, 0.05, size=o.n)
