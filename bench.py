@@ -3,11 +3,15 @@
 Workload: 4096 CMOS ring oscillators, stage counts odd and uniform on
 [11, 201], per-device Gaussian VTO (sigma 30 mV) and KP (sigma 5 %) from a
 per-instance seed, .TRAN 0-20 ns with a 10 ps maximum step, FP64 throughout.
-A "step" is one batched Newton iteration: one launch in which every active
-instance evaluates its MOSFETs, stamps, refactors and solves (or, when its
-previous update converged, runs its step control).  ``value`` counts the
-Newton iterations (eval + stamp + LU + solve) actually completed per second
-over the timed launches, all instances together, inputs resident in HBM.
+A "step" is one launch of the batch: K batched Newton iterations back to back
+(K chosen by the autotuner), in each of which every active instance evaluates
+its MOSFETs, stamps, refactors and solves (or, when its previous update
+converged, runs its step control).  ``value`` counts the Newton iterations
+(eval + stamp + LU + solve) actually completed per second over the timed
+launches, all instances together, inputs resident in HBM.  The default
+warm-up and timed launches cover the same stretch of the run (Newton
+iterations 4800 to 11200 of every instance, all of them still integrating)
+whatever K is.
 
 Also reported: the same engine on one instance of the 101-stage ring
 (single-instance FP64 GPU path) and the ratio, the roofline of the batched
@@ -68,8 +72,11 @@ def cpu_baseline(circuits_list, sample, tstop, tstep, hmax, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=300)
+    # defaults: the same stretch of the run (Newton iterations 4800-11200 of
+    # every instance, where every instance is still integrating) whatever
+    # iterations per launch the autotuner picks
+    ap.add_argument("--steps", type=int, default=None, help="timed launches (default 6400 // iterations per launch)")
+    ap.add_argument("--warmup", type=int, default=None, help="warm-up launches (default 4800 // iterations per launch)")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=512)
@@ -132,12 +139,14 @@ def main():
         k_iter = tune["iters_per_launch"]
     else:
         k_iter = 1 if args.form == "split" else args.iters
-    b.advance(args.warmup, graph_chunk=50)
+    steps = args.steps if args.steps is not None else max(1, 6400 // k_iter)
+    warmup = args.warmup if args.warmup is not None else max(1, 4800 // k_iter)
+    b.advance(warmup, graph_chunk=50)
     c0 = b.counters()
     torch.cuda.synchronize()
     w0 = time.perf_counter()
-    chunk = 50 if args.steps % 50 == 0 else 0
-    b.advance(args.steps, graph_chunk=chunk)
+    chunk = 50 if steps % 50 == 0 else 0
+    b.advance(steps, graph_chunk=chunk)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
     c1 = b.counters()
@@ -145,12 +154,12 @@ def main():
     iters = c1["newton_iters"] - c0["newton_iters"]
     loads = c1["loads"] - c0["loads"]
     value = iters / wall
-    log(f"timed {args.steps} launches: wall {wall*1e3:.2f} ms, events {dev_s*1e3:.2f} ms, "
+    log(f"timed {steps} launches: wall {wall*1e3:.2f} ms, events {dev_s*1e3:.2f} ms, "
         f"{iters:.0f} Newton iterations, {loads:.0f} loads, done {c1['done']}, failed {c1['failed']}, "
         f"dense {c1['dense_solves'] - c0['dense_solves']:.0f}")
 
     if args.profile:
-        print(json.dumps({"profile_run": True, "launches": args.steps, "newton_iterations": int(iters),
+        print(json.dumps({"profile_run": True, "launches": steps, "newton_iterations": int(iters),
                           "events_ms": round(dev_s * 1e3, 3), "tune": tune}), flush=True)
         return
 
@@ -160,20 +169,24 @@ def main():
     b1.config(iters_per_launch=args.iters, split_eval=0)
     tune1 = b1.autotune()
     log(f"single-instance autotune: {tune1}")
-    b1.advance(args.warmup, graph_chunk=50)
+    k1 = tune1["iters_per_launch"]
+    steps1 = args.steps if args.steps is not None else max(1, 6400 // k1)
+    warmup1 = args.warmup if args.warmup is not None else max(1, 4800 // k1)
+    chunk1 = 50 if steps1 % 50 == 0 else 0
+    b1.advance(warmup1, graph_chunk=50)
     s0 = b1.counters()
     torch.cuda.synchronize()
     w1 = time.perf_counter()
-    b1.advance(args.steps, graph_chunk=chunk)
+    b1.advance(steps1, graph_chunk=chunk1)
     torch.cuda.synchronize()
     wall1 = time.perf_counter() - w1
     s1 = b1.counters()
     single = (s1["newton_iters"] - s0["newton_iters"]) / wall1
-    log(f"single instance: {single:.1f} Newton steps/s ({wall1 / args.steps * 1e6:.1f} us/launch)")
+    log(f"single instance: {single:.1f} Newton steps/s ({wall1 / steps1 * 1e6:.1f} us/launch)")
 
     byts = algorithmic_bytes(ckts) * k_iter
-    us_launch = dev_s / args.steps * 1e6
-    achieved = byts / (dev_s / args.steps) / 1e9
+    us_launch = dev_s / steps * 1e6
+    achieved = byts / (dev_s / steps) / 1e9
     traffic = None
     tfile = os.path.join(HERE, "profiles", "traffic_k_batch_step.json")
     if os.path.exists(tfile):
@@ -205,9 +218,9 @@ def main():
         "value": round(value, 1),
         "unit": "Newton steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(wall / steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

@@ -2074,7 +2074,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   // iteration wins.  Split form: workgroup size x instance grouping.  Fused
   // form: workgroup size x grouping with 8 iterations per launch, then, with
   // LDS-sized groups, each group's own workgroup size, the iterations per
-  // launch (4, 8, 16) and whether the groups run side by side.
+  // launch (4, 8, 16, 32) and whether the groups run side by side.
   const int blocks[3] = {128, 256, 512};
   auto time_launches = [&](int warm, int timed, double* us) -> int {
     int st = xg_batch_advance(b, warm, 0);
@@ -2113,7 +2113,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     double best_total = 1e30;
     int best_kk = 8;
     std::vector<int> best_blocks(b->buckets.size(), best_fused.block), best_terms(b->buckets.size(), 0);
-    for (int kk : {4, 8, 16}) {
+    for (int kk : {4, 8, 16, 32}) {
       b->args.iters = kk;
       double total = 0;
       std::vector<int> bb(b->buckets.size(), best_fused.block), bt(b->buckets.size(), 0);
