@@ -141,8 +141,10 @@ int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* va
 int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double* values, int method, xg_lu** out);
 int xg_lu_destroy(xg_lu* lu);
 /* Numeric refactor with the symbolic pivot order (KLU NumericFactorization,
- * :363). Returns >0 (pivot+1) if a pivot is zero/non-finite or fails the
- * threshold test; the caller then re-runs xg_lu_symbolic with values. */
+ * :363).  A pivot that is zero, non-finite or fails the threshold test makes
+ * it choose new pivots from these values and refactor, as Xyce's KLU
+ * re-pivots at every factorisation (repivot_, N_LAS_AmesosSolver.C:87,318);
+ * returns >0 (pivot+1) only if that fails too (a singular matrix). */
 int xg_lu_numeric(xg_lu* lu, const double* vals);
 /* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);

@@ -186,6 +186,30 @@ def test_lu_singular_returns_status(gpu):
     assert st > 0 and np.all(x == 0.0)
 
 
+@pytest.mark.parametrize("method", [1, 2])
+def test_lu_zero_pivot_value_repivots(gpu, method):
+    """A nonsingular matrix with a [[0, 1], [1, 0]] block whose zeros are
+    stored: the pivot order chosen from the pattern alone puts a zero on the
+    diagonal (checked on the host analysis), so xg_lu_numeric re-chooses the
+    pivots from the values, as KLU re-pivots at every factorisation, and the
+    solve succeeds; so does a refactor with new values."""
+    rp = np.array([0, 2, 4, 5], np.int32)
+    ci = np.array([0, 1, 0, 1, 2], np.int32)
+    A = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    h = X.host_lu_analyse(3, rp, ci)
+    assert np.any(np.diag(A[h["perm_row"]][:, h["perm_col"]]) == 0.0)
+    vals = lambda M: np.array([M[r, c] for r in range(3) for c in ci[rp[r]:rp[r + 1]]])
+    s = X.DirectSolver(3, rp, ci, method=method)
+    b = np.array([1.0, 2.0, 3.0])
+    st, x = s.doSolve(vals(A), b)
+    assert st == 0
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-13)
+    A2 = A + np.diag([0.0, 1e-9, 0.5])
+    st, x = s.doSolve(vals(A2), b)
+    assert st == 0
+    np.testing.assert_allclose(x, np.linalg.solve(A2, b), rtol=1e-12)
+
+
 @pytest.mark.parametrize("stages,tstop", [(5, 2e-9), (11, 2e-9)])
 def test_batch_single_transient_matches_oracle(gpu, stages, tstop):
     c = C.ring(stages, seed=stages)

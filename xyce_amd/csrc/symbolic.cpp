@@ -11,9 +11,9 @@
 // 10*sqrt(n) are ordered last, as AMD does with dense rows.
 //
 // Pivots are static (the diagonal of the permuted matrix); the numeric kernels
-// check every pivot and the caller re-analyses with values when one fails,
-// which stands in for KLU's re-pivoting refactor (KLU_REPIVOT=1,
-// N_LAS_AmesosSolver.C:87,318).
+// check every pivot and xg_lu_numeric re-analyses with the values when one
+// fails, which stands in for KLU's re-pivoting at every factorisation
+// (repivot_, N_LAS_AmesosSolver.C:87,318).
 #include <algorithm>
 #include <cmath>
 #include <deque>
