@@ -34,7 +34,8 @@ void mos1_matrices_ext(const Mos1& mi, const SolverState& ss, double* dFdx, doub
 // threshold partial pivoting (KLU's default tol 0.001, the diagonal preferred
 // when it passes).  The symbolic order is computed once per pattern and
 // reused, as Xyce reuses KLU's analysis.  Parity is on the solution (pinned
-// against numpy / scipy in the tests).
+// against numpy / scipy in the tests); KLU's own pivot sequence and factor
+// bits: parity unpinned (KLU, in Trilinos Amesos / SuiteSparse, is absent).
 struct SparseLU {
   int n = 0;
   std::vector<int> q;          // column order
@@ -520,7 +521,9 @@ static double wrms(const double* v, const double* wt, int n) {
 // aggressiveness a = 0.01 for gmin and 1 for source stepping; source stepping
 // disables initJct (setDisableInitJctFlags).  LOCA is
 // third-party (Trilinos); this restates its published natural-continuation
-// scheme with the reference's settings.
+// scheme with the reference's settings.  The continuation and iteration path:
+// parity unpinned; the operating point it reaches is pinned (gate logic levels,
+// the DC residual, and each stage reaching plain Newton's point).
 template <class SetParam>
 static int dc_continuation(Work& w, long* iters, double p0, double p1, double step, double min_step, double max_step,
                            double aggressiveness, bool init_jct, SetParam set_param) {
