@@ -9,6 +9,8 @@ Bars:
     (RELTOL 1e-3, ABSTOL 1e-6, plus the interpolation error between the two
     step sequences, bounded by the LTE tolerance).
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -432,10 +434,13 @@ def test_gate_array_loads_full_size(gpu):
     nt_v = np.bincount(p["mos_lid"].ravel()[p["mos_lid"].ravel() < o.n], minlength=o.n) + np.diff(p["rowptr"])
     d = X.DeviceMgr(c)
     d.set_solver_state(**kw)
-    runs = []
+    runs, walls = [], []
     for _ in range(3):
+        t0 = time.perf_counter()
         d.updateState(x, sta0, sto0)
         runs.append((d.loadDAEVectors(), d.loadDAEMatrices()))
+        walls.append(time.perf_counter() - t0)
+    print(f"one full load through host arrays (PCIe copies included): {1e3 * min(walls):.1f} ms")
     vec, (dF, dQ) = runs[0]
     for k in ("F", "Q", "dFdxdVp", "dQdxdVp"):
         a, b = vec[k], ref[k]
