@@ -1480,7 +1480,10 @@ struct xg_batch {
         buckets.push_back(bk);
       }
     } else {
-      buckets.push_back(Bucket{0, n_inst, args.max_n, args.max_nlu, shmem, 0});
+      Bucket bk{0, n_inst, args.max_n, args.max_nlu, shmem, 0};
+      for (const ClassDesc& C : classes) { bk.blob = std::max(bk.blob, C.blob_len); bk.nterm = std::max(bk.nterm, C.nterm); }
+      bk.shmem_terms = lds_bytes(bk.n, bk.nlu, ((bk.blob + 3) & ~3)) + sizeof(double) * (size_t)bk.nterm;
+      buckets.push_back(bk);
     }
     (void)hipMemcpy(d_order.p, ord.data(), ord.size() * sizeof(int), hipMemcpyHostToDevice);
     order_mode = mode;
