@@ -5,7 +5,8 @@
   method and with FP32 factors;
 * bad arguments are rejected with XG_E_ARG before any device work
   (transpose solves, which the transient path never asks for; empty batches;
-  an instance whose class does not fit a workgroup's LDS);
+  an instance whose class does not fit a workgroup's LDS; launch
+  configurations out of range);
 * an instance that fails (its step size collapses) stops on its own: the
   other instances of the batch follow exactly the trajectory they follow
   alone.
@@ -78,6 +79,12 @@ def test_bad_arguments_are_rejected(gpu):
         X.Batch([], 1e-9, 1e-12, 1e-11)
     with pytest.raises(X.XgError):
         X.Batch([C.gate_dag(250, seed=3)], 1e-9, 1e-12, 1e-11)
+    # launch configuration out of range
+    bt = X.Batch([C.ring(5, seed=1)], 1e-10, 1e-12, 1e-11)
+    assert L.xg_batch_config(bt.h, 96, 0, -1, -1) == -1
+    assert L.xg_batch_config(bt.h, 0, 2000, -1, -1) == -1
+    assert L.xg_batch_config(bt.h, 0, 0, 3, -1) == -1
+    assert L.xg_batch_config(bt.h, 0, 8, 2, 0) == 0
 
 
 def test_failing_instance_does_not_disturb_the_others(gpu):

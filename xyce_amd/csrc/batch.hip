@@ -2219,8 +2219,8 @@ int xg_batch_dc_stage(xg_batch* b, int inst, int* stage) {
 }
 
 int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups) {
-  if (!b || iters_per_launch < 0 || split_eval < -1 || split_eval > 2 || concurrent_groups < -1 ||
-      concurrent_groups > 1)
+  if (!b || iters_per_launch < 0 || iters_per_launch > 1024 || split_eval < -1 || split_eval > 2 ||
+      concurrent_groups < -1 || concurrent_groups > 1)
     return XG_E_ARG;
   if (concurrent_groups >= 0) b->concurrent_groups = concurrent_groups == 1;
   if (block) {
