@@ -8,6 +8,8 @@
   rules (jacStamp with RD=RS=0 merged, Vsrc branch row, ground dropped);
 * the symbolic LU: valid permutations, zero-free diagonal, fill that equals
   a brute-force boolean elimination; a malformed pattern is rejected;
+* a malformed circuit description (model, device or waveform index out of
+  range, a device listed twice, a wrong count) is rejected with XG_E_SETUP;
 * the patterns are one strongly connected block apart from each voltage
   source's two 1x1 blocks, so the block split of KLU's BTF would not change
   the ordering.
@@ -174,3 +176,35 @@ def test_patterns_are_one_strongly_connected_block(name):
     sizes = np.bincount(lab)
     n_vsrc = len(c.vsrc)
     assert k == 1 + 2 * n_vsrc and sizes.max() == n - 2 * n_vsrc
+
+
+@pytest.mark.parametrize("corrupt", ["model_index", "device_twice", "device_index", "wave_index", "pulse_params",
+                                     "n_dev", "node_id"])
+def test_malformed_description_is_rejected(corrupt):
+    """A description with an inconsistent count or index is rejected with
+    XG_E_SETUP before anything is indexed with it."""
+    import ctypes
+    c = C.inverter_chain(2)
+    d, a = X.make_desc(c)
+    sizes = np.zeros(6, np.int32)
+    call = lambda: X.lib().xg_host_pattern(ctypes.byref(d), sizes.ctypes.data_as(ctypes.c_void_p), None, None,
+                                           None, None, None, None, None)
+    assert call() == 0
+    if corrupt == "model_index":
+        a["mos_model"][1] = 7
+    elif corrupt == "device_twice":
+        a["dev_idx"][1] = a["dev_idx"][0]
+        a["dev_type"][1] = a["dev_type"][0]
+    elif corrupt == "device_index":
+        a["dev_idx"][0] = 1000
+    elif corrupt == "wave_index":
+        a["vsrc_wave"][0] = 5
+    elif corrupt == "pulse_params":
+        k = int(np.nonzero(a["wave_type"] == C.W_PULSE)[0][0])
+        d.n_wave = k + 1
+        a["wave_off"][k + 1] = a["wave_off"][k] + 5
+    elif corrupt == "n_dev":
+        d.n_dev = d.n_dev - 1
+    elif corrupt == "node_id":
+        a["mos_nodes"][0] = d.n_nodes + 3
+    assert call() == -3

@@ -201,8 +201,53 @@ void wave_breakpoints(const HostCircuit& hc, int w, double tstep, double tstop, 
   }
 }
 
+// Rejects a description whose counts, device indices, model or waveform
+// indices or waveform parameter counts are inconsistent, before anything is
+// indexed with them (node ids are checked where the node graph is built).
+static void validate_desc(const xg_circuit_desc* d) {
+  auto bad = [](const char* what) { throw std::runtime_error(std::string("invalid circuit description: ") + what); };
+  if (d->n_nodes < 1 || d->n_dev < 0 || d->n_models < 0 || d->n_mos < 0 || d->n_res < 0 || d->n_cap < 0 ||
+      d->n_vsrc < 0 || d->n_isrc < 0 || d->n_wave < 0)
+    bad("negative count");
+  auto need = [&](const void* ptr, int n, const char* what) { if (n > 0 && !ptr) bad(what); };
+  need(d->dev_type, d->n_dev, "dev_type"); need(d->dev_idx, d->n_dev, "dev_idx");
+  need(d->model_par, d->n_models, "model_par");
+  need(d->mos_nodes, d->n_mos, "mos_nodes"); need(d->mos_model, d->n_mos, "mos_model");
+  need(d->mos_inst, d->n_mos, "mos_inst");
+  need(d->res_nodes, d->n_res, "res_nodes"); need(d->res_val, d->n_res, "res_val");
+  need(d->cap_nodes, d->n_cap, "cap_nodes"); need(d->cap_val, d->n_cap, "cap_val");
+  need(d->vsrc_nodes, d->n_vsrc, "vsrc_nodes"); need(d->vsrc_wave, d->n_vsrc, "vsrc_wave");
+  need(d->isrc_nodes, d->n_isrc, "isrc_nodes"); need(d->isrc_wave, d->n_isrc, "isrc_wave");
+  need(d->wave_type, d->n_wave, "wave_type"); need(d->wave_off, d->n_wave, "wave_off");
+  need(d->wave_data, d->n_wave, "wave_data");
+  const int cnt[5] = {d->n_mos, d->n_res, d->n_cap, d->n_vsrc, d->n_isrc};
+  if (d->n_dev != cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4]) bad("n_dev is not the sum of the device counts");
+  std::vector<char> seen[5];
+  for (int t = 0; t < 5; ++t) seen[t].assign(cnt[t], 0);
+  for (int o = 0; o < d->n_dev; ++o) {
+    const int t = d->dev_type[o], i = d->dev_idx[o];
+    if (t < 0 || t > 4) bad("device type");
+    if (i < 0 || i >= cnt[t] || seen[t][i]) bad("device order must list every device once");
+    seen[t][i] = 1;
+  }
+  for (int i = 0; i < d->n_mos; ++i)
+    if (d->mos_model[i] < 0 || d->mos_model[i] >= d->n_models) bad("MOSFET model index");
+  if (d->n_wave > 0 && d->wave_off[0] != 0) bad("wave_off[0] must be 0");
+  for (int w = 0; w < d->n_wave; ++w) {
+    const int np = d->wave_off[w + 1] - d->wave_off[w], t = d->wave_type[w];
+    const bool ok = (t == XG_WAVE_DC && np >= 1) || (t == XG_WAVE_PULSE && np == 7) ||
+                    (t == XG_WAVE_PWL && np >= 2 && np % 2 == 0);
+    if (!ok) bad("waveform type or parameter count (DC >= 1, PULSE 7, PWL time/value pairs)");
+  }
+  for (int i = 0; i < d->n_vsrc; ++i)
+    if (d->vsrc_wave[i] < 0 || d->vsrc_wave[i] >= d->n_wave) bad("voltage source waveform index");
+  for (int i = 0; i < d->n_isrc; ++i)
+    if (d->isrc_wave[i] < 0 || d->isrc_wave[i] >= d->n_wave) bad("current source waveform index");
+}
+
 int build_host_circuit(const xg_circuit_desc* d, HostCircuit& hc) {
   try {
+    validate_desc(d);
     hc.n_nodes = d->n_nodes;
     hc.n_mos = d->n_mos;
     hc.gmin = d->gmin > 0 ? d->gmin : 1e-12;
