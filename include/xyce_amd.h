@@ -110,7 +110,8 @@ int xg_set_solver_state(xg_circuit* c, const xg_solver_state* ss);
 
 /* a2: DeviceMgr::updateState -> MOSFET1 Master::updateState (N_DEV_MOSFET1.C:4241).
  * Host arrays in Xyce layout: x[n], state/store vectors as laid out by
- * xg_circuit_pattern's mos_stasto.  *limited = !allDevicesConverged. */
+ * xg_circuit_pattern's mos_stasto (NULL only for a circuit without MOSFETs).
+ * *limited = !allDevicesConverged (limited may be NULL). */
 int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
                     double* sta_next, double* sto_next, int* limited);
 
@@ -185,7 +186,8 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk);
 int xg_batch_counters(xg_batch* b, double* counters);
 /* Per-instance state: time, h, order, mode (0 DCOP, 1 tran, 2 done, <0 failed). */
 int xg_batch_instance(xg_batch* b, int inst, double* tstate, double* x);
-/* Recorded accepted points of one instance: t[max], x[max*n]; returns count. */
+/* Recorded accepted points of one instance: t[max], x[max*n] (both
+ * required, max >= 0); returns count. */
 int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x);
 /* Launch form.  block: workgroup size (64/128/256/512, 0 = keep).
  * split_eval 1 (default): each launch runs a device-evaluation kernel over

@@ -85,6 +85,12 @@ def test_bad_arguments_are_rejected(gpu):
     assert L.xg_batch_config(bt.h, 0, 2000, -1, -1) == -1
     assert L.xg_batch_config(bt.h, 0, 0, 3, -1) == -1
     assert L.xg_batch_config(bt.h, 0, 8, 2, 0) == 0
+    t = np.zeros(4); xx = np.zeros(64)
+    assert L.xg_batch_points(bt.h, 0, 4, t.ctypes.data_as(P), xx.ctypes.data_as(P)) == -1   # no recording
+    # device loads of a circuit with MOSFETs need its state and store arrays
+    dm = X.DeviceMgr(C.ring(3, seed=1))
+    xs = np.zeros(dm.n)
+    assert L.xg_update_state(dm.h, xs.ctypes.data_as(P), None, None, None, None, None) == -1
 
 
 def test_failing_instance_does_not_disturb_the_others(gpu):

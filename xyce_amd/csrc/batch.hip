@@ -2057,7 +2057,7 @@ int xg_batch_instance(xg_batch* b, int inst, double* ts, double* x) {
 }
 
 int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x) {
-  if (!b || inst < 0 || inst >= b->n_inst || b->args.rec_max == 0) return XG_E_ARG;
+  if (!b || inst < 0 || inst >= b->n_inst || b->args.rec_max == 0 || max_points < 0 || !t || !x) return XG_E_ARG;
   InstState s;
   HIPCHK(hipMemcpy(&s, b->d_st.p + inst, sizeof(s), hipMemcpyDeviceToHost));
   const int cnt = (int)std::min<long long>(s.n_rec, std::min(max_points, b->args.rec_max));

@@ -927,6 +927,7 @@ int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const
   HostCircuit& h = c->hc;
   hipStream_t s = c->stream;
   const int N = h.n_mos;
+  if (N > 0 && (!sta_cur || !sto_cur || !sta_next || !sto_next)) return XG_E_ARG;
   HIPCHK(hipMemcpyAsync(c->x.p, x, sizeof(double) * h.n, hipMemcpyHostToDevice, s));
   k_fill<<<1, 1, 0, s>>>(c->x.p + h.n, 1, 0.0);
   if (N > 0) {
