@@ -172,9 +172,11 @@ int xg_lu_precision_stats(const xg_lu* lu, double* stats);
 
 /* Builds a batch of independent circuits (each evaluated, stamped, factored
  * and solved by one workgroup), one launch per Newton iteration of every
- * instance, with per-instance DCOP + OneStep step control on the GPU. */
+ * instance, with per-instance DCOP + OneStep step control on the GPU.
+ * .TRAN 0 tstop with step tstep (both > 0) and maximum step hmax (0: none);
+ * record_max >= 0 accepted points recorded per instance (0: none). */
 int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, double tstep, double hmax,
-                    int record_stride, xg_batch** out);
+                    int record_max, xg_batch** out);
 int xg_batch_destroy(xg_batch* b);
 /* Launches `launches` batched Newton iterations (captured in a hipGraph of
  * `graph_chunk` launches when graph_chunk > 0).  Returns when done. */

@@ -208,3 +208,21 @@ def test_malformed_description_is_rejected(corrupt):
     elif corrupt == "node_id":
         a["mos_nodes"][0] = d.n_nodes + 3
     assert call() == -3
+
+
+@pytest.mark.parametrize("tstop,tstep,hmax,rec", [(0.0, 1e-12, 1e-11, 0), (1e-9, 0.0, 1e-11, 0),
+                                                  (1e-9, 1e-12, -1.0, 0), (1e-9, 1e-12, 1e-11, -5),
+                                                  (float("nan"), 1e-12, 1e-11, 0), (1e-9, float("inf"), 0.0, 0)])
+def test_batch_create_rejects_bad_arguments(tstop, tstep, hmax, rec):
+    """Out-of-range analysis arguments give XG_E_ARG before any device work
+    (so also without a GPU); valid ones get past the check."""
+    import ctypes
+    d, keep = X.make_desc(C.ring(3, seed=1))
+    h = ctypes.c_void_p()
+    L = X.lib()
+    dbl = ctypes.c_double
+    assert L.xg_batch_create(1, ctypes.byref(d), dbl(tstop), dbl(tstep), dbl(hmax), rec, ctypes.byref(h)) == -1
+    st = L.xg_batch_create(1, ctypes.byref(d), dbl(1e-9), dbl(1e-12), dbl(0.0), 0, ctypes.byref(h))
+    assert st != -1
+    if st == 0:
+        L.xg_batch_destroy(h)

@@ -1534,7 +1534,9 @@ extern "C" {
 
 int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, double tstep, double hmax,
                     int record_max, xg_batch** out) {
-  if (n_inst <= 0 || !descs || !out) return XG_E_ARG;
+  if (n_inst <= 0 || !descs || !out || !(tstop > 0.0) || !std::isfinite(tstop) || !(tstep > 0.0) ||
+      !std::isfinite(tstep) || !(hmax >= 0.0) || !std::isfinite(hmax) || record_max < 0)
+    return XG_E_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return XG_E_NOGPU;
   auto b = std::make_unique<xg_batch>();
