@@ -237,7 +237,7 @@ int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
 /* Autotuner: times the launch forms (split / fused with several iterations
  * per launch) x workgroup sizes x instance grouping on the real batch and
  * keeps the fastest; for the fused form with LDS-sized groups it then picks
- * each group's workgroup size and the iterations per launch (4, 8, 16, 32).
+ * each group's workgroup size and the iterations per launch (4, 8, 16, 32, 64).
  * result[0..9] = block size, grouping (1 = by LDS need), us per batched
  * iteration, split_eval, iterations per launch, workgroup size of groups
  * 0-3, groups side by side (1) or one after another (0). */

@@ -283,6 +283,25 @@ def test_launch_forms_do_not_change_results(gpu):
             assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
 
 
+def test_workgroup_size_does_not_change_results(gpu):
+    """Instances of more than 128 unknowns (151- and 201-stage rings) give
+    bit-identical trajectories with workgroups of 128, 256 and 512 threads,
+    fused or not: the residual norm and the local error estimate are summed
+    in an order that does not depend on the workgroup size."""
+    cs = [C.ring(s, seed=40 + s, mc_seed=40 + s) for s in (151, 201, 61)]
+    ns = [X.host_pattern(c)["n"] for c in cs]
+    out = []
+    for block, split, k in ((128, 0, 4), (256, 0, 4), (512, 0, 4), (256, 1, 1), (512, 2, 4)):
+        b = X.Batch(cs, 0.3e-9, 1e-12, 1e-11, record_max=2000)
+        b.config(block=block, iters_per_launch=k, split_eval=split)
+        cnt = b.run_to_end(chunk=64)
+        assert cnt["done"] == len(cs), cnt
+        out.append(tuple(b.points(i, ns[i]) for i in range(len(cs))))
+    for other in out[1:]:
+        for a, c in zip(out[0], other):
+            assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+
+
 def test_graph_replays_match_eager_launches(gpu):
     """Launches captured in a hipGraph and replayed (with counter reads and
     synchronisation between replays) give the same trajectory as eager

@@ -173,17 +173,32 @@ constexpr int kMaxLong = 16;     // per class
 constexpr int kLongOps = 16;     // LU targets / solve rows this long are reduced by a wavefront
 constexpr int kMaxSrc = 64;      // sources per instance (their values are staged in LDS)
 
-// Block reductions: 64-lane wave shuffles, then one LDS exchange across waves.
-__device__ __forceinline__ double block_sum(double v, double* red) {
+// Sum over rows 0..n-1 whose value does not depend on the workgroup size:
+// each 64-row chunk is summed by one wavefront with a fixed shuffle tree
+// (lane = row mod 64), the chunk sums are added in chunk order.  Callers loop
+// `for (base = 64 * wave; base < n; base += blockDim.x)` (row = base + lane,
+// the same trip count for the whole wavefront), sum each chunk with
+// wave_tree_sum, store it in chs[base / 64] from lane 0 and then call
+// chunk_total.
+constexpr int kMaxChunks = 128;          // rows per instance <= 64 * kMaxChunks
+__device__ __forceinline__ double wave_tree_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  if (lane == 0) red[w] = v;
+  return v;
+}
+__device__ __forceinline__ double chunk_total(const double* chs, int n, double* red) {
   __syncthreads();
-  double r = 0.0;
-  for (int i = 0; i < nw; ++i) r += red[i];
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int c = 0; c < (n + 63) / 64; ++c) r += chs[c];
+    red[0] = r;
+  }
+  __syncthreads();
+  const double r = red[0];
   __syncthreads();
   return r;
 }
+// Block maximum: 64-lane wave shuffles, then one LDS exchange across waves
+// (a maximum does not depend on the order).
 __device__ __forceinline__ double block_max(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -675,6 +690,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   // long term groups (e.g. a supply node shared by every PMOS): one wavefront
   // each, lane-strided loads and a fixed shuffle tree (deterministic)
   __shared__ double lsum[kMaxLong];
+  __shared__ double chs[kMaxChunks];     // per-64-row chunk sums (workgroup-size independent totals)
   {
     const int nl = C.n_vlong + C.n_mlong;
     const int* lg = ci + C.o_long;
@@ -702,7 +718,10 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     const unsigned short* bgp = prog + C.b_bgp; const unsigned short* bge = prog + C.b_bge;
     const unsigned short* lrp = prog + C.b_lrp; const unsigned short* lk = prog + C.b_lk;
     const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
-    for (int r = tid; r < n; r += nth) {
+    for (int base = tid & ~63; base < n; base += nth) {
+      const int r = base + (tid & 63);
+      double res2 = 0.0;
+      if (r < n) {
       double acc[4];
       for (int v = 0; v < 4; ++v) {
         const int beg = vptr[4 * r + v], end = vptr[4 * r + v + 1];
@@ -736,12 +755,15 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
                              ? residual_gear(S.order, h, S.a0, S.a1, S.a2, f, q, b, fl, ql, A.qH0[off + r], A.qH1[off + r])
                              : residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
       rhs[r] = (dc && S.gcond != 0.0 && vrow[r]) ? res - S.gcond * xs[r] : res;
-      n2 += res * res;
+      res2 = res * res;
       ninf = fmax(ninf, fabs(res));
+      }
+      const double cs = wave_tree_sum(res2);
+      if ((tid & 63) == 0) chs[base >> 6] = cs;
     }
   }
   XG_PHASE(2);
-  n2 = block_sum(n2, red);
+  n2 = chunk_total(chs, n, red);
   ninf = block_max(ninf, red);
   XG_PHASE(3);
   // ---- 4. Newton bookkeeping (DampedNewton::solve / converged_)
@@ -1049,12 +1071,17 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   // transient: local error test (StepErrorControl::evaluateStepError)
   bool ok = status > 0;
   if (ok && !S.beginning) {
-    double s2 = 0.0;
-    for (int i = tid; i < n; i += nth) {
-      const double q = (xs[i] - A.xn0[off + i]) / S.errwt;
-      s2 += q * q;
+    for (int base = tid & ~63; base < n; base += nth) {
+      const int i = base + (tid & 63);
+      double q2 = 0.0;
+      if (i < n) {
+        const double q = (xs[i] - A.xn0[off + i]) / S.errwt;
+        q2 = q * q;
+      }
+      const double cs = wave_tree_sum(q2);
+      if ((tid & 63) == 0) chs[base >> 6] = cs;
     }
-    s2 = block_sum(s2, red);
+    const double s2 = chunk_total(chs, n, red);
     if (tid == 0) { S.est = S.ck * sqrt(s2 / n); sh_int[3] = S.est <= 1.0; }
     __syncthreads();
     ok = sh_int[3];
@@ -1891,7 +1918,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
   for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu, C.blob_len));
   for (const InstDesc& D : b->insts) b->inst_cls.push_back(D.cls);
-  if (b->shmem > kMaxLdsBytes) return XG_E_ARG;
+  if (b->shmem > kMaxLdsBytes || max_n > 64 * kMaxChunks) return XG_E_ARG;
   b->set_order(1);
   *out = b.release();
   return XG_OK;
@@ -2079,7 +2106,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   // iteration wins.  Split form: workgroup size x instance grouping.  Fused
   // form: workgroup size x grouping with 8 iterations per launch, then, with
   // LDS-sized groups, each group's own workgroup size, the iterations per
-  // launch (4, 8, 16, 32) and whether the groups run side by side.
+  // launch (4, 8, 16, 32, 64) and whether the groups run side by side.
   const int blocks[3] = {128, 256, 512};
   auto time_launches = [&](int warm, int timed, double* us) -> int {
     int st = xg_batch_advance(b, warm, 0);
@@ -2118,7 +2145,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
     double best_total = 1e30;
     int best_kk = 8;
     std::vector<int> best_blocks(b->buckets.size(), best_fused.block), best_terms(b->buckets.size(), 0);
-    for (int kk : {4, 8, 16, 32}) {
+    for (int kk : {4, 8, 16, 32, 64}) {
       b->args.iters = kk;
       double total = 0;
       std::vector<int> bb(b->buckets.size(), best_fused.block), bt(b->buckets.size(), 0);
