@@ -185,16 +185,12 @@ __device__ __forceinline__ double wave_tree_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
-__device__ __forceinline__ double chunk_total(const double* chs, int n, double* red) {
+// Every thread adds the chunk sums itself, in the same order, so one barrier
+// suffices; chs is written again only after later phases' barriers.
+__device__ __forceinline__ double chunk_total(const double* chs, int n) {
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int c = 0; c < (n + 63) / 64; ++c) r += chs[c];
-    red[0] = r;
-  }
-  __syncthreads();
-  const double r = red[0];
-  __syncthreads();
+  double r = 0.0;
+  for (int c = 0; c < (n + 63) / 64; ++c) r += chs[c];
   return r;
 }
 // Block maximum: 64-lane wave shuffles, then one LDS exchange across waves
@@ -763,7 +759,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }
   }
   XG_PHASE(2);
-  n2 = chunk_total(chs, n, red);
+  n2 = chunk_total(chs, n);
   ninf = block_max(ninf, red);
   XG_PHASE(3);
   // ---- 4. Newton bookkeeping (DampedNewton::solve / converged_)
@@ -1081,7 +1077,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const double cs = wave_tree_sum(q2);
       if ((tid & 63) == 0) chs[base >> 6] = cs;
     }
-    const double s2 = chunk_total(chs, n, red);
+    const double s2 = chunk_total(chs, n);
     if (tid == 0) { S.est = S.ck * sqrt(s2 / n); sh_int[3] = S.est <= 1.0; }
     __syncthreads();
     ok = sh_int[3];
