@@ -25,8 +25,9 @@ static inline double nan_() { return std::numeric_limits<double>::quiet_NaN(); }
 // ---------------------------------------------------------------------------
 // Limiters and Meyer capacitances
 // N_DEV_DeviceSupport.C:161 (limvds), :188 (pnjlim), :309 (fetlim), :507 (qmeyer)
-// Pinned by tests/test_oracle_devsupport.py: fixed points, hand-computed
-// known answers for every branch, and bound properties.
+// Pinned by tests/test_oracle_devices.py: pnjlim's current matching, fixed
+// points, the documented bounds of fetlim / limvds, textbook Meyer values and
+// continuity across region boundaries.
 // ---------------------------------------------------------------------------
 double limvds(double vnew, double vold) {
   if (vold >= 3.5) {
@@ -351,7 +352,9 @@ static void inst_process(Mos1& mi, const Options& opt) {
 // x has length n+1; x[n] is the ground dummy (always 0).
 // Pinned by: closed-form Shichman-Hodges currents per region, diode closed
 // form, depletion charge at MJ=0.5, KCL, n<->p mirror, d<->s swap, and the
-// finite-difference Jacobian check the reference itself uses (TESTJAC).
+// finite-difference Jacobian check the reference itself uses (TESTJAC); the
+// Meyer back-averaging by the trapezoid-integral property of the gate charges
+// over a gate-voltage ramp (tests/test_oracle_pins.py).
 // ---------------------------------------------------------------------------
 static void mos1_update(Mos1& mi, const SolverState& ss, const Options& opt, const double* x,
                         const double* staCur, const double* stoCur, double* staNext, double* stoNext) {
@@ -545,6 +548,10 @@ static void mos1_update(Mos1& mi, const SolverState& ss, const Options& opt, con
 }
 
 // Master::loadDAEVectors N_DEV_MOSFET1.C:4343-4576 (old Meyer branch).
+// The limiter-correction vectors are pinned against the device's own stamps
+// times the limiters' moves, gmin excluded on the junctions, with the
+// reference's bulk-row charge term (gcgb where the stamp has gcbd,
+// :4524-4527) stated in tests/test_oracle_pins.py.
 static void mos1_load_vectors(const Mos1& mi, const SolverState& ss, const Options& opt,
                               double* F, double* Q, double* dFdxdVp, double* dQdxdVp) {
   const int Dtype = mi.mdl.dtype;

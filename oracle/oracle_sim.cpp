@@ -415,7 +415,10 @@ static double two_norm(const std::vector<double>& v, int n) { double s = 0; for 
 struct NewtonStats { int iters = 0; int status = 0; };
 
 // DampedNewton::solve N_NLS_DampedNewton.C:362-509 with FULL steps
-// (fullNewton_ :1147), updateWeights_ :297, converged_ :1191.
+// (fullNewton_ :1147), updateWeights_ :297, converged_ :1191.  The transient
+// stopping rule (weights, DeltaXTol 0.33, limiter flags) is pinned by a
+// hook-observed run in which every solve must stop exactly where the rule,
+// evaluated from the observed iterates, says (tests/test_oracle_pins.py).
 static NewtonStats newton(Work& w, SolverState ss, const NLP& p, const std::vector<double>& xcurr,
                           const double* q0, const double* q2, long* n_loads) {
   const int n = w.n;
