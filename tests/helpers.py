@@ -31,21 +31,77 @@ def waveform_error(t_a, x_a, t_b, x_b):
     return err
 
 
-def waveform_error_tube(t_a, x_a, t_b, x_b, dt):
+def waveform_error_tube(t_a, x_a, t_b, x_b, dt, n_shifts=17):
     """Like waveform_error, but each point of run a may match run b anywhere
     within +-dt in time (the comparison SPICE regression tools make for
     oscillators, whose edges shift by a fraction of a step when the Newton
-    path differs within its tolerances)."""
+    path differs within its tolerances); the shifts tried are ``n_shifts``
+    evenly spaced values in [-dt, dt]."""
     t0 = max(t_a[0], t_b[0]) + dt; t1 = min(t_a[-1], t_b[-1]) - dt
     m = (t_a >= t0) & (t_a <= t1)
     err = 0.0
-    shifts = np.linspace(-dt, dt, 17)
+    shifts = np.linspace(-dt, dt, n_shifts)
     for j in range(x_a.shape[1]):
         d = np.full(int(m.sum()), np.inf)
         for s in shifts:
             d = np.minimum(d, np.abs(x_a[m, j] - np.interp(t_a[m] + s, t_b, x_b[:, j])))
         err = max(err, float(np.max(d)) if d.size else 0.0)
     return err
+
+
+def _quad_interp(tb, xb, ta):
+    """Quadratic (three-point Lagrange) interpolation of the rows of xb at ta,
+    through the sample before and the two after each interval start."""
+    idx = np.clip(np.searchsorted(tb, ta), 1, len(tb) - 2)
+    t0, t1, t2 = tb[idx - 1], tb[idx], tb[idx + 1]
+    l0 = (ta - t1) * (ta - t2) / ((t0 - t1) * (t0 - t2))
+    l1 = (ta - t0) * (ta - t2) / ((t1 - t0) * (t1 - t2))
+    l2 = (ta - t0) * (ta - t1) / ((t2 - t0) * (t2 - t1))
+    return l0[:, None] * xb[idx - 1] + l1[:, None] * xb[idx] + l2[:, None] * xb[idx + 1]
+
+
+def waveform_error_tube_interp(t_a, x_a, t_b, x_b, dt, n_shifts=129):
+    """Largest distance of run a's points from run b's waveform when each
+    point may match b anywhere within +-dt in time, not counting the
+    interpolation error of b's piecewise-linear waveform between its own
+    samples: at each time that error is estimated by the distance between
+    b's linear and quadratic interpolants, and subtracted.  Two runs with
+    different step sequences can only be compared through an interpolant; a
+    run compared with itself at shifted sample times gives about 0."""
+    t0 = max(t_a[0], t_b[1]) + dt
+    t1 = min(t_a[-1], t_b[-3]) - dt
+    m = (t_a >= t0) & (t_a <= t1)
+    ta, a = t_a[m], x_a[m]
+    best = np.full(a.shape, np.inf)
+    for s in np.linspace(-dt, dt, n_shifts) if dt > 0 else (0.0,):
+        lin = np.stack([np.interp(ta + s, t_b, x_b[:, j]) for j in range(x_b.shape[1])], 1)
+        q = _quad_interp(t_b, x_b, ta + s)
+        best = np.minimum(best, np.maximum(0.0, np.abs(a - lin) - np.abs(q - lin)))
+    return float(best.max()) if best.size else 0.0
+
+
+def edge_times(t, v, level=0.9):
+    """Times at which ``v`` crosses ``level`` (linear interpolation between
+    samples) and the direction of each crossing (+1 rising, -1 falling)."""
+    s = np.asarray(v) - level
+    k = np.nonzero((s[:-1] < 0) != (s[1:] < 0))[0]
+    tc = t[k] - s[k] * (t[k + 1] - t[k]) / (s[k + 1] - s[k])
+    return tc, np.sign(s[k + 1])
+
+
+def edge_shift(t_a, x_a, t_b, x_b, level=0.9):
+    """Largest time shift between corresponding crossings of ``level`` in two
+    runs, over every column; the runs must cross the same number of times in
+    the same directions (else +inf)."""
+    worst = 0.0
+    for j in range(x_a.shape[1]):
+        ea, da = edge_times(t_a, x_a[:, j], level)
+        eb, db = edge_times(t_b, x_b[:, j], level)
+        if len(ea) != len(eb) or not np.array_equal(da, db):
+            return float("inf")
+        if len(ea):
+            worst = max(worst, float(np.max(np.abs(ea - eb))))
+    return worst
 
 
 def grid_probe(c, o, extra_nodes=()):

@@ -285,13 +285,13 @@ def test_launch_forms_do_not_change_results(gpu):
 
 def test_workgroup_size_does_not_change_results(gpu):
     """Instances of more than 128 unknowns (151- and 201-stage rings) give
-    bit-identical trajectories with workgroups of 128, 256 and 512 threads,
+    bit-identical trajectories with workgroups of 64, 128, 256 and 512 threads,
     fused or not: the residual norm and the local error estimate are summed
     in an order that does not depend on the workgroup size."""
     cs = [C.ring(s, seed=40 + s, mc_seed=40 + s) for s in (151, 201, 61)]
     ns = [X.host_pattern(c)["n"] for c in cs]
     out = []
-    for block, split, k in ((128, 0, 4), (256, 0, 4), (512, 0, 4), (256, 1, 1), (512, 2, 4)):
+    for block, split, k in ((128, 0, 4), (64, 0, 4), (256, 0, 4), (512, 0, 4), (256, 1, 1), (512, 2, 4)):
         b = X.Batch(cs, 0.3e-9, 1e-12, 1e-11, record_max=2000)
         b.config(block=block, iters_per_launch=k, split_eval=split)
         cnt = b.run_to_end(chunk=64)
@@ -484,18 +484,40 @@ def _period(t, v, t0=5e-9, level=0.9):
     return float(np.mean(np.diff(tc))), len(tc)
 
 
+def _accurate_edge_shift(c, tstop, cols, t_o, x_o):
+    """Largest crossing shift between the oracle's run at the default
+    tolerances and an accurate oracle run (RELTOL 1e-5): how far the
+    reference's own answer at these tolerances is from the exact waveform."""
+    from tests.helpers import edge_shift
+    a = Oracle(c)
+    a.set_tolerances(tia_reltol=1e-5)
+    ra = a.tran(tstop, 1e-12, 10e-12)
+    assert ra["status"] == 0
+    return edge_shift(t_o, x_o[:, cols], ra["t"], ra["x"][:, cols])
+
+
 def test_config1_ring101_50ns_matches_oracle(gpu):
     """The reference's own CPU-runnable case: 101-stage ring oscillator,
     .TRAN 0-50 ns with a 10 ps maximum step (about 22k steps, 49k Newton
-    iterations), on the GPU engine against the oracle: step and Newton
-    counts within 2 %, node voltages within RELTOL |V| + ABSTOL over the
-    first 5 ns, and the oscillation period within 1e-4 over the remaining
-    45 ns (about 32 periods).  Over the whole run the edges of the two runs
-    drift apart by a fraction of a step, as two runs with different rounding
-    do (the oracle at these tolerances differs from an accurate run, RELTOL
-    1e-6, by more than that: edges move by more than a picosecond), so the
-    full-window difference is printed, not bounded."""
-    from tests.helpers import waveform_error_tube
+    iterations), on the GPU engine against the oracle over the whole window:
+
+    * step and Newton counts within 2 %;
+    * node voltages within RELTOL |V| + ABSTOL over the first 5 ns;
+    * every VDD/2 crossing of every 10th node (72 per node) within one tenth
+      of the shift between the oracle's crossings and an accurate run's
+      (RELTOL 1e-5; about 23 ps by the end): an oscillator's phase is neutral,
+      so last-bit differences (the device's exp and log, the LU's elimination
+      order) accumulate into a phase drift, and at RELTOL 1e-3 the reference's
+      own drift from the exact waveform is far larger than the two runs'
+      distance;
+    * with the time axis allowed to move by that crossing shift, the node
+      voltages within RELTOL * VDD + ABSTOL over the whole window, not
+      counting the interpolation error of the oracle's piecewise-linear
+      waveform between its samples (estimated by its distance from the
+      quadratic interpolant; up to 11 mV on 10 ps steps near the top of a
+      swing), since the two runs' samples fall at different times once their
+      step sequences part (after 3.4 ns)."""
+    from tests.helpers import edge_shift, waveform_error_tube_interp
     c = C.ring(101, seed=1)
     tstop = 50e-9
     b = X.Batch([c], tstop, 1e-12, 10e-12, record_max=40000)
@@ -509,16 +531,22 @@ def test_config1_ring101_50ns_matches_oracle(gpu):
     cols = list(range(0, o.n, 10))
     early_g, early_o = t_g <= 5e-9, r["t"] <= 5e-9
     early = waveform_error(t_g[early_g], x_g[early_g][:, cols], r["t"][early_o], r["x"][early_o][:, cols])
-    tube = waveform_error_tube(t_g, x_g[:, cols], r["t"], r["x"][:, cols], 1e-12)
+    shift = edge_shift(t_g, x_g[:, cols], r["t"], r["x"][:, cols])
+    acc_shift = _accurate_edge_shift(c, tstop, cols, r["t"], r["x"])
+    dt = max(shift, 1e-15)
+    tube = waveform_error_tube_interp(t_g, x_g[:, cols], r["t"], r["x"][:, cols], dt)
     col = int(o.pattern()["node_gid"][2])
     pg, ng = _period(t_g, x_g[:, col])
     po, no = _period(r["t"], r["x"][:, col])
     print(f"config 1: steps gpu {st['steps']:.0f} oracle {r['steps']}, Newton iterations gpu {st['newton_iters']:.0f} "
-          f"oracle {r['newton_iters']}; 0-5 ns max |dV| {early:.2e} V; period gpu {pg * 1e9:.6f} ns oracle "
-          f"{po * 1e9:.6f} ns ({ng} periods); whole run within one tstep {tube:.2e} V")
+          f"oracle {r['newton_iters']}; 0-5 ns max |dV| {early:.2e} V; crossing shift gpu-oracle "
+          f"{shift * 1e12:.3f} ps, oracle-accurate {acc_shift * 1e12:.2f} ps; 0-50 ns max |dV| within that shift "
+          f"{tube:.2e} V; period gpu {pg * 1e9:.6f} ns oracle {po * 1e9:.6f} ns ({ng} periods)")
     assert abs(st["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
     assert abs(st["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2
     assert early <= 1e-3 * 1.8 + 1e-6
+    assert shift <= 0.1 * acc_shift
+    assert tube <= 1e-3 * 1.8 + 1e-6
     assert ng == no and abs(pg - po) <= 1e-4 * po
 
 
@@ -526,11 +554,21 @@ def test_mc4096_batch_runs_to_the_end_and_matches_oracle(gpu):
     """The headline workload at full size: 4096 Monte Carlo rings (stages odd
     U[11,201], per-device VTO and KP), .TRAN 0-20 ns, hmax 10 ps, in the
     autotuned launch form that the benchmark times.  Every instance finishes.
-    Six of them (the smallest, the largest, the median and three more) run
-    alone give the same step and Newton counts and the same final solution
-    bit for bit (instances are independent), and against the oracle: step and
-    Newton counts within 2 %, node voltages within RELTOL |V| + ABSTOL over
-    the first 2 ns, the oscillation period within 1e-4 after 5 ns."""
+    64 instances spread evenly over the stage range, run again as a batch of
+    their own, give the same step and Newton counts and the same final
+    solution bit for bit (instances are independent), as do the smallest, the
+    median and the largest run alone.  Against the oracle, for all 64: step
+    and Newton counts within 2 %, node voltages within RELTOL |V| + ABSTOL
+    over the first 2 ns, every VDD/2 crossing of every 10th node over the
+    whole 20 ns within 2 ps (an oscillator's phase drifts with last-bit
+    differences; the oracle's own crossings are several picoseconds from an
+    accurate run's, checked on three of them), over the whole 20 ns the node
+    voltages within RELTOL |V| + ABSTOL once the time axis may move by that
+    crossing shift (the oracle's own interpolation error between its samples
+    not counted, as in the config-1 test), and the oscillation period within
+    1e-4 after 5 ns."""
+    from concurrent.futures import ThreadPoolExecutor
+    from tests.helpers import edge_shift, waveform_error_tube_interp
     cs = C.mc_ring_batch(4096, seed=3)
     tstop, tstep, hmax = 20e-9, 1e-12, 10e-12
     b = X.Batch(cs, tstop, tstep, hmax)
@@ -540,30 +578,68 @@ def test_mc4096_batch_runs_to_the_end_and_matches_oracle(gpu):
     assert cnt["done"] == 4096 and cnt["failed"] == 0, cnt
     stages = np.array([(len(c.mos_model)) // 2 for c in cs])
     order = np.argsort(stages, kind="stable")
-    sample = [int(order[0]), int(order[-1]), int(order[len(order) // 2]), 7, 1234, 4095]
-    for i in sample:
-        c = cs[i]
-        o = Oracle(c)
-        st_b, x_b = b.instance(i, o.n)
-        b1 = X.Batch([c], tstop, tstep, hmax, record_max=20000)
-        c1 = b1.run_to_end(chunk=1024)
-        assert c1["done"] == 1, c1
-        st_1, x_1 = b1.instance(0, o.n)
-        assert st_1["steps"] == st_b["steps"] and st_1["newton_iters"] == st_b["newton_iters"]
-        assert np.array_equal(x_1, x_b)
-        r = o.tran(tstop, tstep, hmax)
+    sample = [int(order[k]) for k in np.linspace(0, len(order) - 1, 64).round().astype(int)]
+    assert stages[sample[0]] == 11 and stages[sample[-1]] == 201
+    sub = X.Batch([cs[i] for i in sample], tstop, tstep, hmax, record_max=20000)
+    csub = sub.run_to_end(chunk=1024)
+    assert csub["done"] == 64, csub
+    ns = [Oracle(cs[i]).n for i in sample]
+    for k, i in enumerate(sample):
+        st_b, x_b = b.instance(i, ns[k])
+        st_s, x_s = sub.instance(k, ns[k])
+        assert st_s["steps"] == st_b["steps"] and st_s["newton_iters"] == st_b["newton_iters"], i
+        assert np.array_equal(x_s, x_b), i
+    for k in (0, 32, 63):
+        i = sample[k]
+        b1 = X.Batch([cs[i]], tstop, tstep, hmax)
+        assert b1.run_to_end(chunk=1024)["done"] == 1
+        st_1, x_1 = b1.instance(0, ns[k])
+        st_b, x_b = b.instance(i, ns[k])
+        assert st_1["steps"] == st_b["steps"] and np.array_equal(x_1, x_b), i
+
+    def oracle_run(k):
+        o = Oracle(cs[sample[k]])
+        return o.tran(tstop, tstep, hmax), int(o.pattern()["node_gid"][2])
+
+    with ThreadPoolExecutor(16) as ex:
+        runs = list(ex.map(oracle_run, range(64)))
+    worst = dict(steps=0.0, early=0.0, shift=0.0, period=0.0)
+    for k, i in enumerate(sample):
+        r, col = runs[k]
         assert r["status"] == 0
-        t_g, x_g = b1.points(0, o.n)
+        st_1, _ = sub.instance(k, ns[k])
+        t_g, x_g = sub.points(k, ns[k])
         assert t_g[-1] == pytest.approx(tstop, rel=1e-12)
+        cols = list(range(0, ns[k], 10))
         eg, eo = t_g <= 2e-9, r["t"] <= 2e-9
         early = waveform_error(t_g[eg], x_g[eg], r["t"][eo], r["x"][eo])
-        col = int(o.pattern()["node_gid"][2])
+        shift = edge_shift(t_g, x_g[:, cols], r["t"], r["x"][:, cols])
+        tube = waveform_error_tube_interp(t_g, x_g[:, cols], r["t"], r["x"][:, cols], max(shift, 1e-15), n_shifts=33)
         pg, ng = _period(t_g, x_g[:, col])
         po, no = _period(r["t"], r["x"][:, col])
+        ds = abs(st_1["steps"] - r["steps"]) / r["steps"]
+        worst = dict(steps=max(worst["steps"], ds), early=max(worst["early"], early),
+                     shift=max(worst["shift"], shift), tube=max(worst.get("tube", 0.0), tube),
+                     period=max(worst["period"], abs(pg - po) / po))
         print(f"instance {i} ({stages[i]} stages): steps gpu {st_1['steps']:.0f} oracle {r['steps']}, Newton "
-              f"{st_1['newton_iters']:.0f} / {r['newton_iters']}; 0-2 ns max |dV| {early:.2e} V; period "
-              f"{pg * 1e9:.6f} / {po * 1e9:.6f} ns ({ng} / {no} crossings)")
-        assert abs(st_1["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
-        assert abs(st_1["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2
-        assert early <= 1e-3 * 1.8 + 1e-6
-        assert abs(ng - no) <= 1 and abs(pg - po) <= 1e-4 * po
+              f"{st_1['newton_iters']:.0f} / {r['newton_iters']}; 0-2 ns max |dV| {early:.2e} V; crossing shift "
+              f"{shift * 1e12:.3f} ps; 0-20 ns |dV| within it {tube:.2e} V; period {pg * 1e9:.6f} / {po * 1e9:.6f} ns")
+        assert abs(st_1["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2, i
+        assert abs(st_1["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2, i
+        assert early <= 1e-3 * 1.8 + 1e-6, i
+        assert shift <= 2e-12, i
+        assert tube <= 1e-3 * 1.8 + 1e-6, i
+        assert ng == no and abs(pg - po) <= 1e-4 * po, i
+    print(f"worst over the 64: {worst}")
+
+    def accurate(k):
+        r, _ = runs[k]
+        return _accurate_edge_shift(cs[sample[k]], tstop, list(range(0, ns[k], 10)), r["t"], r["x"])
+
+    with ThreadPoolExecutor(3) as ex:
+        accs = list(ex.map(accurate, (0, 32, 63)))
+    for k, acc in zip((0, 32, 63), accs):
+        print(f"instance {sample[k]}: oracle crossings {acc * 1e12:.2f} ps from an accurate run's")
+        assert acc >= 3e-12
+
+

@@ -42,6 +42,13 @@
 
 namespace xg {
 
+// Every multiply-add in the engine is rounded twice, as in a build of the
+// reference for baseline x86-64 and in the oracle: no fused multiply-adds in
+// the residual and Jacobian assembly (EpetraMatrix::linearCombo,
+// OneStep::obtainResidual), the LU and solves, the Newton update and norms,
+// the predictor and the step control.
+#pragma clang fp contract(off)
+
 // ---- per structure class (offsets into the concatenated int arrays)
 struct ClassDesc {
   int n, nnz, nlu, n_mos, n_src, nops, nterm;
@@ -206,8 +213,7 @@ __device__ __forceinline__ double block_max(double v, double* red) {
   return r;
 }
 
-#pragma clang fp contract(off)
-// J = a*dQdx + b*dFdx without contraction (EpetraMatrix::linearCombo rounding)
+// J = a*dQdx + b*dFdx (EpetraMatrix::linearCombo rounding)
 __device__ __forceinline__ double jac_combine(double a, double dq, double b, double df) { return a * dq + b * df; }
 // residual combine (OneStep::obtainResidual / NoTimeIntegration::obtainResidual rounding)
 __device__ __forceinline__ double residual_combine(bool dc, int order, double h, double f, double q, double b,
@@ -242,7 +248,6 @@ __device__ __forceinline__ double residual_gear(int order, double h, double a0, 
   res = res + 1.0 * fl;
   return res;
 }
-#pragma clang fp contract(fast)
 
 __device__ double next_bp(const BatchArgs& A, const InstDesc& I, InstState& S) {
   const double* bp = A.bp + I.bp_off;
