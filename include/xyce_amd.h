@@ -111,22 +111,41 @@ int xg_set_solver_state(xg_circuit* c, const xg_solver_state* ss);
 /* a2: DeviceMgr::updateState -> MOSFET1 Master::updateState (N_DEV_MOSFET1.C:4241).
  * Host arrays in Xyce layout: x[n], state/store vectors as laid out by
  * xg_circuit_pattern's mos_stasto (NULL only for a circuit without MOSFETs).
+ * Writes the MOSFETs' entries of sta_next / sto_next and leaves every other
+ * entry as the caller had it (a master writes only its own devices').
  * *limited = !allDevicesConverged (limited may be NULL). */
 int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
                     double* sta_next, double* sto_next, int* limited);
 
 /* a5+a7: DeviceMgr::loadDAEVectors (N_DEV_MOSFET1.C:4343) plus the linear
  * devices' separateLoad contribution (N_LOA_CktLoader.C:700-787).  Uses the
- * device state of the last xg_update_state.  Each output has n entries. */
+ * device state of the last xg_update_state.  Each output has n entries and
+ * receives this handle's contribution ADDED to what it holds, as the
+ * reference's masters "+=" into vectors the loader has zeroed
+ * (N_DEV_MOSFET1.C:4393-4411, N_LOA_NonlinearEquationLoader.C:385-390):
+ * out = out + contribution, one rounding per entry.  NULL outputs are skipped. */
 int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp);
 
 /* a6+a7: DeviceMgr::loadDAEMatrices (N_DEV_MOSFET1.C:4587) plus the linear
- * part (FilteredMatrix::addToMatrix N_LAS_FilteredMatrix.C:632). nnz each. */
+ * part (FilteredMatrix::addToMatrix N_LAS_FilteredMatrix.C:632). nnz each,
+ * added to the caller's values as in xg_load_vectors (the masters' "+="
+ * through raw matrix pointers, N_DEV_MOSFET1.C:4629-4865). */
 int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals);
 
 /* a8: OneStep::obtainResidual / obtainJacobian (N_TIA_OneStep.C:219,469)
- * on the vectors/matrices of the last loads.  q0/q2 = qHistory[0]/[2]. */
+ * on the vectors/matrices of the last loads (this handle's contributions).
+ * q0/q2 = qHistory[0]/[2].  rhs and jac_vals are overwritten. */
 int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals);
+
+/* The same four calls on DEVICE arrays (pointers into HIP device memory of
+ * the handle's device), so a caller that keeps its vectors and matrices on
+ * the GPU pays no host copies.  Same layouts and semantics (accumulation
+ * included); each call returns when its outputs are complete. */
+int xg_update_state_dev(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
+                        double* sta_next, double* sto_next, int* limited);
+int xg_load_vectors_dev(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp);
+int xg_load_matrices_dev(xg_circuit* c, double* dFdx_vals, double* dQdx_vals);
+int xg_combine_dev(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals);
 
 /* ---------------- sparse LU (linear side, a9) ---------------- */
 
@@ -149,6 +168,10 @@ int xg_lu_destroy(xg_lu* lu);
 int xg_lu_numeric(xg_lu* lu, const double* vals);
 /* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
+/* xg_lu_numeric / xg_lu_solve on device arrays (HIP device memory); same
+ * semantics; each call returns when its outputs are complete. */
+int xg_lu_numeric_dev(xg_lu* lu, const double* vals);
+int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose);
 /* info[0..5] = factor entries (nnz(L+U), or front elements when
  * multifrontal), factor levels, solve levels, flops per factorization,
  * method (1 level-scheduled, 2 multifrontal), supernodes */
@@ -167,6 +190,46 @@ int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max);
  * threshold test where the FP64 ones passed (those solve with FP64
  * factors), current precision. */
 int xg_lu_precision_stats(const xg_lu* lu, double* stats);
+
+/* ---------------- one circuit's transient, resident on the GPU ---------------- */
+
+/* DC operating point and .TRAN of ONE circuit (a power grid, a gate array)
+ * with every vector of the loop in device memory: Transient::doLoopProcess
+ * (src/AnalysisPKG/N_ANP_Transient.C:1184) with OneStep's trapezoid, predictor
+ * and step control (N_TIA_OneStep.C:135,1946,2124), DampedNewton
+ * (N_NLS_DampedNewton.C:362-509) and the DC strategy of
+ * Interface::spiceStrategy (N_NLS_NOX_Interface.C:321-437: Newton from
+ * initJct, then gmin stepping, then source stepping).  The device loads are
+ * this library's circuit, the linear solves its LU (lu_method as in
+ * xg_lu_symbolic_ex).  The host takes the decisions from a few scalars read
+ * back per Newton iteration. */
+typedef struct xg_tran xg_tran;
+int xg_tran_create(const xg_circuit_desc* desc, double tstop, double tstep, double hmax, int lu_method,
+                   xg_tran** out);
+int xg_tran_destroy(xg_tran* t);
+/* The LU's factor precision, as xg_lu_set_precision. */
+int xg_tran_set_precision(xg_tran* t, int precision, double ir_tol, int ir_max);
+/* TIA RELTOL/ABSTOL (local error) and NONLIN-TRAN RELTOL/ABSTOL (Newton
+ * weights); defaults 1e-3, 1e-6, 1e-2, 1e-6.  Before the first run only. */
+int xg_tran_set_tolerances(xg_tran* t, double reltol, double abstol, double nl_reltol, double nl_abstol);
+/* Record, at the DC point and every accepted step (up to record_max points),
+ * the k products w_r . x (w: k x n row-major host array; compensated sums)
+ * followed by the n_sel unknowns x[sel[i]], or with k = n_sel = 0 the whole
+ * solution.  Before the first run only. */
+int xg_tran_set_record(xg_tran* t, int k, const double* w, int n_sel, const int* sel, int record_max);
+/* Runs (the DC operating point first) until t_until or max_steps attempted
+ * steps or the end of the window.  stats[0..13] = accepted steps, rejected
+ * steps, Newton iterations, loads, time reached, status (0 ok, -100 no DC
+ * point, -101 15 rejections in a row, -102 step below 1e-18 s), finished,
+ * DC Newton iterations, DC stage (0 Newton, 1 gmin, 2 source stepping),
+ * seconds, singular factorisations, LU refinement corrections, LU solves
+ * finished by FP64 factors, points recorded. */
+int xg_tran_run(xg_tran* t, double t_until, long long max_steps, double* stats);
+/* Recorded points: t[max], the integration order of the step that ended
+ * there (0 = DC point), y[max * (k + n_sel, or n)]; returns the count copied. */
+int xg_tran_points(xg_tran* t, int max_points, double* times, int* order, double* y);
+/* The last accepted solution (n entries). */
+int xg_tran_solution(xg_tran* t, double* x);
 
 /* ---------------- batched Monte Carlo engine ---------------- */
 

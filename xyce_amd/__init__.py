@@ -38,12 +38,16 @@ ERRORS = {-1: "bad argument", -2: "HIP runtime error", -3: "circuit setup failed
 EXPORTS = [
     "xg_circuit_create", "xg_circuit_destroy", "xg_circuit_sizes", "xg_circuit_pattern", "xg_set_solver_state",
     "xg_update_state", "xg_load_vectors", "xg_load_matrices", "xg_combine",
+    "xg_update_state_dev", "xg_load_vectors_dev", "xg_load_matrices_dev", "xg_combine_dev",
     "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
+    "xg_lu_numeric_dev", "xg_lu_solve_dev",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
     "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_group_info", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
+    "xg_tran_create", "xg_tran_destroy", "xg_tran_set_precision", "xg_tran_set_tolerances", "xg_tran_set_record",
+    "xg_tran_run", "xg_tran_points", "xg_tran_solution",
 ]
 
 PRED_WIN = 256      # learned predictor: unknowns per window (XG_PRED_WIN)
@@ -204,14 +208,20 @@ class DeviceMgr:
         _check(lib().xg_update_state(self.h, _p(x), _p(sc), _p(oc), _p(sn), _p(on), C.byref(lim)), "updateState")
         return sn[: self.nsta], on[: self.nsto], int(lim.value)
 
-    def loadDAEVectors(self):
-        out = [np.zeros(self.n) for _ in range(5)]
+    def loadDAEVectors(self, into=None):
+        """This circuit's F, Q, B and limiter vectors.  ``into``: a dict of
+        the five arrays to add them to (the reference's "+=" into vectors
+        that other devices share); by default zero arrays."""
+        names = ("F", "Q", "B", "dFdxdVp", "dQdxdVp")
+        out = [np.zeros(self.n) for _ in names] if into is None else [into[k] for k in names]
+        assert all(v.dtype == np.float64 and v.flags.c_contiguous and v.size == self.n for v in out)
         _check(lib().xg_load_vectors(self.h, *[_p(v) for v in out]), "loadDAEVectors")
-        return dict(zip(("F", "Q", "B", "dFdxdVp", "dQdxdVp"), out))
+        return dict(zip(names, out))
 
-    def loadDAEMatrices(self):
-        dF = np.zeros(self.nnz)
-        dQ = np.zeros(self.nnz)
+    def loadDAEMatrices(self, into=None):
+        """dFdx and dQdx CSR values, added to ``into`` = (dF, dQ) if given."""
+        dF, dQ = (np.zeros(self.nnz), np.zeros(self.nnz)) if into is None else into
+        assert all(v.dtype == np.float64 and v.flags.c_contiguous and v.size == self.nnz for v in (dF, dQ))
         _check(lib().xg_load_matrices(self.h, _p(dF), _p(dQ)), "loadDAEMatrices")
         return dF, dQ
 
@@ -496,6 +506,89 @@ class MLP:
 
     def sync(self):
         _check(lib().xg_mlp_sync(self.h), "xg_mlp_sync")
+
+
+class Transient:
+    """One circuit's DC operating point and .TRAN on the GPU, every vector of
+    the Newton and time-integration loop resident in device memory
+    (``Transient::doLoopProcess`` with OneStep, DampedNewton and the DC
+    strategy; src/AnalysisPKG/N_ANP_Transient.C:1184)."""
+
+    STATS = ("steps", "rejects", "newton_iters", "loads", "t", "status", "finished", "dc_iters", "dc_stage",
+             "seconds", "singular", "ir_corrections", "fp64_fallbacks", "points")
+
+    def __init__(self, circuit, tstop, tstep, hmax=0.0, lu_method=0, gmin=1e-12, temp=300.15):
+        """``lu_method``: 0 automatic, 1 level-scheduled, 2 multifrontal."""
+        self._desc, self._keep = make_desc(circuit, gmin, temp)
+        h = C.c_void_p()
+        _check(lib().xg_tran_create(C.byref(self._desc), C.c_double(tstop), C.c_double(tstep), C.c_double(hmax),
+                                    int(lu_method), C.byref(h)), "xg_tran_create")
+        self.h = h
+        self.n = host_pattern(circuit, gmin, temp)["n"]
+        self._k = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().xg_tran_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision, ir_tol=1e-12, ir_max=10):
+        _check(lib().xg_tran_set_precision(self.h, int(precision), C.c_double(ir_tol), int(ir_max)),
+               "xg_tran_set_precision")
+
+    def set_tolerances(self, reltol=1e-3, abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
+        _check(lib().xg_tran_set_tolerances(self.h, C.c_double(reltol), C.c_double(abstol), C.c_double(nl_reltol),
+                                            C.c_double(nl_abstol)), "xg_tran_set_tolerances")
+
+    def set_record(self, record_max, w=None, sel=None):
+        """Record, at up to ``record_max`` points, the products of the rows
+        of ``w`` (k x n) with the solution followed by the unknowns ``sel``,
+        or the whole solution when both are None."""
+        k = 0
+        wp = None
+        if w is not None:
+            w = np.ascontiguousarray(w, np.float64).reshape(-1, self.n)
+            k = w.shape[0]
+            self._w = w
+            wp = _p(w)
+        ns = 0
+        sp = None
+        if sel is not None:
+            sel = np.ascontiguousarray(sel, np.int32)
+            ns = sel.size
+            self._sel = sel
+            sp = _p(sel)
+        self._k = k + ns
+        _check(lib().xg_tran_set_record(self.h, k, wp, ns, sp, int(record_max)), "xg_tran_set_record")
+
+    def run(self, t_until=np.inf, max_steps=2**62):
+        s = np.zeros(14)
+        _check(lib().xg_tran_run(self.h, C.c_double(min(t_until, 1e300)), C.c_longlong(int(max_steps)), _p(s)),
+               "xg_tran_run")
+        out = dict(zip(self.STATS, (float(v) for v in s)))
+        for k in ("steps", "rejects", "newton_iters", "loads", "status", "finished", "dc_iters", "dc_stage",
+                  "singular", "ir_corrections", "fp64_fallbacks", "points"):
+            out[k] = int(out[k])
+        return out
+
+    def points(self, max_points):
+        width = self._k or self.n
+        t = np.zeros(max(max_points, 1))
+        o = np.zeros(max(max_points, 1), np.int32)
+        y = np.zeros(max(max_points * width, 1))
+        m = _check(lib().xg_tran_points(self.h, int(max_points), _p(t), _p(o), _p(y)), "xg_tran_points")
+        return t[:m], o[:m], y[: m * width].reshape(m, width)
+
+    def solution(self):
+        x = np.zeros(self.n)
+        _check(lib().xg_tran_solution(self.h, _p(x)), "xg_tran_solution")
+        return x
 
 
 def host_mf_plan(n, rowptr, colind, values=None):

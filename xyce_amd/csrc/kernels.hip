@@ -668,15 +668,15 @@ struct xg_lu {
     *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
     return XG_OK;
   }
-  // solve with the factors f and device vectors db -> dx (arithmetic in T);
-  // flag[1] collects non-finite results
+  // solve with the factors f and device vectors db -> dx (arithmetic in T),
+  // yy the permuted work vector; *nan_flag collects non-finite results
   template <class T>
-  int solve_device(const T* f, T* yy, const double* db, double* dx) {
+  int solve_device(const T* f, T* yy, const double* db, double* dx, int* nan_flag) {
     const int n = s.n;
     if (one_wg) {
       k_lu_solve_1wg<T><<<1, 1024, 0, stream>>>(n, lu_rowptr.p, lu_colind.p, lu_diag.p, f, (int)h_fs.size() - 1,
                                                 fs_ptr.p, fs_rows.p, (int)h_bs.size() - 1, bs_ptr.p, bs_rows.p,
-                                                prow.p, pcol.p, db, yy, dx, flag.p + 1);
+                                                prow.p, pcol.p, db, yy, dx, nan_flag);
     } else {
       k_permute_in<T><<<nblk(n), 256, 0, stream>>>(n, prow.p, db, yy);
       for (size_t l = 0; l + 1 < h_fs.size(); ++l)
@@ -685,7 +685,7 @@ struct xg_lu {
       for (size_t l = 0; l + 1 < h_bs.size(); ++l)
         if (h_bs[l + 1] > h_bs[l])
           k_solve_bwd_level<T><<<nblk(h_bs[l + 1] - h_bs[l]), 256, 0, stream>>>(h_bs[l], h_bs[l + 1], bs_rows.p, lu_rowptr.p, lu_colind.p, lu_diag.p, f, yy);
-      k_permute_out<T><<<nblk(n), 256, 0, stream>>>(n, pcol.p, yy, dx, flag.p + 1);
+      k_permute_out<T><<<nblk(n), 256, 0, stream>>>(n, pcol.p, yy, dx, nan_flag);
     }
     HIPCHK(hipGetLastError());
     return XG_OK;
@@ -699,7 +699,8 @@ struct xg_lu {
   }
   int solve_any(bool fp32, const double* db, double* dx) {
     if (method == 2) return mfp->solve(db, dx, fp32, flag.p + 1);
-    return fp32 ? solve_device<float>(lu32.p, y32.p, db, dx) : solve_device<double>(lu.p, y.p, db, dx);
+    return fp32 ? solve_device<float>(lu32.p, y32.p, db, dx, flag.p + 1)
+                : solve_device<double>(lu.p, y.p, db, dx, flag.p + 1);
   }
   // r = b - A x; returns max|r| / max|b| (0/0 -> 0)
   int residual(const double* db, const double* dx, double* rel) {
@@ -775,6 +776,17 @@ struct xg_circuit {
   int n_long_rows = 0, n_long_slots = 0, n_vchunks = 0, n_mchunks = 0;
   DBuf<double> wave_data, xy_sta, xy_sto;
   xg_solver_state ss{};
+  // pinned host staging for the loads into host arrays (allocated on first use)
+  double* hstage = nullptr;
+  size_t hstage_n = 0;
+  ~xg_circuit() { if (hstage) (void)hipHostFree(hstage); }
+  hipError_t stage(size_t cnt) {
+    if (cnt <= hstage_n) return hipSuccess;
+    if (hstage) { (void)hipHostFree(hstage); hstage = nullptr; hstage_n = 0; }
+    hipError_t e = hipHostMalloc((void**)&hstage, cnt * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) hstage_n = cnt;
+    return e;
+  }
 };
 
 static int circuit_upload(xg_circuit* c) {
@@ -860,6 +872,9 @@ static int circuit_upload(xg_circuit* c) {
   return XG_OK;
 }
 
+static void gather_vectors(xg_circuit* c);
+static void gather_matrices(xg_circuit* c);
+
 static SrcArgs src_args(xg_circuit* c) {
   SrcArgs a;
   a.bg_ptr = c->bg_ptr.p; a.bg_ent = c->bg_ent.p; a.src_wave = c->src_wave.p; a.wave_type = c->wave_type.p;
@@ -867,6 +882,75 @@ static SrcArgs src_args(xg_circuit* c) {
   a.time = c->ss.time; a.tstep = c->ss.tstep; a.tstop = c->ss.tstop;
   return a;
 }
+
+namespace xg {
+
+CircuitDev circuit_dev(xg_circuit* c) {
+  CircuitDev d;
+  d.n = c->hc.n; d.nnz = c->hc.nnz; d.n_mos = c->hc.n_mos; d.stream = (void*)c->stream;
+  d.x = c->x.p; d.F = c->F.p; d.Q = c->Q.p; d.B = c->B.p; d.FL = c->FL.p; d.QL = c->QL.p;
+  d.dFdx = c->dFdx.p; d.dQdx = c->dQdx.p; d.limited = c->limited.p;
+  return d;
+}
+
+const HostCircuit& circuit_host(const xg_circuit* c) { return c->hc; }
+
+int circuit_load(xg_circuit* c) {
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  const int N = h.n_mos;
+  k_fill<<<1, 1, 0, s>>>(c->limited.p, 1, 0);
+  if (N > 0) {
+    EvalArgs a;
+    a.n_mos = N; a.cls = c->cls.p; a.mcls = c->mcls.p; a.tvbi = c->tvbi.p; a.beta = c->beta.p; a.tvto = c->tvto.p;
+    a.lid = c->lid.p; a.mask = c->mask.p; a.x = c->x.p; a.sto_cur = c->sto_cur.p; a.sto_nxt_in = c->sto_nxt.p;
+    a.sta_cur = c->sta_cur.p; a.sto_out = c->sto_nxt.p; a.sta_out = c->sta_nxt.p; a.rec = c->rec.p;
+    a.limited = c->limited.p;
+    a.fl.dcop = c->ss.dcop; a.fl.tran = c->ss.tran; a.fl.init_jct = c->ss.init_jct; a.fl.newton_iter = c->ss.newton_iter;
+    k_mos1_eval<<<nblk(N), 256, 0, s>>>(a);
+  }
+  gather_vectors(c);
+  gather_matrices(c);
+  HIPCHK(hipGetLastError());
+  return XG_OK;
+}
+
+int circuit_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* J) {
+  HostCircuit& h = c->hc;
+  hipStream_t s = c->stream;
+  const int n = h.n;
+  k_combine_rhs<<<nblk(n), 256, 0, s>>>(n, c->ss.dcop, c->ss.order, c->ss.h, c->F.p, c->Q.p, c->B.p, c->FL.p,
+                                        c->QL.p, c->ss.dcop ? c->F.p : q0, c->ss.dcop ? c->F.p : q2, rhs);
+  double a, b;
+  if (c->ss.dcop) { a = 1.0e-20; b = 1.0; } else { a = 1.0 / c->ss.h; b = c->ss.order == 2 ? 0.5 : 1.0; }
+  k_combine_jac<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, a, b, c->dFdx.p, c->dQdx.p, J);
+  HIPCHK(hipGetLastError());
+  return XG_OK;
+}
+
+int circuit_history(xg_circuit* c, int op) {
+  const size_t N = c->hc.n_mos;
+  if (N == 0) return XG_OK;
+  hipStream_t s = c->stream;
+  if (op == 0) {
+    k_fill<<<nblk(8 * N), 256, 0, s>>>(c->sta_cur.p, 8 * N, 0.0);
+    k_fill<<<nblk(8 * N), 256, 0, s>>>(c->sta_nxt.p, 8 * N, 0.0);
+    k_fill<<<nblk(6 * N), 256, 0, s>>>(c->sto_cur.p, 6 * N, 0.0);
+    k_fill<<<nblk(6 * N), 256, 0, s>>>(c->sto_nxt.p, 6 * N, 0.0);
+  } else if (op == 1) {
+    HIPCHK(hipMemcpyAsync(c->sta_cur.p, c->sta_nxt.p, sizeof(double) * 8 * N, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->sto_cur.p, c->sto_nxt.p, sizeof(double) * 6 * N, hipMemcpyDeviceToDevice, s));
+  } else if (op == 2) {
+    HIPCHK(hipMemcpyAsync(c->sta_nxt.p, c->sta_cur.p, sizeof(double) * 8 * N, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->sto_nxt.p, c->sto_cur.p, sizeof(double) * 6 * N, hipMemcpyDeviceToDevice, s));
+  } else {
+    return XG_E_ARG;
+  }
+  HIPCHK(hipGetLastError());
+  return XG_OK;
+}
+
+}  // namespace xg
 
 extern "C" {
 
@@ -921,24 +1005,41 @@ int xg_set_solver_state(xg_circuit* c, const xg_solver_state* ss) {
   return XG_OK;
 }
 
-int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur, double* sta_next,
-                    double* sto_next, int* limited) {
+// a2 on host (dev = false) or device (dev = true) arrays in Xyce's layout.
+// Only the MOSFETs' state and store entries of sta_next / sto_next are
+// written; every other entry keeps the caller's value, as each reference
+// master writes only its own devices' entries.
+static int update_state_impl(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
+                             double* sta_next, double* sto_next, int* limited, bool dev) {
   if (!c || !x) return XG_E_ARG;
   HostCircuit& h = c->hc;
   hipStream_t s = c->stream;
   const int N = h.n_mos;
   if (N > 0 && (!sta_cur || !sto_cur || !sta_next || !sto_next)) return XG_E_ARG;
-  HIPCHK(hipMemcpyAsync(c->x.p, x, sizeof(double) * h.n, hipMemcpyHostToDevice, s));
+  const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(c->x.p, x, sizeof(double) * h.n, in, s));
   k_fill<<<1, 1, 0, s>>>(c->x.p + h.n, 1, 0.0);
   if (N > 0) {
-    HIPCHK(hipMemcpyAsync(c->xy_sta.p, sta_cur, sizeof(double) * h.nsta, hipMemcpyHostToDevice, s));
-    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->xy_sta.p, c->sta_cur.p);
-    HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_cur, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
-    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->xy_sto.p, c->sto_cur.p);
-    HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_next, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
-    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->xy_sto.p, c->sto_nxt.p);
+    const double* sc = sta_cur;
+    const double* oc = sto_cur;
+    const double* on = sto_next;
+    if (!dev) {
+      HIPCHK(hipMemcpyAsync(c->xy_sta.p, sta_cur, sizeof(double) * h.nsta, hipMemcpyHostToDevice, s));
+      sc = c->xy_sta.p;
+    }
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, sc, c->sta_cur.p);
+    if (!dev) {
+      HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_cur, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
+      oc = c->xy_sto.p;
+    }
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, oc, c->sto_cur.p);
+    if (!dev) {
+      HIPCHK(hipMemcpyAsync(c->xy_sto.p, sto_next, sizeof(double) * h.nsto, hipMemcpyHostToDevice, s));
+      on = c->xy_sto.p;
+    }
+    k_sta_to_soa<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, on, c->sto_nxt.p);
   }
-  HIPCHK(hipMemsetAsync(c->limited.p, 0, sizeof(int), s));
+  k_fill<<<1, 1, 0, s>>>(c->limited.p, 1, 0);
   if (N > 0) {
     EvalArgs a;
     a.n_mos = N; a.cls = c->cls.p; a.mcls = c->mcls.p; a.tvbi = c->tvbi.p; a.beta = c->beta.p; a.tvto = c->tvto.p;
@@ -950,10 +1051,17 @@ int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const
     HIPCHK(hipGetLastError());
     // in-place read of sto_nxt_in vs write of sto_out: each thread reads its
     // own inputs before writing its own outputs, so aliasing is safe.
-    k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->sta_nxt.p, c->xy_sta.p);
-    HIPCHK(hipMemcpyAsync(sta_next, c->xy_sta.p, sizeof(double) * h.nsta, hipMemcpyDeviceToHost, s));
-    k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->sto_nxt.p, c->xy_sto.p);
-    HIPCHK(hipMemcpyAsync(sto_next, c->xy_sto.p, sizeof(double) * h.nsto, hipMemcpyDeviceToHost, s));
+    if (dev) {
+      k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->sta_nxt.p, sta_next);
+      k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->sto_nxt.p, sto_next);
+    } else {
+      // the caller's other entries come back unchanged
+      HIPCHK(hipMemcpyAsync(c->xy_sta.p, sta_next, sizeof(double) * h.nsta, hipMemcpyHostToDevice, s));
+      k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 8, c->sta0.p, c->sta_nxt.p, c->xy_sta.p);
+      HIPCHK(hipMemcpyAsync(sta_next, c->xy_sta.p, sizeof(double) * h.nsta, hipMemcpyDeviceToHost, s));
+      k_soa_to_sta<<<nblk(N), 256, 0, s>>>(N, 6, c->sto0.p, c->sto_nxt.p, c->xy_sto.p);
+      HIPCHK(hipMemcpyAsync(sto_next, c->xy_sto.p, sizeof(double) * h.nsto, hipMemcpyDeviceToHost, s));
+    }
   }
   int lim = 0;
   HIPCHK(hipMemcpyAsync(&lim, c->limited.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -962,8 +1070,47 @@ int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const
   return XG_OK;
 }
 
-int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp) {
-  if (!c) return XG_E_ARG;
+int xg_update_state(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur, double* sta_next,
+                    double* sto_next, int* limited) {
+  return update_state_impl(c, x, sta_cur, sto_cur, sta_next, sto_next, limited, false);
+}
+
+int xg_update_state_dev(xg_circuit* c, const double* x, const double* sta_cur, const double* sto_cur,
+                        double* sta_next, double* sto_next, int* limited) {
+  return update_state_impl(c, x, sta_cur, sto_cur, sta_next, sto_next, limited, true);
+}
+
+// Adds k device arrays of m entries each (this handle's contributions) into
+// the caller's arrays: device arrays by a kernel on the handle's stream, host
+// arrays through the pinned staging buffer.  Each output entry becomes
+// caller's value + contribution, one rounding, as a master's "+=" into a
+// vector the loader has zeroed or other devices have filled.
+static int accumulate_out(xg_circuit* c, int k, const double* const* src, double* const* dst, size_t m, bool dev) {
+  hipStream_t s = c->stream;
+  if (dev) {
+    for (int i = 0; i < k; ++i)
+      if (dst[i] && m) k_axpy_f64<<<nblk(m), 256, 0, s>>>((int)m, src[i], dst[i]);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    return XG_OK;
+  }
+  int used = 0;
+  for (int i = 0; i < k; ++i) used += dst[i] != nullptr;
+  if (used == 0 || m == 0) { HIPCHK(hipStreamSynchronize(s)); return XG_OK; }
+  HIPCHK(c->stage((size_t)used * m));
+  for (int i = 0, j = 0; i < k; ++i)
+    if (dst[i]) HIPCHK(hipMemcpyAsync(c->hstage + (size_t)(j++) * m, src[i], sizeof(double) * m, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int i = 0, j = 0; i < k; ++i) {
+    if (!dst[i]) continue;
+    const double* st = c->hstage + (size_t)(j++) * m;
+    double* d = dst[i];
+    for (size_t e = 0; e < m; ++e) d[e] += st[e];
+  }
+  return XG_OK;
+}
+
+static void gather_vectors(xg_circuit* c) {
   HostCircuit& h = c->hc;
   hipStream_t s = c->stream;
   const int n = h.n;
@@ -975,18 +1122,9 @@ int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdx
     k_gather_vectors_fin<<<c->n_long_rows, 64, 0, s>>>(c->long_rows.p, c->vrc.p, c->vpart.p, src_args(c), c->F.p,
                                                        c->Q.p, c->B.p, c->FL.p, c->QL.p);
   }
-  HIPCHK(hipGetLastError());
-  if (F) HIPCHK(hipMemcpyAsync(F, c->F.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  if (Q) HIPCHK(hipMemcpyAsync(Q, c->Q.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  if (B) HIPCHK(hipMemcpyAsync(B, c->B.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  if (dFdxdVp) HIPCHK(hipMemcpyAsync(dFdxdVp, c->FL.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  if (dQdxdVp) HIPCHK(hipMemcpyAsync(dQdxdVp, c->QL.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return XG_OK;
 }
 
-int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
-  if (!c) return XG_E_ARG;
+static void gather_matrices(xg_circuit* c) {
   HostCircuit& h = c->hc;
   hipStream_t s = c->stream;
   k_gather_matrices<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, h.n_mos, c->mg_ptr.p, c->mg_ent.p, c->rec.p, c->linF.p,
@@ -996,22 +1134,54 @@ int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
     k_gather_matrices_fin<<<c->n_long_slots, 64, 0, s>>>(c->long_slots.p, c->msc.p, c->mpart.p, c->linF.p, c->linQ.p,
                                                          c->dFdx.p, c->dQdx.p, 0.0, 0.0, nullptr, nullptr, nullptr);
   }
-  HIPCHK(hipGetLastError());
-  if (dFdx_vals) HIPCHK(hipMemcpyAsync(dFdx_vals, c->dFdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
-  if (dQdx_vals) HIPCHK(hipMemcpyAsync(dQdx_vals, c->dQdx.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
-  return XG_OK;
 }
 
-int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals) {
+static int load_vectors_impl(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp,
+                             bool dev) {
+  if (!c) return XG_E_ARG;
+  gather_vectors(c);
+  HIPCHK(hipGetLastError());
+  const double* src[5] = {c->F.p, c->Q.p, c->B.p, c->FL.p, c->QL.p};
+  double* dst[5] = {F, Q, B, dFdxdVp, dQdxdVp};
+  return accumulate_out(c, 5, src, dst, (size_t)c->hc.n, dev);
+}
+
+static int load_matrices_impl(xg_circuit* c, double* dFdx_vals, double* dQdx_vals, bool dev) {
+  if (!c) return XG_E_ARG;
+  gather_matrices(c);
+  HIPCHK(hipGetLastError());
+  const double* src[2] = {c->dFdx.p, c->dQdx.p};
+  double* dst[2] = {dFdx_vals, dQdx_vals};
+  return accumulate_out(c, 2, src, dst, (size_t)c->hc.nnz, dev);
+}
+
+int xg_load_vectors(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp) {
+  return load_vectors_impl(c, F, Q, B, dFdxdVp, dQdxdVp, false);
+}
+
+int xg_load_vectors_dev(xg_circuit* c, double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp) {
+  return load_vectors_impl(c, F, Q, B, dFdxdVp, dQdxdVp, true);
+}
+
+int xg_load_matrices(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
+  return load_matrices_impl(c, dFdx_vals, dQdx_vals, false);
+}
+
+int xg_load_matrices_dev(xg_circuit* c, double* dFdx_vals, double* dQdx_vals) {
+  return load_matrices_impl(c, dFdx_vals, dQdx_vals, true);
+}
+
+static int combine_impl(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals, bool dev) {
   if (!c) return XG_E_ARG;
   HostCircuit& h = c->hc;
   hipStream_t s = c->stream;
   const int n = h.n;
+  const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind out = dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
   if (!c->ss.dcop) {
     if (!q0) return XG_E_ARG;
-    HIPCHK(hipMemcpyAsync(c->q0.p, q0, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    if (q2) HIPCHK(hipMemcpyAsync(c->q2.p, q2, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->q0.p, q0, sizeof(double) * n, in, s));
+    if (q2) HIPCHK(hipMemcpyAsync(c->q2.p, q2, sizeof(double) * n, in, s));
     else k_fill<<<nblk(n), 256, 0, s>>>(c->q2.p, n, 0.0);
   }
   k_combine_rhs<<<nblk(n), 256, 0, s>>>(n, c->ss.dcop, c->ss.order, c->ss.h, c->F.p, c->Q.p, c->B.p, c->FL.p, c->QL.p,
@@ -1020,10 +1190,18 @@ int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, d
   if (c->ss.dcop) { a = 1.0e-20; b = 1.0; } else { a = 1.0 / c->ss.h; b = c->ss.order == 2 ? 0.5 : 1.0; }
   k_combine_jac<<<nblk(h.nnz), 256, 0, s>>>(h.nnz, a, b, c->dFdx.p, c->dQdx.p, c->J.p);
   HIPCHK(hipGetLastError());
-  if (rhs) HIPCHK(hipMemcpyAsync(rhs, c->rhs.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-  if (jac_vals) HIPCHK(hipMemcpyAsync(jac_vals, c->J.p, sizeof(double) * h.nnz, hipMemcpyDeviceToHost, s));
+  if (rhs) HIPCHK(hipMemcpyAsync(rhs, c->rhs.p, sizeof(double) * n, out, s));
+  if (jac_vals) HIPCHK(hipMemcpyAsync(jac_vals, c->J.p, sizeof(double) * h.nnz, out, s));
   HIPCHK(hipStreamSynchronize(s));
   return XG_OK;
+}
+
+int xg_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals) {
+  return combine_impl(c, q0, q2, rhs, jac_vals, false);
+}
+
+int xg_combine_dev(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* jac_vals) {
+  return combine_impl(c, q0, q2, rhs, jac_vals, true);
 }
 
 // ---------------- LU ----------------
@@ -1062,10 +1240,11 @@ int xg_lu_destroy(xg_lu* lu) {
   return XG_OK;
 }
 
-int xg_lu_numeric(xg_lu* lu, const double* vals) {
+static int lu_numeric_impl(xg_lu* lu, const double* vals, bool dev) {
   if (!lu || !vals) return XG_E_ARG;
   hipStream_t s = lu->stream;
-  HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
+  const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, in, s));
   lu->have64 = false;
   int bad = 0, st;
   bool fp32_failed = false;
@@ -1085,14 +1264,21 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   }
   // the pivot order itself fails the threshold test (KLU would re-pivot):
   // analyse again steering the transversal by the values, then refactor
+  std::vector<double> hv;
+  if (dev) {
+    hv.resize(lu->s.nnz_a);
+    HIPCHK(hipMemcpyAsync(hv.data(), lu->avals.p, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  }
+  const double* hvals = dev ? hv.data() : vals;
   try {
-    if (lu->analyse(vals, lu->method) != XG_OK) return bad;
+    if (lu->analyse(hvals, lu->method) != XG_OK) return bad;
   } catch (...) {
     return bad;
   }
   st = lu->upload();
   if (st != XG_OK) return st;
-  HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(lu->avals.p, hvals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
   if (lu->mode == 1) {
     st = lu->factor_any(true, &bad);
     if (st != XG_OK) return st;
@@ -1106,25 +1292,45 @@ int xg_lu_numeric(xg_lu* lu, const double* vals) {
   return XG_OK;
 }
 
-int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
+int xg_lu_numeric(xg_lu* lu, const double* vals) { return lu_numeric_impl(lu, vals, false); }
+
+int xg_lu_numeric_dev(xg_lu* lu, const double* vals) { return lu_numeric_impl(lu, vals, true); }
+
+static int lu_solve_impl(xg_lu* lu, const double* b, double* x, int transpose, bool dev) {
   if (!lu || !b || !x) return XG_E_ARG;
   if (transpose) return XG_E_ARG;  // Xyce's transient path never requests a transpose solve
   hipStream_t s = lu->stream;
   const int n = lu->s.n;
-  HIPCHK(hipMemcpyAsync(lu->b.p, b, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIPCHK(hipMemsetAsync(lu->flag.p + 1, 0, sizeof(int), s));
+  HIPCHK(hipMemcpyAsync(lu->b.p, b, sizeof(double) * n, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  k_fill<<<1, 1, 0, s>>>(lu->flag.p + 1, 1, 0);
   int st;
   if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
   else st = lu->solve_any(false, lu->b.p, lu->x.p);
   if (st != XG_OK) return st;
-  HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   int nanf = 0;
   HIPCHK(hipMemcpyAsync(&nanf, lu->flag.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+  if (dev) {
+    HIPCHK(hipStreamSynchronize(s));
+    // the solve kernels flag every non-finite result
+    if (nanf) { k_fill<<<nblk(n), 256, 0, s>>>(x, (size_t)n, 0.0); HIPCHK(hipStreamSynchronize(s)); return XG_E_NAN; }
+    HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return XG_OK;
+  }
+  HIPCHK(hipMemcpyAsync(x, lu->x.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   bool bad = nanf != 0;
   for (int i = 0; i < n && !bad; ++i) bad = !std::isfinite(x[i]);
   if (bad) { std::fill(x, x + n, 0.0); return XG_E_NAN; }
   return XG_OK;
+}
+
+int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
+  return lu_solve_impl(lu, b, x, transpose, false);
+}
+
+int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose) {
+  return lu_solve_impl(lu, b, x, transpose, true);
 }
 
 int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max) {

@@ -800,9 +800,21 @@ struct DevVec {
   }
 };
 
+// The refactor and the solve are fixed launch sequences for a given plan and
+// set of buffers: each is captured once into a hipGraph and replayed.
+struct MfGraph {
+  const void* key[8] = {};
+  hipGraphExec_t exec = nullptr;
+};
+
 struct MfLu::Impl {
   MfPlan P;
   hipStream_t st = nullptr;
+  std::vector<MfGraph> graphs;
+  ~Impl() {
+    for (auto& g : graphs)
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  }
   DevVec<int> sfirst, m, iptr, idx, cptr, clist, eptr, emap, vo, prow, pcol, bad;
   DevVec<long long> fo, amap;
   DevVec<double> F64, V64, y64, b64;
@@ -947,14 +959,20 @@ int MfLu::levels() const { return (int)impl_->P.lvl_ptr.size() - 1; }
 int MfLu::supernodes() const { return impl_->P.ns; }
 
 template <class T>
-static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int* bad_out) {
+__global__ void k_mf_flag_init(int* bad) {
+  bad[0] = 0x7fffffff;
+  bad[1] = 0;
+}
+
+// the launches of one refactor (no host work: the sequence is captured)
+template <class T>
+static int mf_factor_enqueue(MfLu::Impl& I, const double* vals_dev, int nnz, T* F) {
   const MfPlan& P = I.P;
   hipStream_t st = I.st;
   const long long tot = P.fo[P.ns];
   hipLaunchKernelGGL(k_mf_fill<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, F, tot);
   hipLaunchKernelGGL(k_mf_scatter<T>, dim3((nnz + 255) / 256), dim3(256), 0, st, nnz, vals_dev, I.amap.p, F);
-  const int init[2] = {0x7fffffff, 0};
-  MFCHK(hipMemcpyAsync(I.bad.p, init, sizeof(init), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_mf_flag_init<T>, dim3(1), dim3(1), 0, st, I.bad.p);
   const MfDev d = I.dev();
   T* dbuf = reinterpret_cast<T*>(I.dbuf.p);
   const int nl = (int)P.lvl_ptr.size() - 1;
@@ -975,15 +993,49 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
     }
   }
   MFCHK(hipGetLastError());
+  return XG_OK;
+}
+
+// Replays the graph captured for `key`, capturing it on first use.
+template <class Enqueue>
+static int mf_replay(MfLu::Impl& I, const void* const* key, int nkey, Enqueue enqueue) {
+  MfGraph* g = nullptr;
+  for (auto& c : I.graphs)
+    if (std::equal(key, key + nkey, c.key)) { g = &c; break; }
+  if (!g) {
+    hipGraph_t graph = nullptr;
+    MFCHK(hipStreamBeginCapture(I.st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipError_t e = hipStreamEndCapture(I.st, &graph);
+    if (rc != XG_OK) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+    MFCHK(e);
+    MfGraph ng;
+    std::copy(key, key + nkey, ng.key);
+    e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MFCHK(e);
+    I.graphs.push_back(ng);
+    g = &I.graphs.back();
+  }
+  MFCHK(hipGraphLaunch(g->exec, I.st));
+  return XG_OK;
+}
+
+template <class T>
+static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int* bad_out) {
+  const void* key[8] = {(const void*)1, vals_dev, F, (const void*)(long long)nnz, nullptr, nullptr, nullptr, nullptr};
+  int rc = mf_replay(I, key, 8, [&] { return mf_factor_enqueue<T>(I, vals_dev, nnz, F); });
+  if (rc != XG_OK) return rc;
   int res[2];
-  MFCHK(hipMemcpyAsync(res, I.bad.p, sizeof(res), hipMemcpyDeviceToHost, st));
-  MFCHK(hipStreamSynchronize(st));
+  MFCHK(hipMemcpyAsync(res, I.bad.p, sizeof(res), hipMemcpyDeviceToHost, I.st));
+  MFCHK(hipStreamSynchronize(I.st));
   *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
   return XG_OK;
 }
 
 template <class T>
-static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, const double* b_dev, double* x_dev, int* nan_flag) {
+static int mf_solve_enqueue(MfLu::Impl& I, const T* F, T* V, T* y, double* bp, const double* b_dev, double* x_dev,
+                            int* nan_flag) {
   const MfPlan& P = I.P;
   hipStream_t st = I.st;
   const MfDev d = I.dev();
@@ -993,16 +1045,16 @@ static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, const double* b_dev
   const size_t blk_bytes = 64 * 65 * sizeof(T);
   const bool ldsv = blk_bytes + (size_t)I.big_maxm * sizeof(T) <= 65536;
   const size_t big_lds = blk_bytes + (ldsv ? (size_t)I.big_maxm * sizeof(T) : 0);
-  hipLaunchKernelGGL(k_mf_perm_in<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, I.prow.p, b_dev, I.b64.p);
+  hipLaunchKernelGGL(k_mf_perm_in<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, I.prow.p, b_dev, bp);
   const int nl = (int)P.lvl_ptr.size() - 1;
   for (int l = 0; l < nl; ++l) {
     const int nt = I.tiny_off[l + 1] - I.tiny_off[l], nb = I.big_off[l + 1] - I.big_off[l];
     if (nt)
       hipLaunchKernelGGL(k_mf_fwd_tiny<T>, dim3(nt), dim3(64), 0, st, d, I.tiny_list.p + I.tiny_off[l], F, V,
-                         I.b64.p, y);
+                         bp, y);
     if (nb)
       hipLaunchKernelGGL(k_mf_fwd_big<T>, dim3(nb), dim3(kMfSolveBlock), big_lds, st, d, I.big_list.p + I.big_off[l],
-                         F, V, I.b64.p, y, (int)ldsv);
+                         F, V, bp, y, (int)ldsv);
   }
   for (int l = nl - 1; l >= 0; --l) {
     const int nt = I.tiny_off[l + 1] - I.tiny_off[l], nb = I.big_off[l + 1] - I.big_off[l];
@@ -1015,6 +1067,13 @@ static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, const double* b_dev
   hipLaunchKernelGGL(k_mf_perm_out<T>, dim3((n + 255) / 256), dim3(256), 0, st, n, I.pcol.p, y, x_dev, nan_flag);
   MFCHK(hipGetLastError());
   return XG_OK;
+}
+
+template <class T>
+static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, double* bp, const double* b_dev, double* x_dev,
+                      int* nan_flag) {
+  const void* key[8] = {(const void*)2, F, V, y, bp, b_dev, x_dev, nan_flag};
+  return mf_replay(I, key, 8, [&] { return mf_solve_enqueue<T>(I, F, V, y, bp, b_dev, x_dev, nan_flag); });
 }
 
 int MfLu::factor(const double* vals_dev, int nnz, bool fp32, int* bad) {
@@ -1030,9 +1089,9 @@ int MfLu::solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag) {
   Impl& I = *impl_;
   if (fp32) {
     if (!I.F32.p) return XG_E_ARG;
-    return mf_solve_t<float>(I, I.F32.p, I.V32.p, I.y32.p, b_dev, x_dev, nan_flag);
+    return mf_solve_t<float>(I, I.F32.p, I.V32.p, I.y32.p, I.b64.p, b_dev, x_dev, nan_flag);
   }
-  return mf_solve_t<double>(I, I.F64.p, I.V64.p, I.y64.p, b_dev, x_dev, nan_flag);
+  return mf_solve_t<double>(I, I.F64.p, I.V64.p, I.y64.p, I.b64.p, b_dev, x_dev, nan_flag);
 }
 
 }  // namespace xg

@@ -156,6 +156,30 @@ class MfLu {
   Impl* impl_;
 };
 
+// ---- a drop-in circuit's device state, for the single-circuit transient
+// engine (tran.hip); implemented in kernels.hip.  Every call is enqueued on
+// the circuit's stream and returns without waiting.
+struct CircuitDev {
+  int n = 0, nnz = 0, n_mos = 0;
+  void* stream = nullptr;                 // hipStream_t
+  double* x = nullptr;                    // n + 1 (entry n: the ground dummy, 0)
+  const double *F = nullptr, *Q = nullptr, *FL = nullptr, *QL = nullptr;
+  double* B = nullptr;                    // source stepping scales it in place
+  const double *dFdx = nullptr, *dQdx = nullptr;
+  const int* limited = nullptr;           // set by the last load when a junction limiter acted
+};
+CircuitDev circuit_dev(xg_circuit* c);
+const HostCircuit& circuit_host(const xg_circuit* c);
+// device evaluation at the circuit's x under its solver state, then the
+// vector and matrix gathers (F, Q, B, limiter vectors, dFdx, dQdx)
+int circuit_load(xg_circuit* c);
+// residual and Jacobian (OneStep / NoTimeIntegration) from the last load into
+// device arrays rhs (n) and J (nnz); q0, q2 device arrays (unused at DC)
+int circuit_combine(xg_circuit* c, const double* q0, const double* q2, double* rhs, double* J);
+// MOSFET state / store history: 0 zero both, 1 current <- next (a step or
+// operating point accepted), 2 next <- current (a step rejected)
+int circuit_history(xg_circuit* c, int op);
+
 // learned-predictor network (mlp.hip): forward on a caller stream (a
 // hipStream_t), and its dimensions
 int mlp_forward_on_stream(xg_mlp* net, int rows, const unsigned short* x, float* y, void* stream);
