@@ -57,6 +57,17 @@ def algorithmic_bytes(circuits_list):
     return total
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(circuits_list, sample, tstop, tstep, hmax, threads):
     import ctypes as C
     from oracle.pyoracle import Oracle, lib
@@ -198,20 +209,37 @@ def main():
         traffic = int(per_it * k_iter) if per_it else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": "stored rocprofv3 FETCH_SIZE + WRITE_SIZE per batched iteration, copy-kernel "
+                                  "scaled (profiles/k_batch_step_mc4096_traffic.md), times this launch's iterations"
+                if traffic else None,
                 "kernel": "k_batch_eval + k_batch_step" if k_iter == 1 and tune["split_eval"] else "k_batch_step",
                 "bytes_per_launch": int(byts), "us_per_launch": round(us_launch, 2),
                 "iterations_per_launch": k_iter}
 
     cpu = None
+    single_cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # one GPU's share of the host: 16 threads (the benchmark host runs one
+        # GPU's work per 16 cores, whatever the machine's thread count)
+        host_threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = min(16, host_threads)
         r = cpu_baseline(ckts, args.cpu_sample, args.cpu_tstop, tstep, hmax, threads)
         cpu = {"value": round(r["newton_iters"] / r["seconds"], 1), "unit": "Newton steps/s", "cores": threads,
-               "kind": "port",
+               "kind": "port", "cpu_model": cpu_model(), "host_threads": host_threads,
                "sample": f"first {args.cpu_sample} instances of the batch, .TRAN 0-{args.cpu_tstop * 1e9:g} ns each, "
                          f"one instance per thread at a time ({r['newton_iters']:.0f} Newton iterations in "
                          f"{r['seconds']:.1f} s)"}
         log(f"cpu baseline: {cpu}")
+        # the reference's own serial case: the 101-stage ring, 0-50 ns, one core
+        from oracle.pyoracle import Oracle
+        o1 = Oracle(r1)
+        t1 = time.perf_counter()
+        ro = o1.tran(50e-9, tstep, hmax, record=False)
+        dt1 = time.perf_counter() - t1
+        single_cpu = {"value": round(ro["newton_iters"] / dt1, 1), "unit": "Newton steps/s", "cores": 1,
+                      "kind": "port", "sample": f"101-stage ring, .TRAN 0-50 ns, hmax 10 ps: {ro['newton_iters']} "
+                                                f"Newton iterations in {dt1:.2f} s"}
+        log(f"single-instance cpu: {single_cpu}")
 
     out = {
         "metric": "Newton steps/sec (eval+stamp+LU) per batch; x vs plain FP64 GPU at RELTOL",
@@ -226,6 +254,7 @@ def main():
         "vs_baseline": None,
         "x_vs_single_instance_gpu": round(value / single, 2) if single > 0 else None,
         "single_instance_gpu": round(single, 1),
+        "single_instance_cpu": single_cpu,
         "dtype": "f64",
         "data": "synthetic (seeded Monte Carlo ring oscillators)",
         "config": {"workload": f"{args.instances} Monte Carlo CMOS ring oscillators, stages odd U[11,201], "
