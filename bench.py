@@ -99,6 +99,9 @@ def main():
                     help="workgroup size; with --form split|fused skips the autotuner (instances by LDS need)")
     ap.add_argument("--profile", action="store_true",
                     help="batch leg only (no single-instance run, no CPU baseline): for rocprofv3 runs")
+    ap.add_argument("--phases", action="store_true",
+                    help="batch leg only, with the engine's per-phase device timers on over the timed launches "
+                         "(one instance per launch group): prints microseconds per Newton iteration per phase")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device (the reference kernel for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -153,6 +156,8 @@ def main():
     steps = args.steps if args.steps is not None else max(1, 6400 // k_iter)
     warmup = args.warmup if args.warmup is not None else max(1, 4800 // k_iter)
     b.advance(warmup, graph_chunk=50)
+    if args.phases:
+        b.profile(True)
     c0 = b.counters()
     torch.cuda.synchronize()
     w0 = time.perf_counter()
@@ -169,6 +174,13 @@ def main():
         f"{iters:.0f} Newton iterations, {loads:.0f} loads, done {c1['done']}, failed {c1['failed']}, "
         f"dense {c1['dense_solves'] - c0['dense_solves']:.0f}")
 
+    if args.phases:
+        ph = b.profile(False)
+        it = max(ph.pop("iterations"), 1)
+        per = {k: round(v / it, 3) for k, v in ph.items()}
+        print(json.dumps({"phases_us_per_iteration": per, "iterations_timed": it,
+                          "sum_us": round(sum(per.values()), 3), "launches": steps, "tune": tune}), flush=True)
+        return
     if args.profile:
         print(json.dumps({"profile_run": True, "launches": steps, "newton_iterations": int(iters),
                           "events_ms": round(dev_s * 1e3, 3), "tune": tune}), flush=True)

@@ -138,6 +138,25 @@ def grid_probe(c, o, extra_nodes=()):
     return w
 
 
+def _pwl_source_sums(c, t):
+    """(S_B, sum |I|) at times t for a circuit whose current sources are all
+    PWL and grounded on one side: the node-row sum of the sources' B entries
+    (-I for a current drawn from a node to ground) and the magnitude it is
+    summed from; the waveforms evaluated by linear interpolation between
+    their points, held constant outside them (the PWL definition)."""
+    from xyce_amd import circuits as C
+    sb = np.zeros(len(t))
+    ssum = np.zeros(len(t))
+    for a, b, wv in c.isrc:
+        wt, p = c.waves[wv]
+        assert wt == C.W_PWL
+        cur = np.interp(t, np.asarray(p[0::2]), np.asarray(p[1::2]))
+        sgn = (-1.0 if a != 0 else 0.0) + (1.0 if b != 0 else 0.0)
+        sb += sgn * cur
+        ssum += np.abs(cur)
+    return sb, ssum
+
+
 def grid_charge_balance(c, o, r, tstep, tstop):
     """Largest defect of the summed node equations over the accepted steps
     of a run recorded with ``grid_probe`` rows, relative to what floating
@@ -150,16 +169,20 @@ def grid_charge_balance(c, o, r, tstep, tstop):
     of the conductance terms they sum (S_abs), and the probes are compensated
     sums, exact to a few units of their last place.  A converged linear network stays
     within it (value <= 1); a wrong load, solve or integration coefficient
-    does not."""
+    does not.  With ``o`` None the source sum is evaluated in numpy (PWL
+    sources only), for runs too long for one oracle call per point."""
     t, y, order = r["t"], r["x"], r["order"]
     sq, sf, sabs = y[:, 0], y[:, 1], y[:, 2]
-    sb = np.zeros(len(t))
-    ssum = np.zeros(len(t))
-    for a, b, wv in c.isrc:
-        cur = np.array([o.wave(wv, tk, tstep, tstop) for tk in t])
-        sgn = (-1.0 if a != 0 else 0.0) + (1.0 if b != 0 else 0.0)
-        sb += sgn * cur
-        ssum += np.abs(cur)
+    if o is None:
+        sb, ssum = _pwl_source_sums(c, t)
+    else:
+        sb = np.zeros(len(t))
+        ssum = np.zeros(len(t))
+        for a, b, wv in c.isrc:
+            cur = np.array([o.wave(wv, tk, tstep, tstop) for tk in t])
+            sgn = (-1.0 if a != 0 else 0.0) + (1.0 if b != 0 else 0.0)
+            sb += sgn * cur
+            ssum += np.abs(cur)
     g = sf - sb
     eps = np.finfo(float).eps
     worst = 0.0

@@ -95,9 +95,10 @@ def test_bad_arguments_are_rejected(gpu):
 
 def test_failing_instance_does_not_disturb_the_others(gpu):
     """A gate array whose transient stalls (step size below the minimum, as
-    in the oracle) sits in the same batch as ring oscillators; the rings'
+    in the oracle: without diffusion capacitance a NAND2's internal node
+    carries no charge) sits in the same batch as ring oscillators; the rings'
     trajectories are bit-identical to their runs alone."""
-    stall = C.gate_dag(80, seed=8, frac_nand=0.8)
+    stall = C.gate_dag(80, seed=8, frac_nand=0.8, diffusion_caps=False)
     rings = [C.ring(9, seed=21, mc_seed=21), C.ring(15, seed=22, mc_seed=22)]
     tstop = 0.4e-9
     b = X.Batch([rings[0], stall, rings[1]], tstop, 1e-12, 1e-11, record_max=4000)

@@ -297,18 +297,35 @@ def power_grid(nx: int = 512, ny: int = 512, seed: int = 2, pad_pitch: int = 64,
     return c
 
 
-def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5, window: int = 64) -> Circuit:
+def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5, window: int = 64,
+             diffusion_caps: bool = True) -> Circuit:
     """Level-1 INV/NAND2 array wired as a random DAG: gate k draws its inputs
     from the outputs of the last ``window`` gates that still have fan-out
     left (or from primary inputs driven by PULSE sources), fan-out per gate
     ~ U{1..4}; each gate output carries a wire cap ~ U[0.5,5] fF.  The window
     models placement locality: with unbounded random wiring the graph is an
-    expander and any direct factorisation fills in almost completely."""
+    expander and any direct factorisation fills in almost completely.
+
+    ``diffusion_caps``: every source and drain diffusion is W x 1 um with a
+    junction capacitance (CJ 2e-4 F/m^2, CJSW 1e-10 F/m).  Without it the
+    node between a NAND2's two series NMOS carries no charge at all (these
+    cards have no TOX, so no Meyer or overlap capacitance): when both devices
+    are off it is held only by ~3e-12 S of junction leakage and gmin, Newton
+    moves it by hundreds of volts (fetlim and limvds clamp it without raising
+    the limiter flag, which only pnjlim sets), the uniform local-error weight
+    rejects every step and the step size falls below its minimum within a few
+    picoseconds, in the oracle as on the GPU."""
     c = Circuit()
     rng = np.random.default_rng(seed)
     vdd = c.node("vdd")
-    mn = c.add_model(model_card(+1, **NMOS_RING))
-    mp = c.add_model(model_card(-1, **PMOS_RING))
+    jcap = dict(CJ=2e-4, CJSW=1e-10) if diffusion_caps else {}
+    mn = c.add_model(model_card(+1, **NMOS_RING, **jcap))
+    mp = c.add_model(model_card(-1, **PMOS_RING, **jcap))
+
+    def dev(w):
+        if not diffusion_caps:
+            return inst_card(W=w, L=0.5e-6)
+        return inst_card(W=w, L=0.5e-6, AD=w * 1e-6, AS=w * 1e-6, PD=2 * w + 2e-6, PS=2 * w + 2e-6)
     c.add_vsrc(vdd, 0, c.add_wave(W_DC, [VDD]))
     n_pi = max(4, n_gates // 1000)
     pis = []
@@ -337,14 +354,14 @@ def gate_dag(n_gates: int = 250_000, seed: int = 4, frac_nand: float = 0.5, wind
                 pool.pop(); pool_left.pop()
         out = c.node(f"g{k}")
         if nin == 1:
-            c.add_mos(out, ins[0], 0, 0, mn, inst_card(W=2e-6, L=0.5e-6))
-            c.add_mos(out, ins[0], vdd, vdd, mp, inst_card(W=4e-6, L=0.5e-6))
+            c.add_mos(out, ins[0], 0, 0, mn, dev(2e-6))
+            c.add_mos(out, ins[0], vdd, vdd, mp, dev(4e-6))
         else:
             mid = c.node(f"m{k}")
-            c.add_mos(out, ins[0], mid, 0, mn, inst_card(W=4e-6, L=0.5e-6))
-            c.add_mos(mid, ins[1], 0, 0, mn, inst_card(W=4e-6, L=0.5e-6))
-            c.add_mos(out, ins[0], vdd, vdd, mp, inst_card(W=4e-6, L=0.5e-6))
-            c.add_mos(out, ins[1], vdd, vdd, mp, inst_card(W=4e-6, L=0.5e-6))
+            c.add_mos(out, ins[0], mid, 0, mn, dev(4e-6))
+            c.add_mos(mid, ins[1], 0, 0, mn, dev(4e-6))
+            c.add_mos(out, ins[0], vdd, vdd, mp, dev(4e-6))
+            c.add_mos(out, ins[1], vdd, vdd, mp, dev(4e-6))
         c.add_cap(out, 0, wire[k])
         outs.append(out)
         pool.append(out)
