@@ -172,6 +172,18 @@ int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
  * semantics; each call returns when its outputs are complete. */
 int xg_lu_numeric_dev(xg_lu* lu, const double* vals);
 int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): xg_lu_solve_dev with the last FP64 factors, allocating
+ * nothing.  b and x are device arrays of n entries.  The solve's work
+ * vectors and NaN flag live in one static 32 MiB work area per GPU (4 Mi
+ * doubles; the level-scheduled method needs n + 32 of them, the multifrontal
+ * one its front vectors plus 2n + 32); a system that needs more returns
+ * XG_E_ARG before any device work, as does a handle whose current factors are
+ * FP32 (xg_lu_set_precision 1: its refinement may refactor in FP64).  Every
+ * call on one GPU uses the same work area: calls that overlap on a GPU race.
+ * NaN -> XG_E_NAN with x zeroed, as xg_lu_solve. */
+int xg_lu_solve_noalloc(xg_lu* lu, const double* b, double* x);
 /* info[0..5] = factor entries (nnz(L+U), or front elements when
  * multifrontal), factor levels, solve levels, flops per factorization,
  * method (1 level-scheduled, 2 multifrontal), supernodes */

@@ -40,7 +40,7 @@ EXPORTS = [
     "xg_update_state", "xg_load_vectors", "xg_load_matrices", "xg_combine",
     "xg_update_state_dev", "xg_load_vectors_dev", "xg_load_matrices_dev", "xg_combine_dev",
     "xg_lu_symbolic", "xg_lu_symbolic_ex", "xg_lu_destroy", "xg_lu_numeric", "xg_lu_solve", "xg_lu_info",
-    "xg_lu_numeric_dev", "xg_lu_solve_dev",
+    "xg_lu_numeric_dev", "xg_lu_solve_dev", "xg_lu_solve_noalloc",
     "xg_lu_set_precision", "xg_lu_precision_stats",
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
     "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_group_info", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "mos1_device.h"
@@ -530,6 +531,33 @@ __global__ void k_soa_to_sta(int n_mos, int nslots, const int* off0, const doubl
 
 inline unsigned nblk(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// ---- the work area of xg_lu_solve_noalloc: one file-scope array per GPU,
+// held from module load until the process ends.  Host code never uses the
+// name `scratch` as an address (it is a host-side stand-in there): the
+// device address is looked up once per device and kept in scratch_dev.
+constexpr size_t kScratchDoubles = (size_t)4 << 20;   // 32 MiB
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static std::mutex scratch_mu;
+static double* scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (hipGetSymbolAddress,
+// once per device, with that device current).
+static int scratch_address(double** out) {
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return XG_E_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return XG_E_ARG;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return XG_E_HIP;
+    scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = scratch_dev[dev];
+  return XG_OK;
+}
+
 }  // namespace xg
 
 // ---------------------------------------------------------------------------
@@ -561,6 +589,9 @@ struct xg_lu {
   // method: 1 = level-scheduled entry LU (small systems), 2 = multifrontal
   int method = 1;
   std::unique_ptr<MfLu> mfp;
+  // device address of the static work area on this handle's device (set up
+  // with the handle; null if the lookup failed)
+  double* scratch_work = nullptr;
 
   // symbolic analysis for the chosen (or automatically picked) method
   int analyse(const double* values, int want_method) {
@@ -701,6 +732,21 @@ struct xg_lu {
     if (method == 2) return mfp->solve(db, dx, fp32, flag.p + 1);
     return fp32 ? solve_device<float>(lu32.p, y32.p, db, dx, flag.p + 1)
                 : solve_device<double>(lu.p, y.p, db, dx, flag.p + 1);
+  }
+  // FP64-factor solve whose work vectors and flag are the caller's device
+  // memory `work` (work_doubles() doubles, 256-byte aligned) instead of the
+  // handle's buffers; *nan_flag is the first word of the work area
+  size_t work_doubles() const {
+    const size_t flag_d = 32;                       // one int, padded to 256 bytes
+    if (method == 2) return flag_d + mfp->solve_work_doubles();
+    return flag_d + (((size_t)s.n + 31) & ~(size_t)31);
+  }
+  int solve_with_work(const double* db, double* dx, double* work) {
+    int* nan_flag = reinterpret_cast<int*>(work);
+    double* w = work + 32;
+    k_fill<<<1, 1, 0, stream>>>(nan_flag, 1, 0);
+    if (method == 2) return mfp->solve_ext(db, dx, nan_flag, w);
+    return solve_device<double>(lu.p, w, db, dx, nan_flag);
   }
   // r = b - A x; returns max|r| / max|b| (0/0 -> 0)
   int residual(const double* db, const double* dx, double* rel) {
@@ -1224,6 +1270,8 @@ int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double*
   lu->own_stream = true;
   int st = lu->upload();
   if (st != XG_OK) return st;
+  // the no-allocation solve's work area on this device (looked up once per device)
+  if (scratch_address(&lu->scratch_work) != XG_OK) lu->scratch_work = nullptr;
   *out = lu.release();
   return XG_OK;
 }
@@ -1331,6 +1379,27 @@ int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose) {
 
 int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose) {
   return lu_solve_impl(lu, b, x, transpose, true);
+}
+
+int xg_lu_solve_noalloc(xg_lu* lu, const double* b, double* x) {
+  if (!lu || !b || !x) return XG_E_ARG;
+  if (!lu->scratch_work) return XG_E_HIP;
+  // FP32 factors are refined, and refinement can fall back to an FP64
+  // factorisation: not a solve with the current factors
+  if (lu->mode == 1 && !lu->have64) return XG_E_ARG;
+  if (lu->work_doubles() > kScratchDoubles) return XG_E_ARG;
+  hipStream_t s = lu->stream;
+  int st = lu->solve_with_work(b, x, lu->scratch_work);
+  if (st != XG_OK) return st;
+  int nanf = 0;
+  HIPCHK(hipMemcpyAsync(&nanf, lu->scratch_work, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (nanf) {
+    k_fill<<<nblk(lu->s.n), 256, 0, s>>>(x, (size_t)lu->s.n, 0.0);
+    HIPCHK(hipStreamSynchronize(s));
+    return XG_E_NAN;
+  }
+  return XG_OK;
 }
 
 int xg_lu_set_precision(xg_lu* lu, int precision, double ir_tol, int ir_max) {

@@ -1094,4 +1094,20 @@ int MfLu::solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag) {
   return mf_solve_t<double>(I, I.F64.p, I.V64.p, I.y64.p, I.b64.p, b_dev, x_dev, nan_flag);
 }
 
+static size_t round32(size_t v) { return (v + 31) & ~(size_t)31; }
+
+size_t MfLu::solve_work_doubles() const {
+  const MfPlan& P = impl_->P;
+  return round32((size_t)P.vo[P.ns]) + 2 * round32((size_t)P.n);
+}
+
+int MfLu::solve_ext(const double* b_dev, double* x_dev, int* nan_flag, double* work) {
+  Impl& I = *impl_;
+  const MfPlan& P = I.P;
+  double* V = work;
+  double* y = V + round32((size_t)P.vo[P.ns]);
+  double* bp = y + round32((size_t)P.n);
+  return mf_solve_t<double>(I, I.F64.p, V, y, bp, b_dev, x_dev, nan_flag);
+}
+
 }  // namespace xg

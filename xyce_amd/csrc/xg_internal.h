@@ -147,6 +147,11 @@ class MfLu {
   // *bad = 0, or (first failing pivot column of B) + 1
   int factor(const double* vals_dev, int nnz, bool fp32, int* bad);
   int solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag);
+  // FP64 solve whose front vectors, work vector and permuted right-hand side
+  // are the caller's device memory `work` (solve_work_doubles() doubles,
+  // 256-byte aligned) instead of the plan's buffers
+  size_t solve_work_doubles() const;
+  int solve_ext(const double* b_dev, double* x_dev, int* nan_flag, double* work);
   double flops() const;
   long long front_elements() const;
   int levels() const;
