@@ -32,7 +32,7 @@ extern "C" {
 #define XG_E_NOGPU (-10)     /* no HIP device */
 
 #define XG_PRED_WIN 256      /* learned predictor: unknowns per window */
-#define XG_PRED_FEAT 1344    /* learned predictor: features per window */
+#define XG_PRED_FEAT 1600    /* learned predictor: features per window */
 
 /* Device types in netlist order (dev_type[]). */
 #define XG_DEV_MOS1 0
@@ -161,10 +161,15 @@ int xg_lu_symbolic(int n, const int* rowptr, const int* colind, const double* va
 int xg_lu_symbolic_ex(int n, const int* rowptr, const int* colind, const double* values, int method, xg_lu** out);
 int xg_lu_destroy(xg_lu* lu);
 /* Numeric refactor with the symbolic pivot order (KLU NumericFactorization,
- * :363).  A pivot that is zero, non-finite or fails the threshold test makes
- * it choose new pivots from these values and refactor, as Xyce's KLU
- * re-pivots at every factorisation (repivot_, N_LAS_AmesosSolver.C:87,318);
- * returns >0 (pivot+1) only if that fails too (a singular matrix). */
+ * :363).  A pivot that is zero or non-finite, or a multiplier beyond KLU's
+ * threshold (|l| > 1/0.001), makes it choose new pivots from these values and
+ * refactor, as Xyce's KLU re-pivots at every factorisation (repivot_,
+ * N_LAS_AmesosSolver.C:87,318); returns >0 (pivot+1) only if the new pivots
+ * meet a zero pivot too (a singular matrix).  If the new pivots still leave
+ * multipliers beyond the threshold, those factors are kept and the solves
+ * refine x against A in FP64 while the residual falls (at most ir_max
+ * corrections, to ir_tol); from then on the handle keeps such factors
+ * without choosing pivots again. */
 int xg_lu_numeric(xg_lu* lu, const double* vals);
 /* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
@@ -180,7 +185,8 @@ int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose);
  * doubles; the level-scheduled method needs n + 32 of them, the multifrontal
  * one its front vectors plus 2n + 32); a system that needs more returns
  * XG_E_ARG before any device work, as does a handle whose current factors are
- * FP32 (xg_lu_set_precision 1: its refinement may refactor in FP64).  Every
+ * refined by its solves (FP32 factors, xg_lu_set_precision 1, or factors with
+ * multipliers beyond KLU's threshold).  Every
  * call on one GPU uses the same work area: calls that overlap on a GPU race.
  * NaN -> XG_E_NAN with x zeroed, as xg_lu_solve. */
 int xg_lu_solve_noalloc(xg_lu* lu, const double* b, double* x);
@@ -233,10 +239,18 @@ int xg_tran_set_record(xg_tran* t, int k, const double* w, int n_sel, const int*
  * steps or the end of the window.  stats[0..13] = accepted steps, rejected
  * steps, Newton iterations, loads, time reached, status (0 ok, -100 no DC
  * point, -101 15 rejections in a row, -102 step below 1e-18 s), finished,
- * DC Newton iterations, DC stage (0 Newton, 1 gmin, 2 source stepping),
+ * DC Newton iterations, DC stage (0 Newton, 1 gmin, 2 source stepping,
+ * -1 none: UIC),
  * seconds, singular factorisations, LU refinement corrections, LU solves
  * finished by FP64 factors, points recorded. */
 int xg_tran_run(xg_tran* t, double t_until, long long max_steps, double* stats);
+/* on != 0: .TRAN ... UIC (NOOP; N_ANP_Transient.C:879-1000): no operating
+ * point, the run starts from x = 0 with the devices' state of one transient
+ * load there.  Before the first run only. */
+int xg_tran_set_uic(xg_tran* t, int on);
+/* on != 0: one line on stderr per Newton solve of the DC operating point
+ * (the continuation stages of a large circuit can take minutes). */
+int xg_tran_set_progress(xg_tran* t, int on);
 /* Recorded points: t[max], the integration order of the step that ended
  * there (0 = DC point), y[max * (k + n_sel, or n)]; returns the count copied. */
 int xg_tran_points(xg_tran* t, int max_points, double* times, int* order, double* y);
@@ -311,8 +325,9 @@ int xg_batch_dc_stage(xg_batch* b, int inst, int* stage);
 int xg_batch_profile(xg_batch* b, int enable, double* phase_us);
 /* Autotuner: times the launch forms (split / fused with several iterations
  * per launch) x workgroup sizes x instance grouping on the real batch and
- * keeps the fastest; for the fused form with LDS-sized groups it then picks
- * each group's workgroup size and the iterations per launch (4, 8, 16, 32, 64).
+ * keeps the fastest (workgroups of 128, 256 or 512 threads); for the fused
+ * form with LDS-sized groups it then picks each group's workgroup size (64 to
+ * 512 threads) and the iterations per launch (4, 8, 16, 32, 64).
  * result[0..9] = block size, grouping (1 = by LDS need), us per batched
  * iteration, split_eval, iterations per launch, workgroup size of groups
  * 0-3, groups side by side (1) or one after another (0). */
@@ -322,20 +337,23 @@ int xg_batch_autotune(xg_batch* b, double* result);
  * mode 0 off; 1 collect training pairs, up to pair_capacity accepted steps
  * per instance; 2 serve `net` (in_dim XG_PRED_FEAT, out_dim XG_PRED_WIN):
  * after each launch the network proposes the Newton initial guess of every
- * instance that starts a step, x = x_poly + s * y per window of
- * XG_PRED_WIN unknowns (features: see xg_batch_pairs).  A proposal with a
- * non-finite or |y| > 8 entry is not used, and a step whose Newton solve
+ * instance that starts a step, x = x_poly + w * y per unknown, w its Newton
+ * weight (features: see xg_batch_pairs).  A proposal with a non-finite or
+ * |y| > 512 entry is not used, and a step whose Newton solve
  * fails from a proposal is retried from the polynomial predictor, so the
  * step control and tolerances are those of the run without it. */
 int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity);
-/* Collected pairs (mode 1), one record per window of an accepted step:
- * features x [XG_PRED_FEAT] bf16 = (x_n - x_{n-1}, x_{n-1} - x_{n-2},
- * x_{n-2} - x_{n-3}, x_poly - x_n, x_cubic - x_poly) / s (XG_PRED_WIN each;
- * s = the window's largest |entry| of the first four + 1e-3 max|x_n| +
- * 1e-6), then
- * h_{n+1}/h_n, h_n/h_{n-1}, h_{n-1}/h_{n-2}, order, log10(s)/10, 1, zeros;
- * target t [XG_PRED_WIN] = (x_{n+1} - x_poly) / s; mask m = 1 where the
- * window row exists.  Copies up to max_records into the device buffers
+/* Collected pairs (mode 1), one record per window of XG_PRED_WIN unknowns
+ * of an accepted step, in units of each unknown's Newton weight
+ * w = 1e-2 max(|x_n|, |x_poly|) + 1e-6 (the transient Newton weight of the
+ * first iteration, N_NLS_DampedNewton.C:297-337): features x [XG_PRED_FEAT]
+ * bf16 = g((x_n - x_{n-1}) / w), g((x_{n-1} - x_{n-2}) / w),
+ * g((x_{n-2} - x_{n-3}) / w), g((x_poly - x_n) / w), g((x_cubic - x_poly) / w),
+ * x_n (XG_PRED_WIN each; g(v) = sign(v) log2(1 + |v|), x_cubic the cubic
+ * through x_n..x_{n-3} at t_{n+1}), then h_{n+1}/h_n, h_n/h_{n-1},
+ * h_{n-1}/h_{n-2}, order, (log10 h_{n+1} + 12) / 4, 1, zeros;
+ * target t [XG_PRED_WIN] = (x_{n+1} - x_poly) / w clipped to +-256; mask
+ * m = 1 where the window row exists.  Copies up to max_records into the device buffers
  * (NULL: count only); *count = records copied (or available). */
 int xg_batch_pairs(xg_batch* b, long long max_records, unsigned short* x_dev, float* t_dev, float* m_dev,
                    long long* count);

@@ -200,6 +200,7 @@ struct Work {
   bool dc_initjct = true;
   int dc_strategy = 0;               // 0 plain Newton, 1 gmin stepping, 2 source stepping (the last DC solve)
   int dc_first_stage = 0;            // tests: start the strategy at this stage (0 = the reference's order)
+  int uic = 0;                       // .TRAN ... UIC: no operating point, the run starts from x = 0
   std::vector<int> vrow, vdiag;      // node-voltage rows and their diagonal CSR slots
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
@@ -646,8 +647,21 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
   w.tstep = tstep; w.tstop = tstop;
   auto t0 = std::chrono::steady_clock::now();
   long dc_iters = 0;
-  int st = dcop(w, &dc_iters);
-  if (st <= 0) { R.status = -100; return R; }
+  if (w.uic) {
+    // .TRAN ... UIC / NOOP (N_ANP_Transient.C:879-1000): no operating point;
+    // the solution starts at zero (no .IC here) and one load there
+    // (initializeProblem, loadRHS) gives the devices' state and F, Q, B from
+    // which the histories are set as after an operating point
+    std::fill(w.x.begin(), w.x.end(), 0.0);
+    std::fill(w.staCur.begin(), w.staCur.end(), 0.0); std::fill(w.stoCur.begin(), w.stoCur.end(), 0.0);
+    std::fill(w.staNext.begin(), w.staNext.end(), 0.0); std::fill(w.stoNext.begin(), w.stoNext.end(), 0.0);
+    SolverState ss; ss.dcop = 0; ss.tran = 1; ss.newtonIter = 0; ss.time = 0.0; ss.h = tstep; ss.method = w.method;
+    load_rhs(w, ss);
+    w.dc_strategy = -1;
+  } else {
+    int st = dcop(w, &dc_iters);
+    if (st <= 0) { R.status = -100; return R; }
+  }
   // tranop -> transient
   w.staCur = w.staNext; w.stoCur = w.stoNext;
   std::vector<double> xH0(w.x.begin(), w.x.begin() + n), xH1(n, 0.0), xH2(n, 0.0);
@@ -1083,6 +1097,12 @@ int orc_dc_strategy(OrcHandle* h) { return h->w->dc_strategy; }
 int orc_set_dc_first_stage(OrcHandle* h, int stage) {
   if (stage < 0 || stage > 2) return -1;
   h->w->dc_first_stage = stage;
+  return 0;
+}
+
+// Later orc_tran calls skip the operating point (.TRAN ... UIC) when on != 0.
+int orc_set_uic(OrcHandle* h, int on) {
+  h->w->uic = on != 0;
   return 0;
 }
 

@@ -47,6 +47,8 @@ def lib():
         L.orc_dc_strategy.restype = C.c_int
         L.orc_set_method.argtypes = [P, C.c_int]
         L.orc_set_method.restype = C.c_int
+        L.orc_set_uic.argtypes = [P, C.c_int]
+        L.orc_set_uic.restype = C.c_int
         for nm in ("orc_limvds", "orc_fetlim", "orc_qmeyer", "orc_pnjlim"):
             getattr(L, nm).restype = C.c_int
         _LIB = L
@@ -158,7 +160,13 @@ class Oracle:
     def dc_strategy(self):
         """Strategy of the last DC operating point: 'newton', 'gmin' or
         'source' (stepping)."""
-        return ("newton", "gmin", "source")[lib().orc_dc_strategy(self.h)]
+        k = lib().orc_dc_strategy(self.h)
+        return ("newton", "gmin", "source")[k] if k >= 0 else "none"
+
+    def set_uic(self, on=True):
+        """Later transients skip the operating point (.TRAN ... UIC): they
+        start from x = 0 with the devices' state of one load there."""
+        assert lib().orc_set_uic(self.h, int(bool(on))) == 0
 
     def set_dc_first_stage(self, stage):
         """Start the DC strategy at 'newton' (the reference's order), 'gmin'

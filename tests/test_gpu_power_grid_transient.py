@@ -136,3 +136,46 @@ def test_power_grid_512_transient_full_size(gpu):
     bal = grid_charge_balance(c, drv, r, 1e-12, tstop_circuit)
     print(f"charge balance: worst defect {bal:.3g} of the round-off allowance")
     assert bal <= 1.0
+
+
+def test_power_grid_512_on_the_device_resident_engine(gpu):
+    """The config-2 workload on the single-circuit engine (every vector in
+    HBM, the multifrontal LU, no host round trips of vectors): the 512x512
+    mesh with its 4096 PWL sinks, .TRAN with a 1 ps step, by default over
+    0-50 ps (about 1000 steps); with XG_GRID_WINDOW=2e-9 in the environment,
+    over the config's whole 0-2 ns window (about 61k steps, breakpoints at the
+    sinks' 16,384 corners).  At every accepted step the summed node equations
+    (charge balance) hold to round-off, the pads stay at VDD and every node
+    within the supply droop; a progress line is printed every 100 ps."""
+    import os
+    t_window = float(os.environ.get("XG_GRID_WINDOW", "50e-12"))
+    nx = 512
+    c = C.power_grid(nx, nx, seed=2, pad_pitch=64, n_sinks=4096, tstop=2e-9)
+    probe = Oracle(c)
+    pad = c.vsrc[0][0]
+    center = 1 + (nx // 2) * nx + nx // 2
+    w = grid_probe(c, probe, extra_nodes=[pad, center] + [a for a, _, _ in c.isrc[:4]])
+    del probe
+    e = X.Transient(c, 2e-9, 1e-12, 0.0)
+    e.set_record(400_000, w)
+    t0 = time.perf_counter()
+    st = None
+    t_next = 0.0
+    while True:
+        t_next = min(t_window, t_next + 100e-12)
+        st = e.run(t_until=t_next)
+        print(f"  t = {st['t'] * 1e12:.1f} ps: {st['steps']} steps, {st['rejects']} rejected, {st['newton_iters']} "
+              f"Newton iterations, {time.perf_counter() - t0:.1f} s", flush=True)
+        if st["status"] != 0 or st["t"] >= t_window * (1 - 1e-12) or st["finished"]:
+            break
+    wall = time.perf_counter() - t0
+    assert st["status"] == 0, st
+    t, order, y = e.points(st["points"])
+    print(f"512x512 grid on the engine, 0-{t[-1] * 1e12:.0f} ps: {st['steps']} steps, {st['newton_iters']} Newton "
+          f"iterations, {wall:.1f} s ({1e3 * wall / max(st['steps'], 1):.2f} ms per step)")
+    assert len(t) == st["steps"] + 1
+    assert np.max(np.abs(y[:, 3] - VDD)) <= 1e-9
+    assert np.all(y[:, 4:] <= VDD + 1e-9) and np.all(y[:, 4:] >= VDD - 0.2)
+    bal = grid_charge_balance(c, None, dict(t=t, x=y, order=order), 1e-12, 2e-9)
+    print(f"charge balance over {len(t)} points: worst defect {bal:.3g} of the round-off allowance")
+    assert bal <= 1.0

@@ -9,8 +9,9 @@ Checks:
   RELTOL of the FP64 oracle, accepted step counts within 2 % of the run
   without the network;
 * what the network gains: Newton iterations per accepted step with and
-  without it (printed; the trained network must not need more iterations
-  than the polynomial predictor).
+  without it, which must fall strictly on both families (ring oscillators,
+  about 2.07 iterations per step with the polynomial predictor, and the
+  power-grid family, about 1.10).
 """
 import numpy as np
 import pytest
@@ -39,6 +40,12 @@ def test_pairs_are_collected(gpu):
     assert c["pairs"] >= c["steps"] - 8 * 4
     assert torch.isfinite(x.float()).all() and torch.isfinite(t).all()
     assert float(m.sum(1).min()) >= 13 and float(m.sum(1).max()) <= 23
+    # targets are in Newton weights, clipped to +-256; the padding rows are zero
+    assert float(t.abs().max()) <= 256.0
+    assert float((t * (1 - m)).abs().max()) == 0.0
+    # the sixth feature plane is x_n itself (the supply node sits at 1.8 V)
+    xn = x[:, 5 * X.PRED_WIN:6 * X.PRED_WIN].float()
+    assert float(xn.max()) == pytest.approx(1.8, abs=0.01)
 
 
 def _grids(n, seed0):
@@ -98,17 +105,16 @@ def test_trained_predictor_on_power_grids(gpu):
     assert hist["loss"][-1] < hist["poly_loss"]
     base, learned, ipb, ipl = _evaluate(net, _grids(16, 5000), 2e-9, 0.0)
     assert abs(learned["steps"] - base["steps"]) <= 0.02 * base["steps"] + 16
-    # the selector keeps the cost of trying the network within 2 %
-    assert ipl <= ipb * 1.02
+    assert ipl < ipb
 
 
 def test_trained_predictor_on_rings(gpu):
     """Ring oscillators (nonlinear)."""
-    x, t, m, c = P.collect_pairs(_rings(256, 100), 2e-9, 1e-12, 1e-11, capacity=4000)
+    x, t, m, c = P.collect_pairs(_rings(1024, 100), 2e-9, 1e-12, 1e-11, capacity=1200)
     net = P.new_net(max_rows=2048, seed=5)
-    hist = P.train(net, x, t, m, epochs=3, batch=1024, lr=1e-3)
+    hist = P.train(net, x, t, m, epochs=4, batch=2048, lr=1e-3)
     print(f"{x.shape[0]} pairs; loss {hist['loss']} (polynomial predictor {hist['poly_loss']:.3e})")
     assert hist["loss"][-1] < hist["poly_loss"]
     base, learned, ipb, ipl = _evaluate(net, _rings(16, 900), 2e-9, 1e-11, oscillator=True, check=(0, 7))
     assert abs(learned["steps"] - base["steps"]) <= 0.02 * base["steps"] + 16
-    assert ipl <= ipb * 1.01
+    assert ipl < ipb

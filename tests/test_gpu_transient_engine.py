@@ -13,7 +13,9 @@ all-CPU oracle on circuits it can run:
   within 1e-9 V, and the summed node equations (charge balance) met at every
   accepted step, recorded through the engine's probe rows;
 * a gate array whose DC point needs the DC strategy: the same strategy stage
-  and operating point.
+  and operating point;
+* the same gate array with UIC (no operating point, the run starts from
+  x = 0): the oracle's step and Newton counts and its early waveform.
 """
 import numpy as np
 import pytest
@@ -95,4 +97,24 @@ def test_engine_gate_array_dc_strategy(gpu):
     assert ("newton", "gmin", "source")[st["dc_stage"]] == o.dc_strategy()
     np.testing.assert_allclose(xg[0], r["x"][0], rtol=0, atol=1e-6)
     assert abs(st["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
+    assert _early(tg, xg, r, 0.1e-9) <= 1e-3 * 1.8 + 1e-6
+
+
+def test_engine_gate_array_uic(gpu):
+    c = C.gate_dag(400, seed=1)
+    o = Oracle(c)
+    o.set_uic()
+    r = o.tran(0.3e-9, 1e-12, 1e-11)
+    assert r["status"] == 0
+    e = X.Transient(c, 0.3e-9, 1e-12, 1e-11)
+    e.set_uic()
+    e.set_record(len(r["t"]) + 2000)
+    st = e.run()
+    assert st["finished"] == 1 and st["status"] == 0, st
+    tg, og, xg = e.points(st["points"])
+    print(f"gate array 400, UIC: {st}; oracle steps {r['steps']}, Newton {r['newton_iters']}")
+    assert st["dc_stage"] == -1 and st["dc_iters"] == 0
+    assert np.all(xg[0] == 0.0)
+    assert abs(st["steps"] - r["steps"]) <= 0.02 * r["steps"] + 2
+    assert abs(st["newton_iters"] - r["newton_iters"]) <= 0.02 * r["newton_iters"] + 2
     assert _early(tg, xg, r, 0.1e-9) <= 1e-3 * 1.8 + 1e-6

@@ -145,3 +145,25 @@ def test_integration_method_option(line, method):
 def test_unknown_integration_method_is_rejected():
     with pytest.raises(NetlistError):
         parse_netlist("t\nR1 a 0 1k\n.OPTIONS TIMEINT METHOD=BDF5\n.END\n")
+
+
+@pytest.mark.parametrize("line", [".OPTIONS DEVICE VOLTLIM=0", ".OPTIONS DEVICE GMIN=1e-9", ".OPTIONS TIMEINT RELTOL=1e-4",
+                                  ".OPTIONS NONLIN MAXSTEP=50", ".OPTIONS LINSOL TYPE=SUPERLU", ".OPTIONS OUTPUT INITIAL_INTERVAL=1p",
+                                  ".OPTIONS"])
+def test_options_the_path_would_not_honour_are_refused(line):
+    with pytest.raises(NetlistError):
+        parse_netlist("t\nR1 a 0 1k\nV1 a 0 DC 1\n" + line + "\n.TRAN 1p 1n\n.END\n")
+
+
+def test_default_options_may_be_restated():
+    _, an = parse_netlist("t\nR1 a 0 1k\nV1 a 0 DC 1\n.OPTIONS DEVICE VOLTLIM=1 GMIN=1e-12\n"
+                          ".OPTIONS LINSOL TYPE=klu\n.OPTIONS TIMEINT METHOD=GEAR ABSTOL=1u RELTOL=1e-3\n.TRAN 1p 1n\n.END\n")
+    assert an["method"] == "gear"
+
+
+@pytest.mark.parametrize("tail", ["OFF", "L=0.5u W=2u OFF", "IC=1,2,0"])
+def test_mosfet_off_and_initial_conditions_are_refused(tail):
+    text = ("t\nVDD vdd 0 DC 1.8\nM1 out in 0 0 NM " + tail + "\n.MODEL NM NMOS (LEVEL=1 VTO=0.7 KP=110u)\n"
+            "R1 vdd out 1k\nV2 in 0 DC 1\n.TRAN 1p 1n\n.END\n")
+    with pytest.raises(NetlistError, match="OFF|IC"):
+        parse_netlist(text)

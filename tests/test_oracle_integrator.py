@@ -265,3 +265,24 @@ def test_gear12_rc_closed_forms():
     m = r["t"] >= 0.106e-9
     ref = np.exp(-(r["t"][m] - 0.1005e-9) / 1e-9)
     assert np.max(np.abs(r["x"][m, _node_col(o, c, b)] - ref)) <= 3e-3
+
+
+def test_uic_starts_from_zero_and_charges_like_the_closed_form():
+    """.TRAN ... UIC (no operating point): the run starts from x = 0 with the
+    source at 1 V from the first step on, so the capacitor charges as
+    1 - exp(-t / RC); without UIC the operating point has it at 1 V."""
+    c, a, b = _rc([0.0, 1.0, 1.0, 1.0], r=2e3, cap=0.5e-12)
+    o = Oracle(c)
+    o.set_uic()
+    r = o.tran(4e-9, 1e-12, 0.0)
+    assert r["status"] == 0 and o.dc_strategy() == "none"
+    assert np.all(r["x"][0] == 0.0)
+    col = _node_col(o, c, b)
+    t = r["t"][1:]
+    ref = 1.0 - np.exp(-t / 1e-9)
+    err = np.max(np.abs(r["x"][1:, col] - ref))
+    assert err <= 3e-3, err
+    assert np.max(np.abs(r["x"][1:, _node_col(o, c, a)] - 1.0)) <= 1e-12
+    o2 = Oracle(c)
+    r2 = o2.tran(4e-9, 1e-12, 0.0)
+    assert abs(r2["x"][0, col] - 1.0) <= 1e-12 and np.max(np.abs(r2["x"][:, col] - 1.0)) <= 1e-9

@@ -47,11 +47,11 @@ EXPORTS = [
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
     "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
     "xg_tran_create", "xg_tran_destroy", "xg_tran_set_precision", "xg_tran_set_tolerances", "xg_tran_set_record",
-    "xg_tran_run", "xg_tran_points", "xg_tran_solution",
+    "xg_tran_run", "xg_tran_points", "xg_tran_solution", "xg_tran_set_progress", "xg_tran_set_uic",
 ]
 
 PRED_WIN = 256      # learned predictor: unknowns per window (XG_PRED_WIN)
-PRED_FEAT = 1344    # features per window (XG_PRED_FEAT)
+PRED_FEAT = 1600    # features per window (XG_PRED_FEAT)
 
 
 class XgError(RuntimeError):
@@ -545,6 +545,15 @@ class Transient:
     def set_tolerances(self, reltol=1e-3, abstol=1e-6, nl_reltol=1e-2, nl_abstol=1e-6):
         _check(lib().xg_tran_set_tolerances(self.h, C.c_double(reltol), C.c_double(abstol), C.c_double(nl_reltol),
                                             C.c_double(nl_abstol)), "xg_tran_set_tolerances")
+
+    def set_uic(self, on=True):
+        """Skip the operating point (.TRAN ... UIC): the run starts from
+        x = 0 with the devices' state of one load there."""
+        _check(lib().xg_tran_set_uic(self.h, int(bool(on))), "xg_tran_set_uic")
+
+    def set_progress(self, on=True):
+        """A line on stderr per Newton solve of the DC operating point."""
+        _check(lib().xg_tran_set_progress(self.h, int(bool(on))), "xg_tran_set_progress")
 
     def set_record(self, record_max, w=None, sel=None):
         """Record, at up to ``record_max`` points, the products of the rows

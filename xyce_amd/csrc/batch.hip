@@ -161,7 +161,6 @@ struct BatchArgs {
   double* xP;              // last 4 accepted solutions, planes of total_n
   long long total_n;
   unsigned short* feat;    // [windows][XG_PRED_FEAT] bf16 features of the current step
-  double* wscale;          // [windows] feature scale
   const float* pred_out;   // [windows][XG_PRED_WIN] network output
   unsigned short* pair_x;  // [records][XG_PRED_FEAT]
   float* pair_t;           // [records][XG_PRED_WIN] target (x_new - x_poly) / scale
@@ -170,6 +169,10 @@ struct BatchArgs {
 
 constexpr double kRelTol = 1e-3, kAbsTol = 1e-6;                       // TIA defaults
 constexpr double kNlRel = 1e-2, kNlAbs = 1e-6, kDxTol = 0.33, kRhsTol = 1e-2;  // transient Newton
+// learned predictor: training targets clipped to +-kPredClip Newton weights;
+// proposals with an entry beyond kPredTrust weights are not used
+constexpr double kPredClip = 256.0;
+constexpr float kPredTrust = 512.0f;
 constexpr int kMaxNewton = 20;
 constexpr double kDcRel = 1e-3, kDcAbs = 1e-12, kDcDxTol = 1.0, kDcRhsTol = 1e-6;
 constexpr int kDcMaxNewton = 200;
@@ -263,15 +266,23 @@ __device__ __forceinline__ unsigned short d_f2bf(float f) {
 }
 
 // Learned-predictor features of the step about to be taken, per window of
-// XG_PRED_WIN unknowns: with x_n..x_{n-3} the last accepted solutions and
-// x_poly the polynomial predictor, the scaled vectors
-//   (x_n - x_{n-1}, x_{n-1} - x_{n-2}, x_{n-2} - x_{n-3}, x_poly - x_n,
-//    x_cubic - x_poly) / s,
-// x_cubic the cubic through the 4 points at t_{n+1}, s the window's largest
-// |entry| of the first four plus RELTOL max|x_n| + ABSTOL, then the
-// step-size ratios.  The
-// network's output y gives x = x_poly + s y.
-__device__ void pred_features(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S, double* red) {
+// XG_PRED_WIN unknowns, in units of each unknown's Newton weight
+//   w = 1e-2 max(|x_n|, |x_poly|) + 1e-6
+// (the transient Newton weight of the first iteration, DampedNewton's
+// updateWeights_): with x_n..x_{n-3} the last accepted solutions, x_poly the
+// polynomial predictor and x_cubic the cubic through the 4 points at t_{n+1},
+// the planes
+//   g((x_n - x_{n-1}) / w), g((x_{n-1} - x_{n-2}) / w), g((x_{n-2} - x_{n-3}) / w),
+//   g((x_poly - x_n) / w), g((x_cubic - x_poly) / w), x_n,
+// g(v) = sign(v) log2(1 + |v|) (BF16 keeps 3 digits of every magnitude,
+// so nodes near 0 V are resolved as well as the rails), then the step-size
+// ratios.  The network's output y gives x = x_poly + w y: a proposal within
+// 1/3 of a weight of the solution on every unknown converges in one Newton
+// iteration.
+__device__ __forceinline__ double pred_weight(double xn, double xp) { return kNlRel * fmax(fabs(xn), fabs(xp)) + kNlAbs; }
+__device__ __forceinline__ float pred_slog(double v) { return (float)copysign(log2(1.0 + fabs(v)), v); }
+
+__device__ void pred_features(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S) {
   const int n = C.n, off = I.vec_off, tid = threadIdx.x, nth = blockDim.x;
   const long long TN = A.total_n;
   const int nw = (n + XG_PRED_WIN - 1) / XG_PRED_WIN;
@@ -287,45 +298,32 @@ __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const Class
   }
   for (int w = 0; w < nw; ++w) {
     const int r0 = w * XG_PRED_WIN, r1 = min(n, r0 + XG_PRED_WIN);
-    double mx = 0.0, mv = 0.0;
-    for (int r = r0 + tid; r < r1; r += nth) {
-      const double x0 = A.xP[off + r], x1 = A.xP[TN + off + r], x2 = A.xP[2 * TN + off + r],
-                   x3 = A.xP[3 * TN + off + r];
-      const double xp = A.xn0[off + r];
-      mx = fmax(mx, fmax(fmax(fabs(x0 - x1), fabs(x1 - x2)), fmax(fabs(x2 - x3), fabs(xp - x0))));
-      mv = fmax(mv, fabs(x0));
-    }
-    mx = block_max(mx, red);
-    mv = block_max(mv, red);
-    // scale: the window's largest change, floored at the local-error weight
-    // (RELTOL |x| + ABSTOL), so quiet windows are not blown up
-    const double sc = mx + kRelTol * mv + kAbsTol;
     unsigned short* f = A.feat + (size_t)(I.win_off + w) * XG_PRED_FEAT;
     for (int j = tid; j < XG_PRED_WIN; j += nth) {
       const int r = r0 + j;
-      float v[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+      float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
       if (r < r1) {
         const double x0 = A.xP[off + r], x1 = A.xP[TN + off + r], x2 = A.xP[2 * TN + off + r],
                      x3 = A.xP[3 * TN + off + r];
         const double xp = A.xn0[off + r];
         const double xc = L[0] * x0 + L[1] * x1 + L[2] * x2 + L[3] * x3;
-        v[0] = (float)((x0 - x1) / sc); v[1] = (float)((x1 - x2) / sc); v[2] = (float)((x2 - x3) / sc);
-        v[3] = (float)((xp - x0) / sc); v[4] = (float)((xc - xp) / sc);
+        const double wr = pred_weight(x0, xp);
+        v[0] = pred_slog((x0 - x1) / wr); v[1] = pred_slog((x1 - x2) / wr); v[2] = pred_slog((x2 - x3) / wr);
+        v[3] = pred_slog((xp - x0) / wr); v[4] = pred_slog((xc - xp) / wr); v[5] = (float)x0;
       }
-      for (int q = 0; q < 5; ++q) f[q * XG_PRED_WIN + j] = d_f2bf(v[q]);
+      for (int q = 0; q < 6; ++q) f[q * XG_PRED_WIN + j] = d_f2bf(v[q]);
     }
-    for (int j = 5 * XG_PRED_WIN + tid; j < XG_PRED_FEAT; j += nth) {
-      const int q = j - 5 * XG_PRED_WIN;
+    for (int j = 6 * XG_PRED_WIN + tid; j < XG_PRED_FEAT; j += nth) {
+      const int q = j - 6 * XG_PRED_WIN;
       float v = 0.f;
       if (q == 0) v = (float)(h1 / ha);
       else if (q == 1) v = (float)(ha / hb);
       else if (q == 2) v = (float)(hb / hc);
       else if (q == 3) v = (float)S.order;
-      else if (q == 4) v = (float)(log10(sc) / 10.0);
+      else if (q == 4) v = (float)((log10(h1) + 12.0) / 4.0);
       else if (q == 5) v = 1.0f;
       f[j] = d_f2bf(v);
     }
-    if (tid == 0) A.wscale[I.win_off + w] = sc;
   }
 }
 
@@ -361,11 +359,16 @@ __device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDe
       const size_t rec = (size_t)I.pair_off + (size_t)S.n_pairs * nw + w;
       const unsigned short* f = A.feat + (size_t)(I.win_off + w) * XG_PRED_FEAT;
       for (int j = tid; j < XG_PRED_FEAT; j += nth) A.pair_x[rec * XG_PRED_FEAT + j] = f[j];
-      const double sc = A.wscale[I.win_off + w];
       for (int j = tid; j < XG_PRED_WIN; j += nth) {
         const int r = w * XG_PRED_WIN + j;
         const bool ok = r < n;
-        A.pair_t[rec * XG_PRED_WIN + j] = ok ? (float)((xs[r] - A.xn0[off + r]) / sc) : 0.f;
+        double tv = 0.0;
+        if (ok) {
+          const double xp = A.xn0[off + r];
+          tv = (xs[r] - xp) / pred_weight(A.xP[off + r], xp);
+          tv = fmin(fmax(tv, -kPredClip), kPredClip);
+        }
+        A.pair_t[rec * XG_PRED_WIN + j] = (float)tv;
         A.pair_m[rec * XG_PRED_WIN + j] = ok ? 1.f : 0.f;
       }
     }
@@ -462,7 +465,7 @@ __device__ void begin_step(const BatchArgs& A, InstState& S, const ClassDesc& C,
   if (threadIdx.x == 0) S.errwt = kRelTol * mx + kAbsTol;
   __syncthreads();
   if (A.pred_mode != 0 && S.n_acc >= 4) {
-    pred_features(A, I, C, S, red);
+    pred_features(A, I, C, S);
     __syncthreads();
     if (threadIdx.x == 0) {
       S.feat_valid = 1;
@@ -1310,9 +1313,9 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
 }
 
 // Resumes the instances waiting for the network's proposal (mode 3):
-// x = x_poly + s * y per window when every |y| <= kPredTrust and finite,
-// else x = x_poly (the polynomial predictor is the fallback).
-constexpr float kPredTrust = 8.0f;
+// x = x_poly + w y (w the Newton weight of each unknown) when every
+// |y| <= kPredTrust and finite, else x = x_poly (the polynomial predictor is
+// the fallback).
 __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
   const int inst = blockIdx.x;
   if (inst >= A.n_inst) return;
@@ -1334,7 +1337,8 @@ __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
     const int w = r / XG_PRED_WIN;
     const double y = A.pred_out[(size_t)(I.win_off + w) * XG_PRED_WIN + r % XG_PRED_WIN];
-    A.x[off + r] = A.xn0[off + r] + (use ? A.wscale[I.win_off + w] * y : 0.0);
+    const double xp = A.xn0[off + r];
+    A.x[off + r] = use ? xp + pred_weight(A.xP[off + r], xp) * y : xp;
   }
   if (threadIdx.x == 0) {
     InstState& S = A.st[inst];
@@ -1433,7 +1437,7 @@ struct xg_batch {
   int pred_mode = 0, pred_rows = 0, total_windows = 0, pair_cap = 0;
   long long total_n = 0, total_records = 0;
   xg_mlp* net = nullptr;
-  DArr<double> d_xP, d_wscale;
+  DArr<double> d_xP;
   DArr<unsigned short> d_feat, d_pair_x;
   DArr<float> d_pred_out, d_pair_t, d_pair_m;
   BatchArgs args{};
@@ -2002,7 +2006,6 @@ int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity) {
   b->pair_cap = mode == 1 ? pair_capacity : 0; b->total_n = tn; b->total_records = rec;
   if (mode != 0) {
     HIPCHK(b->d_xP.zeros((size_t)4 * tn));
-    HIPCHK(b->d_wscale.zeros(rows));
     HIPCHK(b->d_feat.zeros((size_t)rows * XG_PRED_FEAT));
     HIPCHK(b->d_pred_out.zeros((size_t)rows * XG_PRED_WIN));
   }
@@ -2014,7 +2017,7 @@ int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity) {
   HIPCHK(hipMemcpy(b->d_inst.p, b->insts.data(), sizeof(InstDesc) * b->n_inst, hipMemcpyHostToDevice));
   BatchArgs& A = b->args;
   A.pred_mode = mode; A.pair_cap = b->pair_cap; A.xP = b->d_xP.p; A.total_n = tn; A.feat = b->d_feat.p;
-  A.wscale = b->d_wscale.p; A.pred_out = b->d_pred_out.p; A.pair_x = b->d_pair_x.p; A.pair_t = b->d_pair_t.p;
+  A.pred_out = b->d_pred_out.p; A.pair_x = b->d_pair_x.p; A.pair_t = b->d_pair_t.p;
   A.pair_m = b->d_pair_m.p;
   b->drop_graph();
   return XG_OK;
@@ -2108,7 +2111,10 @@ int xg_batch_autotune(xg_batch* b, double* result) {
   // form: workgroup size x grouping with 8 iterations per launch, then, with
   // LDS-sized groups, each group's own workgroup size, the iterations per
   // launch (4, 8, 16, 32, 64) and whether the groups run side by side.
+  // Workgroup sizes: 128, 256, 512 threads, per group also 64.
   const int blocks[3] = {128, 256, 512};
+  // per group also one wavefront per instance (its barriers cost nothing)
+  const int gblocks[4] = {64, 128, 256, 512};
   auto time_launches = [&](int warm, int timed, double* us) -> int {
     int st = xg_batch_advance(b, warm, 0);
     if (st != XG_OK) return st;
@@ -2152,7 +2158,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
       std::vector<int> bb(b->buckets.size(), best_fused.block), bt(b->buckets.size(), 0);
       for (size_t j = 0; j < b->buckets.size(); ++j) {
         double bj = 1e30;
-        for (int bs : blocks) {
+        for (int bs : gblocks) {
           for (int lt = 0; lt < 2; ++lt) {
             // the term arrays in LDS where they fit a workgroup's LDS
             if (lt && b->buckets[j].shmem_terms > kMaxLdsBytes) continue;

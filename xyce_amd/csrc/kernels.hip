@@ -374,9 +374,11 @@ __global__ void k_lu_pivots(int n, const int* lu_rowptr, const int* diag, const 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double d = (double)lu[diag[i]];
-  bool b = !(fabs(d) > 0.0) || !isfinite(d);
+  const bool hard = !(fabs(d) > 0.0) || !isfinite(d);
+  bool b = hard;
   for (int p = lu_rowptr[i]; p < diag[i] && !b; ++p) b = !(fabs((double)lu[p]) <= 1.0e3);
   if (b) atomicMin(bad, i + 1);
+  if (hard) atomicOr(bad + 1, 1);   // a zero or non-finite pivot (else only the threshold)
 }
 
 // Level-scheduled triangular solves; b is permuted on the way in and x on the
@@ -516,6 +518,10 @@ __global__ void k_axpy_f64(int n, const double* d, double* x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += d[i];
 }
+__global__ void k_axpy_neg_f64(int n, const double* d, double* x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] -= d[i];
+}
 
 // state permutation between the Xyce layout and the SoA planes
 __global__ void k_sta_to_soa(int n_mos, int nslots, const int* off0, const double* xy, double* soa) {
@@ -585,6 +591,8 @@ struct xg_lu {
   double ir_tol = 1e-12;
   int ir_max = 10;
   bool have64 = false;    // FP64 factors of the current values exist (refinement fallback)
+  bool soft64 = false;    // those factors have multipliers beyond KLU's threshold: solves refine
+  bool repivot_futile = false;  // a re-pivot left threshold failures: keep such factors at once
   double stats[6] = {0, 0, 0, 0, 0, 0};  // last solve: refinements, rel. residual; totals: solves, refinements, fallbacks, FP32 pivot failures
   // method: 1 = level-scheduled entry LU (small systems), 2 = multifrontal
   int method = 1;
@@ -689,7 +697,7 @@ struct xg_lu {
     return XG_OK;
   }
   template <class T>
-  int check_pivots(const T* f, int* bad_out) {
+  int check_pivots(const T* f, int* bad_out, int* hard_out) {
     int init[2] = {0x7fffffff, 0};
     HIPCHK(hipMemcpyAsync(flag.p, init, sizeof(init), hipMemcpyHostToDevice, stream));
     k_lu_pivots<T><<<nblk(s.n), 256, 0, stream>>>(s.n, lu_rowptr.p, lu_diag.p, f, flag.p);
@@ -697,6 +705,7 @@ struct xg_lu {
     HIPCHK(hipMemcpyAsync(res, flag.p, sizeof(res), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
+    *hard_out = res[1];
     return XG_OK;
   }
   // solve with the factors f and device vectors db -> dx (arithmetic in T),
@@ -721,12 +730,13 @@ struct xg_lu {
     HIPCHK(hipGetLastError());
     return XG_OK;
   }
-  // either method: factor the values in `avals` (*bad = failing pivot + 1)
-  int factor_any(bool fp32, int* bad) {
-    if (method == 2) return mfp->factor(avals.p, s.nnz_a, fp32, bad);
+  // either method: factor the values in `avals` (*bad = failing pivot + 1,
+  // *hard = 1 if a pivot was zero or not finite)
+  int factor_any(bool fp32, int* bad, int* hard) {
+    if (method == 2) return mfp->factor(avals.p, s.nnz_a, fp32, bad, hard);
     const int st = fp32 ? factor_device<float>(lu32.p) : factor_device<double>(lu.p);
     if (st != XG_OK) return st;
-    return fp32 ? check_pivots<float>(lu32.p, bad) : check_pivots<double>(lu.p, bad);
+    return fp32 ? check_pivots<float>(lu32.p, bad, hard) : check_pivots<double>(lu.p, bad, hard);
   }
   int solve_any(bool fp32, const double* db, double* dx) {
     if (method == 2) return mfp->solve(db, dx, fp32, flag.p + 1);
@@ -747,6 +757,31 @@ struct xg_lu {
     k_fill<<<1, 1, 0, stream>>>(nan_flag, 1, 0);
     if (method == 2) return mfp->solve_ext(db, dx, nan_flag, w);
     return solve_device<double>(lu.p, w, db, dx, nan_flag);
+  }
+  // FP64 factors whose multipliers exceeded KLU's threshold: solve, then
+  // refine x += A^-1 r against A in FP64 while the residual decreases, up to
+  // ir_max corrections or ir_tol; the best iterate is kept
+  int solve_refined64(const double* db, double* dx) {
+    int st = solve_any(false, db, dx);
+    if (st != XG_OK) return st;
+    double rel = 0.0, prev = INFINITY;
+    int it = 0;
+    for (;;) {
+      st = residual(db, dx, &rel);
+      if (st != XG_OK) return st;
+      if (rel <= ir_tol || it >= ir_max || !(rel < prev)) break;
+      prev = rel;
+      st = solve_any(false, r.p, d.p);
+      if (st != XG_OK) return st;
+      k_axpy_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, d.p, dx);
+      ++it;
+    }
+    if (!(rel < prev) && it > 0) {
+      // the last correction did not lower the residual: take it back
+      k_axpy_neg_f64<<<nblk(s.n), 256, 0, stream>>>(s.n, d.p, dx);
+    }
+    stats[0] = it; stats[1] = std::min(rel, prev); stats[2] += 1; stats[3] += it;
+    return XG_OK;
   }
   // r = b - A x; returns max|r| / max|b| (0/0 -> 0)
   int residual(const double* db, const double* dx, double* rel) {
@@ -783,10 +818,11 @@ struct xg_lu {
         // refinement does not converge with the FP32 factors: FP64 factors
         HIPCHK(hipMemsetAsync(flag.p + 1, 0, sizeof(int), stream));
         if (!have64) {
-          int bad64 = 0;
-          st = factor_any(false, &bad64);
+          int bad64 = 0, hard64 = 0;
+          st = factor_any(false, &bad64, &hard64);
           if (st != XG_OK) return st;
           have64 = true;
+          soft64 = bad64 != 0 && !hard64;
         }
         st = solve_any(false, db, dx);
         if (st != XG_OK) return st;
@@ -1288,29 +1324,36 @@ int xg_lu_destroy(xg_lu* lu) {
   return XG_OK;
 }
 
+// FP64 factors of the values in avals after the FP32 ones (mode 1) or alone:
+// threshold-only failures are kept (solves refine), a zero or non-finite
+// pivot is reported through *hard
+static int lu_factor64(xg_lu* lu, bool fp32_failed, int* bad, int* hard) {
+  int st = lu->factor_any(false, bad, hard);
+  if (st != XG_OK) return st;
+  if (*bad && *hard) return XG_OK;
+  if (fp32_failed) lu->stats[5] += 1;
+  lu->have64 = true;
+  lu->soft64 = *bad != 0;
+  return XG_OK;
+}
+
 static int lu_numeric_impl(xg_lu* lu, const double* vals, bool dev) {
   if (!lu || !vals) return XG_E_ARG;
   hipStream_t s = lu->stream;
   const hipMemcpyKind in = dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
   HIPCHK(hipMemcpyAsync(lu->avals.p, vals, sizeof(double) * lu->s.nnz_a, in, s));
   lu->have64 = false;
-  int bad = 0, st;
-  bool fp32_failed = false;
+  lu->soft64 = false;
+  int bad = 0, hard = 0, st;
   if (lu->mode == 1) {
-    st = lu->factor_any(true, &bad);
+    st = lu->factor_any(true, &bad, &hard);
     if (st != XG_OK) return st;
     if (!bad) return XG_OK;
-    fp32_failed = true;
   }
-  st = lu->factor_any(false, &bad);
+  st = lu_factor64(lu, lu->mode == 1, &bad, &hard);
   if (st != XG_OK) return st;
-  if (!bad) {
-    // the FP32 pivots failed where the FP64 ones pass: solve with FP64 factors
-    if (fp32_failed) lu->stats[5] += 1;
-    lu->have64 = true;
-    return XG_OK;
-  }
-  // the pivot order itself fails the threshold test (KLU would re-pivot):
+  if (!bad || (!hard && lu->repivot_futile)) return XG_OK;
+  // a pivot that fails KLU's threshold or is zero (KLU would pivot elsewhere):
   // analyse again steering the transversal by the values, then refactor
   std::vector<double> hv;
   if (dev) {
@@ -1328,16 +1371,14 @@ static int lu_numeric_impl(xg_lu* lu, const double* vals, bool dev) {
   if (st != XG_OK) return st;
   HIPCHK(hipMemcpyAsync(lu->avals.p, hvals, sizeof(double) * lu->s.nnz_a, hipMemcpyHostToDevice, s));
   if (lu->mode == 1) {
-    st = lu->factor_any(true, &bad);
+    st = lu->factor_any(true, &bad, &hard);
     if (st != XG_OK) return st;
     if (!bad) return XG_OK;
-    lu->stats[5] += 1;
   }
-  st = lu->factor_any(false, &bad);
+  st = lu_factor64(lu, lu->mode == 1, &bad, &hard);
   if (st != XG_OK) return st;
-  if (bad) return bad;
-  lu->have64 = true;
-  return XG_OK;
+  if (bad && !hard) lu->repivot_futile = true;  // kept, refined from now on without a re-pivot
+  return (bad && hard) ? bad : XG_OK;
 }
 
 int xg_lu_numeric(xg_lu* lu, const double* vals) { return lu_numeric_impl(lu, vals, false); }
@@ -1353,6 +1394,7 @@ static int lu_solve_impl(xg_lu* lu, const double* b, double* x, int transpose, b
   k_fill<<<1, 1, 0, s>>>(lu->flag.p + 1, 1, 0);
   int st;
   if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
+  else if (lu->soft64) st = lu->solve_refined64(lu->b.p, lu->x.p);
   else st = lu->solve_any(false, lu->b.p, lu->x.p);
   if (st != XG_OK) return st;
   int nanf = 0;
@@ -1384,9 +1426,10 @@ int xg_lu_solve_dev(xg_lu* lu, const double* b, double* x, int transpose) {
 int xg_lu_solve_noalloc(xg_lu* lu, const double* b, double* x) {
   if (!lu || !b || !x) return XG_E_ARG;
   if (!lu->scratch_work) return XG_E_HIP;
-  // FP32 factors are refined, and refinement can fall back to an FP64
-  // factorisation: not a solve with the current factors
-  if (lu->mode == 1 && !lu->have64) return XG_E_ARG;
+  // FP32 factors, and FP64 factors with multipliers beyond the threshold,
+  // are refined against A (and FP32 refinement may refactor in FP64): not a
+  // plain solve with the current factors
+  if ((lu->mode == 1 && !lu->have64) || lu->soft64) return XG_E_ARG;
   if (lu->work_doubles() > kScratchDoubles) return XG_E_ARG;
   hipStream_t s = lu->stream;
   int st = lu->solve_with_work(b, x, lu->scratch_work);

@@ -208,7 +208,10 @@ __device__ __forceinline__ T mf_wave_sum(T v) {
 // order on every refactor (no atomics).  A child's rows map to increasing
 // parent rows; alo holds, per entry and child, the first child row that lands
 // at or after the entry's first row.  The four wavefronts take the child rows
-// in turn, lanes over the columns.
+// in turn (wave w: rows a0 + w, a0 + w + 4, ...), lanes over 256-column
+// chunks; a chunk's parent column positions are loaded once, and each wave
+// loads four rows' child and parent values before it stores any, so a row
+// costs two memory round trips shared with the wave's other rows.
 constexpr int kMfAsmRows = 16;
 
 template <class T>
@@ -223,25 +226,47 @@ __global__ void __launch_bounds__(256) k_mf_assemble(MfDev d, const int4* ent, c
     const int mc = d.m[ch], npc = d.sfirst[ch + 1] - d.sfirst[ch], nu = mc - npc;
     const int* em = d.emap + d.eptr[ch];
     const T* Fc = F + d.fo[ch];
-    for (int a = alo[e.z + (c - d.cptr[s])] + wave; a < nu; a += 4) {
-      const int pr = em[a];
-      if (pr >= r1) break;
-      const T* crow = Fc + (long long)(npc + a) * mc + npc;
-      T* prow = Fs + (long long)pr * ms;
-      for (int b0 = lane; b0 < nu; b0 += 256) {
-        int eb[4];
-        T cv[4], pv[4];
+    const int a0 = alo[e.z + (c - d.cptr[s])] + wave;
+    for (int c0 = 0; c0 < nu; c0 += 256) {
+      const int b0 = c0 + lane;
+      int eb[4];
+      bool ok[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int b = b0 + 64 * q;
-          eb[q] = b < nu ? em[b] : 0;
-          cv[q] = b < nu ? crow[b] : (T)0;
+      for (int q = 0; q < 4; ++q) {
+        const int b = b0 + 64 * q;
+        ok[q] = b < nu;
+        eb[q] = ok[q] ? em[b] : 0;
+      }
+      for (int a = a0; a < nu; a += 16) {
+        // rows a, a + 4, a + 8, a + 12 of this wave (those that land in the range)
+        int pr[4];
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int ak = a + 4 * k;
+          pr[k] = ak < nu ? em[ak] : r1;
+          in[k] = pr[k] < r1;
+        }
+        if (!in[0]) break;
+        T cv[4][4], pv[4][4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const T* crow = Fc + (long long)(npc + a + 4 * k) * mc + npc;
+          const T* prow = Fs + (long long)(in[k] ? pr[k] : 0) * ms;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            cv[k][q] = (in[k] && ok[q]) ? crow[b0 + 64 * q] : (T)0;
+            pv[k][q] = (in[k] && ok[q]) ? prow[eb[q]] : (T)0;
+          }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) pv[q] = b0 + 64 * q < nu ? prow[eb[q]] : (T)0;
+        for (int k = 0; k < 4; ++k) {
+          T* prow = Fs + (long long)(in[k] ? pr[k] : 0) * ms;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          if (b0 + 64 * q < nu) prow[eb[q]] = pv[q] + cv[q];
+          for (int q = 0; q < 4; ++q)
+            if (in[k] && ok[q]) prow[eb[q]] = pv[k][q] + cv[k][q];
+        }
+        if (!in[3]) break;
       }
     }
     __syncthreads();
@@ -266,12 +291,12 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
   const int ld = ms + 1;
   T* Fs = F + d.fo[s];
   const int t = threadIdx.x;
-  for (int i0 = 0; i0 < ms; i0 += 8) {
-    T v[8];
+  for (int i0 = 0; i0 < ms; i0 += 16) {
+    T v[16];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = (t < ms && i0 + q < ms) ? Fs[(long long)(i0 + q) * ms + t] : (T)0;
+    for (int q = 0; q < 16; ++q) v[q] = (t < ms && i0 + q < ms) ? Fs[(long long)(i0 + q) * ms + t] : (T)0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q)
+    for (int q = 0; q < 16; ++q)
       if (t < ms && i0 + q < ms) A[(i0 + q) * ld + t] = v[q];
   }
   __syncthreads();
@@ -285,6 +310,16 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
     const int eb = t < nu ? em[t] : 0;     // lane t: parent position of the child's row/column t
     const T* cb = Fc + (long long)npc * mc + npc + t;
     int a = 0;
+    for (; a + 16 <= nu; a += 16) {
+      T v[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = t < nu ? cb[(long long)(a + q) * mc] : (T)0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int pr = __builtin_amdgcn_readlane(eb, a + q);
+        if (t < nu) A[pr * ld + eb] += v[q];
+      }
+    }
     for (; a + 4 <= nu; a += 4) {
       T v[4];
 #pragma unroll
@@ -301,10 +336,10 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
     }
     __syncthreads();
   }
-  bool fail = false;
+  bool fail = false, hard = false;
   for (int k = 0; k < np; ++k) {
     const T piv = A[k * ld + k];
-    if (mf_bad_pivot(piv)) { fail = true; break; }
+    if (mf_bad_pivot(piv)) { fail = true; hard = true; break; }
     if (t > k && t < ms) {
       T* ri = A + t * ld;
       const T* rk = A + k * ld;
@@ -328,6 +363,7 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
   for (int i = 0; i < ms; ++i)
     if (t < ms) Fs[(long long)i * ms + t] = A[i * ld + t];
   if (__any(fail) && t == 0) atomicMin(bad, d.sfirst[s] + 1);
+  if (hard && t == 0) atomicOr(bad + 1, 1);
 }
 
 // larger fronts, blocked by panels of kMfPanel columns; all fronts of a level
@@ -341,6 +377,7 @@ __global__ void __launch_bounds__(64) k_mf_factor_tiny(MfDev d, const int* list,
 template <class T>
 __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p0, T* F, T* dbuf, int* bad) {
   __shared__ T D[kMfPanel][kMfPanel + 1];
+  __shared__ T Rd[kMfPanel];
   __shared__ T Tl[64][kMfPanel + 1];
   const int4 e = ent[blockIdx.x];
   const int s = e.x, blk = e.y, slot = e.z;
@@ -383,6 +420,8 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
   if (t < kMfPanel) {
 #pragma unroll
     for (int q = 0; q < kMfPanel; ++q) D[t][q] = r[q];
+    // pivot reciprocals, all at once: the row solves below multiply
+    Rd[t] = t < pw ? (T)1 / D[t][t] : (T)0;
   }
   __syncthreads();
   if (blk == 0) {
@@ -412,7 +451,7 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
 #pragma unroll
     for (int k = 0; k < kMfPanel; ++k) {
       if (k < pw) {
-        a[k] = a[k] / D[k][k];
+        a[k] = a[k] * Rd[k];
         if (valid && !(fabs((double)a[k]) <= 1.0e3)) fail = true;
 #pragma unroll
         for (int q = 0; q < kMfPanel; ++q)
@@ -447,6 +486,7 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
       if (valid && q < pw) Fs[(long long)(p0 + q) * ms + c] = b[q];
   }
   if (__any(fail) && t == 0) atomicMin(bad, d.sfirst[s] + p0 + 1);
+  if (__any(stop) && t == 0) atomicOr(bad + 1, 1);
 }
 
 // (2) trailing update F[i][j] -= sum_q L[i][q] U[q][j], i, j >= p0 + pw, one
@@ -454,9 +494,33 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
 // slot}; tile row -1: the front has nothing below the panel), the panel's L
 // rows and U columns staged in LDS.  Tile (0, 0) or the -1 entry copies the
 // panel's diagonal-block factors from dbuf into the front.
+// Trailing update of the larger fronts on the matrix cores: the 64x64 tile
+// of a workgroup as 4 wavefronts x 2 x 2 blocks of 16 x 16, rank-4 products
+// per v_mfma_{f64,f32}_16x16x4 (A: lane l holds row l % 16, column l / 16 of
+// the 16 x 4 block; B: row l / 16, column l % 16 of the 4 x 16 block;
+// accumulator register r of lane l: column l % 16, row l / 16 + 4 r for FP64
+// and 4 (l / 16) + r for FP32).
+typedef double mf_d4 __attribute__((ext_vector_type(4)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+template <class T> struct MfAcc;
+template <> struct MfAcc<double> {
+  typedef mf_d4 V;
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+  static __device__ __forceinline__ V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MfAcc<float> {
+  typedef mf_f4 V;
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+  static __device__ __forceinline__ V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
 template <class T>
 __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int p0, T* F, const T* dbuf) {
-  constexpr int TB = 64, KB = 16;
+  constexpr int TB = 64, KB = kMfPanel;
   __shared__ T Ls[TB][KB + 1];
   __shared__ T Us[KB][TB + 1];
   const int4 e = ent[blockIdx.x];
@@ -474,47 +538,54 @@ __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int
   }
   if (e.y < 0) return;
   const int i0 = base + e.y * TB, j0 = base + e.z * TB;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4 x 4 outputs each
-  T acc[4][4];
-  for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) acc[a][b] = (T)0;
-  for (int q0 = 0; q0 < pw; q0 += KB) {
-    {
-      // all loads issued before the LDS stores
-      T lv[TB * KB / 256], uv[TB * KB / 256];
+  {
+    // the L21 rows and U12 columns of the tile (zero beyond the front and the
+    // panel), all loads issued before the LDS stores
+    T lv[TB * KB / 256], uv[TB * KB / 256];
 #pragma unroll
-      for (int it = 0; it < TB * KB / 256; ++it) {
-        const int x = threadIdx.x + it * 256;
-        const int r = x / KB, q = x % KB;
-        const int gi = i0 + r, gq = p0 + q0 + q;
-        lv[it] = (gi < ms && q0 + q < pw) ? Fs[(long long)gi * ms + gq] : (T)0;
-        const int qq = x / TB, c = x % TB;
-        const int gj = j0 + c, gq2 = p0 + q0 + qq;
-        uv[it] = (gj < ms && q0 + qq < pw) ? Fs[(long long)gq2 * ms + gj] : (T)0;
-      }
+    for (int it = 0; it < TB * KB / 256; ++it) {
+      const int x = threadIdx.x + it * 256;
+      const int r = x / KB, q = x % KB;
+      const int gi = i0 + r;
+      lv[it] = (gi < ms && q < pw) ? Fs[(long long)gi * ms + p0 + q] : (T)0;
+      const int qq = x / TB, c = x % TB;
+      const int gj = j0 + c;
+      uv[it] = (gj < ms && qq < pw) ? Fs[(long long)(p0 + qq) * ms + gj] : (T)0;
+    }
 #pragma unroll
-      for (int it = 0; it < TB * KB / 256; ++it) {
-        const int x = threadIdx.x + it * 256;
-        Ls[x / KB][x % KB] = lv[it];
-        Us[x / TB][x % TB] = uv[it];
-      }
-    }
-    __syncthreads();
-    for (int q = 0; q < KB; ++q) {
-      T lv[4], uv[4];
-      for (int a = 0; a < 4; ++a) lv[a] = Ls[ty * 4 + a][q];
-      for (int b = 0; b < 4; ++b) uv[b] = Us[q][tx * 4 + b];
-      for (int a = 0; a < 4; ++a) for (int b = 0; b < 4; ++b) acc[a][b] += lv[a] * uv[b];
-    }
-    __syncthreads();
-  }
-  for (int a = 0; a < 4; ++a) {
-    const int gi = i0 + ty * 4 + a;
-    if (gi >= ms) continue;
-    for (int b = 0; b < 4; ++b) {
-      const int gj = j0 + tx * 4 + b;
-      if (gj < ms) Fs[(long long)gi * ms + gj] -= acc[a][b];
+    for (int it = 0; it < TB * KB / 256; ++it) {
+      const int x = threadIdx.x + it * 256;
+      Ls[x / KB][x % KB] = lv[it];
+      Us[x / TB][x % TB] = uv[it];
     }
   }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, lr = lane & 15, lk = lane >> 4;
+  typedef typename MfAcc<T>::V V;
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = V{};
+#pragma unroll
+  for (int k = 0; k < KB; k += 4) {
+    const T a0 = Ls[wr + lr][k + lk], a1 = Ls[wr + 16 + lr][k + lk];
+    const T b0 = Us[k + lk][wc + lr], b1 = Us[k + lk][wc + 16 + lr];
+    acc[0][0] = MfAcc<T>::mma(a0, b0, acc[0][0]);
+    acc[0][1] = MfAcc<T>::mma(a0, b1, acc[0][1]);
+    acc[1][0] = MfAcc<T>::mma(a1, b0, acc[1][0]);
+    acc[1][1] = MfAcc<T>::mma(a1, b1, acc[1][1]);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = i0 + wr + 16 * a + MfAcc<T>::row(lk, r), gj = j0 + wc + 16 * b + lr;
+        if (gi < ms && gj < ms) Fs[(long long)gi * ms + gj] -= acc[a][b][r];
+      }
 }
 
 // sums over c in [c0, c1) of F[r + k][c] v[c] for the rows r .. r + nr - 1
@@ -1022,7 +1093,7 @@ static int mf_replay(MfLu::Impl& I, const void* const* key, int nkey, Enqueue en
 }
 
 template <class T>
-static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int* bad_out) {
+static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int* bad_out, int* hard_out) {
   const void* key[8] = {(const void*)1, vals_dev, F, (const void*)(long long)nnz, nullptr, nullptr, nullptr, nullptr};
   int rc = mf_replay(I, key, 8, [&] { return mf_factor_enqueue<T>(I, vals_dev, nnz, F); });
   if (rc != XG_OK) return rc;
@@ -1030,6 +1101,7 @@ static int mf_factor_t(MfLu::Impl& I, const double* vals_dev, int nnz, T* F, int
   MFCHK(hipMemcpyAsync(res, I.bad.p, sizeof(res), hipMemcpyDeviceToHost, I.st));
   MFCHK(hipStreamSynchronize(I.st));
   *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
+  *hard_out = res[1];
   return XG_OK;
 }
 
@@ -1076,13 +1148,13 @@ static int mf_solve_t(MfLu::Impl& I, const T* F, T* V, T* y, double* bp, const d
   return mf_replay(I, key, 8, [&] { return mf_solve_enqueue<T>(I, F, V, y, bp, b_dev, x_dev, nan_flag); });
 }
 
-int MfLu::factor(const double* vals_dev, int nnz, bool fp32, int* bad) {
+int MfLu::factor(const double* vals_dev, int nnz, bool fp32, int* bad, int* hard) {
   Impl& I = *impl_;
   if (fp32) {
     if (!I.F32.p) return XG_E_ARG;
-    return mf_factor_t<float>(I, vals_dev, nnz, I.F32.p, bad);
+    return mf_factor_t<float>(I, vals_dev, nnz, I.F32.p, bad, hard);
   }
-  return mf_factor_t<double>(I, vals_dev, nnz, I.F64.p, bad);
+  return mf_factor_t<double>(I, vals_dev, nnz, I.F64.p, bad, hard);
 }
 
 int MfLu::solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag) {

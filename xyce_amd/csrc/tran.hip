@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -269,6 +270,8 @@ struct xg_tran {
   int dc_stage = 0;
   // counters
   double steps = 0, rejects = 0, newton_iters = 0, loads = 0, dc_iters = 0, lu_fail = 0, seconds = 0;
+  bool progress = false;   // a line on stderr per DC Newton solve
+  bool uic = false;        // .TRAN ... UIC: no operating point, the run starts from x = 0
 
   ~xg_tran() {
     if (lu) xg_lu_destroy(lu);
@@ -425,6 +428,9 @@ int dc_continuation(xg_tran* T, double p0, double p1, double step, double min_st
     NewtonResult r;
     if ((st = newton(T, true, 0.0, 0.0, p, T->zeros.p, nullptr, nullptr, &r)) != XG_OK) return st;
     T->dc_iters += r.iters;
+    if (T->progress)
+      std::fprintf(stderr, "xg_tran DC %s stepping: parameter %.6g, Newton status %d after %d iterations\n",
+                   T->dc_stage == 1 ? "gmin" : "source", par, r.status, r.iters);
     if (r.status > 0) {
       first = false;
       par_ok = par;
@@ -461,6 +467,8 @@ int dcop(xg_tran* T, int* ok) {
   NewtonResult r;
   if ((st = newton(T, true, 0.0, 0.0, p, T->zeros.p, nullptr, nullptr, &r)) != XG_OK) return st;
   T->dc_iters += r.iters;
+  if (T->progress)
+    std::fprintf(stderr, "xg_tran DC Newton: status %d after %d iterations\n", r.status, r.iters);
   if (r.status > 0) { *ok = 1; return XG_OK; }
   int good = -1;
   T->dc_stage = 1;
@@ -523,12 +531,34 @@ void initialize(xg_tran* T, bool initial) {
   T->beginning = true;
 }
 
-// DC point, then the transient's start (the oracle's tran() before its loop)
-int start(xg_tran* T) {
-  int ok = 0;
-  int st = dcop(T, &ok);
+// .TRAN ... UIC / NOOP (N_ANP_Transient.C:879-1000): no operating point; the
+// solution starts at zero (no .IC) and one transient load there
+// (initializeProblem, loadRHS) gives the devices' state and F, Q, B, from
+// which the histories are set as after an operating point
+int uic_point(xg_tran* T) {
+  const int n = T->n;
+  hipLaunchKernelGGL(k_fillv, dim3(blocks(n + 1)), dim3(256), 0, T->s, (size_t)(n + 1), T->d.x, 0.0);
+  int st = circuit_history(T->c, 0);
   if (st != XG_OK) return st;
-  if (ok <= 0) { T->status = -100; T->finished = true; return XG_OK; }
+  if ((st = set_state(T, 0, 0, 0, T->tstep, 0.0)) != XG_OK) return st;
+  if ((st = circuit_load(T->c)) != XG_OK) return st;
+  T->loads += 1;
+  T->dc_stage = -1;
+  return XG_OK;
+}
+
+// DC point (or the UIC start), then the transient's start (the oracle's
+// tran() before its loop)
+int start(xg_tran* T) {
+  int st;
+  if (T->uic) {
+    if ((st = uic_point(T)) != XG_OK) return st;
+  } else {
+    int ok = 0;
+    st = dcop(T, &ok);
+    if (st != XG_OK) return st;
+    if (ok <= 0) { T->status = -100; T->finished = true; return XG_OK; }
+  }
   const int n = T->n;
   if ((st = circuit_history(T->c, 1)) != XG_OK) return st;
   hipLaunchKernelGGL(k_copy, dim3(blocks(n)), dim3(256), 0, T->s, n, T->d.x, T->xH0.p);
@@ -744,6 +774,18 @@ int xg_tran_set_record(xg_tran* T, int k, const double* w, int n_sel, const int*
   }
   HIPCHK(T->rec_y.alloc(std::max<size_t>(1, (size_t)record_max * T->rec_width)));
   HIPCHK(hipStreamSynchronize(T->s));
+  return XG_OK;
+}
+
+int xg_tran_set_uic(xg_tran* T, int on) {
+  if (!T || T->started) return XG_E_ARG;
+  T->uic = on != 0;
+  return XG_OK;
+}
+
+int xg_tran_set_progress(xg_tran* T, int on) {
+  if (!T) return XG_E_ARG;
+  T->progress = on != 0;
   return XG_OK;
 }
 

@@ -144,8 +144,10 @@ class MfLu {
   MfLu(const MfLu&) = delete;
   MfLu& operator=(const MfLu&) = delete;
   int setup(const LuSymbolic& s, const int* rowptr, const int* colind, void* stream, bool want_fp32);
-  // *bad = 0, or (first failing pivot column of B) + 1
-  int factor(const double* vals_dev, int nnz, bool fp32, int* bad);
+  // *bad = 0, or (first failing pivot column of B) + 1; *hard = 1 if a pivot
+  // was zero or not finite (else every failure is a multiplier beyond KLU's
+  // threshold)
+  int factor(const double* vals_dev, int nnz, bool fp32, int* bad, int* hard);
   int solve(const double* b_dev, double* x_dev, bool fp32, int* nan_flag);
   // FP64 solve whose front vectors, work vector and permuted right-hand side
   // are the caller's device memory `work` (solve_work_doubles() doubles,

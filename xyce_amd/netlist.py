@@ -11,7 +11,10 @@ importer) and builds a ``Circuit``:
 * ``.TRAN tstep tstop [tstart [tmax]]``, ``.PRINT TRAN V(node) ...``,
   ``.END``; ``.OPTIONS TIMEINT METHOD=TRAP|GEAR`` (also 7|8, the reference's
   numbering, src/TimeIntegrationPKG/N_TIA_fwd.h:52-58) selects the
-  integrator, other ``.OPTIONS`` lines are accepted and ignored;
+  integrator; ``.OPTIONS`` lines may restate the defaults the path runs with
+  (TIMEINT RELTOL=1e-3 ABSTOL=1e-6, DEVICE GMIN=1e-12 VOLTLIM=1, LINSOL
+  TYPE=KLU) and anything else there is refused, as are the MOSFET ``OFF``
+  and ``IC`` instance settings;
 * ``*`` comment lines, ``;`` end-of-line comments, ``+`` continuation lines,
   case-insensitive names, SPICE scale suffixes (T G MEG K MIL M U N P F).
 
@@ -125,6 +128,49 @@ def _source(tokens, no):
     return W_DC, [value(rest[0])]
 
 
+# .OPTIONS settings the path runs with, all of them the reference's defaults
+# (TIMEINT RELTOL/ABSTOL: src/TimeIntegrationPKG/N_TIA_TIAParams.C; DEVICE GMIN
+# and VOLTLIM, the voltage limiter flag read at N_DEV_MOSFET1.C:2785,2864:
+# src/DeviceModelPKG/Core/N_DEV_DeviceOptions.C; LINSOL TYPE=KLU).  A netlist
+# may restate them; another value, or any other setting, is refused rather than
+# ignored, since the path would not honour it.
+_FIXED_OPTIONS = {
+    "TIMEINT": {"RELTOL": 1e-3, "ABSTOL": 1e-6},
+    "DEVICE": {"GMIN": 1e-12, "VOLTLIM": 1.0},
+    "LINSOL": {"TYPE": "KLU"},
+}
+_METHODS = {"TRAP": "trap", "TRAPEZOIDAL": "trap", "7": "trap", "GEAR": "gear", "8": "gear"}
+
+
+def _options(tok, no, analysis):
+    """One ``.OPTIONS <package> NAME=value ...`` line."""
+    if len(tok) < 2:
+        raise NetlistError(f"line {no}: .OPTIONS needs a package name")
+    pkg = tok[1].upper()
+    if pkg not in _FIXED_OPTIONS:
+        raise NetlistError(f"line {no}: .OPTIONS {tok[1]} is not supported")
+    i = 2
+    while i < len(tok):
+        if i + 2 >= len(tok) or tok[i + 1] != "=":
+            raise NetlistError(f"line {no}: expected NAME=value after .OPTIONS {tok[1]}, got {tok[i]!r}")
+        name, raw = tok[i].upper(), tok[i + 2]
+        i += 3
+        if pkg == "TIMEINT" and name == "METHOD":
+            if raw.upper() not in _METHODS:
+                raise NetlistError(f"line {no}: integration method {raw!r} is not supported")
+            analysis["method"] = _METHODS[raw.upper()]
+            continue
+        fixed = _FIXED_OPTIONS[pkg]
+        if name not in fixed:
+            raise NetlistError(f"line {no}: .OPTIONS {pkg} {name} is not supported")
+        want = fixed[name]
+        ok = raw.upper() == want if isinstance(want, str) else value(raw) == want
+        if not ok:
+            hint = " (voltage limiting cannot be switched off)" if name == "VOLTLIM" else ""
+            raise NetlistError(f"line {no}: .OPTIONS {pkg} {name}={raw} is not supported, "
+                               f"only the default {want}{hint}")
+
+
 def parse_netlist(text: str):
     """Returns (Circuit, analysis) with analysis = {'tran': (tstep, tstop,
     tstart, tmax) or None, 'print': [node names], 'title': str, 'method':
@@ -177,14 +223,7 @@ def parse_netlist(text: str):
                     else:
                         i += 1
             elif head in (".OPTIONS", ".OPTION"):
-                if len(tok) > 1 and tok[1].upper() == "TIMEINT":
-                    for i in range(2, len(tok) - 2):
-                        if tok[i].upper() == "METHOD" and tok[i + 1] == "=":
-                            m = tok[i + 2].upper()
-                            methods = {"TRAP": "trap", "TRAPEZOIDAL": "trap", "7": "trap", "GEAR": "gear", "8": "gear"}
-                            if m not in methods:
-                                raise NetlistError(f"line {no}: integration method {tok[i + 2]!r} is not supported")
-                            analysis["method"] = methods[m]
+                _options(tok, no, analysis)
             elif head == ".END":
                 break
             else:
@@ -207,6 +246,11 @@ def parse_netlist(text: str):
             if len(tok) < 6:
                 raise NetlistError(f"line {no}: MOSFET needs d g s b and a model")
             d, g, s_, b = (node(t) for t in tok[1:5])
+            for t in tok[6:]:
+                if t.upper() in ("OFF", "IC"):
+                    # OFF selects the initFix treatment and IC the initial junction
+                    # voltages (N_DEV_MOSFET1.C:2785-2796,2864-2867); neither exists here
+                    raise NetlistError(f"line {no}: MOSFET {t.upper()} is not supported")
             kv = _kv(tok[6:], no)
             bad = [k for k in kv if k not in INST_PARAMS[:9]]
             if bad:

@@ -76,5 +76,5 @@ def run(circuits, tstop, tstep, hmax, net=None, record_max=0, chunk=512, max_lau
 
 
 def new_net(max_rows=4096, seed=5):
-    """The network of the learned predictor: 1344 -> 3 x 512 GELU -> 256."""
+    """The network of the learned predictor: 1600 -> 3 x 512 GELU -> 256."""
     return MLP(PRED_FEAT, 512, 3, PRED_WIN, max_rows=max_rows, seed=seed)
