@@ -73,6 +73,9 @@ typedef struct xg_circuit_desc {
 
 /* Analysis flags of SolverState::setupSolverInfo (N_DEV_SolverState.C:292-430). */
 typedef struct xg_solver_state {
+  /* init_fix changes nothing here: the reference applies it only to devices
+   * with OFF (N_DEV_MOSFET1.C:2822,2864-2867), which the library does not
+   * model; voltage limiting is always on (DeviceOptions voltageLimiterFlag). */
   int dcop, tran, init_jct, init_fix;
   int newton_iter;      /* 0 on the first load of a Newton solve */
   int order;            /* integration order 1 or 2 (OneStep) */

@@ -141,6 +141,38 @@ def test_device_zoo_load_parity(gpu, seed, dcop, init_jct, newton_iter):
         assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
 
 
+@pytest.mark.parametrize("temp", [250.0, 400.0])
+def test_device_zoo_load_parity_off_nominal_temperature(gpu, temp):
+    """The same 200 devices with TEMP != TNOM (updateTemperature,
+    N_DEV_MOSFET1.C:3487; the oracle's temperature scaling is pinned against
+    device physics in test_oracle_devices.py): every entry meets the 1e-12 bar."""
+    from tests.helpers import device_zoo, zoo_states
+    c = device_zoo()
+    d = X.DeviceMgr(c, temp=temp)
+    o = Oracle(c, temp=temp)
+    x, sta0, sto0 = zoo_states(o, 1)
+    kw = dict(dcop=0, tran=1, init_jct=0, newton_iter=2, order=2, h=3e-12, time=0.4e-9, tstep=1e-12, tstop=5e-9)
+    ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    d.set_solver_state(**kw)
+    sta_n, sto_n, lim = d.updateState(x, sta0, sto0, sto_next=sto0)
+    assert lim == ref["limited"]
+    assert entry_tol(sta_n, ref["sta_next"], np.abs(ref["sta_next"]) + 1e-300) <= 1
+    assert entry_tol(sto_n, ref["sto_next"], np.abs(ref["sto_next"]) + 1e-300) <= 1
+    vec = d.loadDAEVectors()
+    for k in ("F", "Q", "B", "dFdxdVp", "dQdxdVp"):
+        a, b = vec[k], ref[k]
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1, k
+    dF, dQ = d.loadDAEMatrices()
+    for a, b in ((dF, ref["dFdx"]), (dQ, ref["dQdx"])):
+        scale = np.maximum(np.abs(b), 1e-9 * (np.max(np.abs(b)) + 1e-300))
+        assert entry_tol(a, b, scale, abs_floor=1e-30) <= 1
+    # the temperature did change the loads
+    o0 = Oracle(c)
+    r0 = o0.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    assert np.max(np.abs(r0["F"] - ref["F"])) > 1e-6 * np.max(np.abs(ref["F"]))
+
+
 def test_dc_load_parity(gpu):
     c = C.ring(7, seed=9, mc_seed=9)
     d = X.DeviceMgr(c)

@@ -146,9 +146,12 @@ def test_power_grid_512_on_the_device_resident_engine(gpu):
     over the config's whole 0-2 ns window (about 61k steps, breakpoints at the
     sinks' 16,384 corners).  At every accepted step the summed node equations
     (charge balance) hold to round-off, the pads stay at VDD and every node
-    within the supply droop; a progress line is printed every 100 ps."""
+    within the supply droop; a progress line is printed every 100 ps.  With
+    XG_GRID_WALL (seconds) the run stops at the first progress point past that
+    wall-clock budget and checks the part it ran."""
     import os
     t_window = float(os.environ.get("XG_GRID_WINDOW", "50e-12"))
+    wall_budget = float(os.environ.get("XG_GRID_WALL", "inf"))
     nx = 512
     c = C.power_grid(nx, nx, seed=2, pad_pitch=64, n_sinks=4096, tstop=2e-9)
     probe = Oracle(c)
@@ -167,6 +170,9 @@ def test_power_grid_512_on_the_device_resident_engine(gpu):
         print(f"  t = {st['t'] * 1e12:.1f} ps: {st['steps']} steps, {st['rejects']} rejected, {st['newton_iters']} "
               f"Newton iterations, {time.perf_counter() - t0:.1f} s", flush=True)
         if st["status"] != 0 or st["t"] >= t_window * (1 - 1e-12) or st["finished"]:
+            break
+        if time.perf_counter() - t0 > wall_budget:
+            print(f"  wall-clock budget of {wall_budget:.0f} s reached at t = {st['t'] * 1e12:.1f} ps", flush=True)
             break
     wall = time.perf_counter() - t0
     assert st["status"] == 0, st

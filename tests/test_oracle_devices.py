@@ -208,6 +208,84 @@ def test_lambda_and_body_effect():
     assert _drain_current(p, r, nodes[0]) == pytest.approx(ids - ibd, rel=1e-12)
 
 
+# ------------------------------------------------- temperature (TEMP != TNOM)
+K_B, Q_E = 1.3806226e-23, 1.6021918e-19      # the reference's constants (N_UTL_Math / SPICE3)
+T_NOM = 300.15
+
+
+def _eg_si(t):
+    """Silicon band gap in eV, Varshni form with the SPICE silicon parameters."""
+    return 1.16 - 7.02e-4 * t * t / (t + 1108.0)
+
+
+def _phi_of_t(phi0, t, t0=T_NOM):
+    """Surface potential 2 (kT/q) ln(N / n_i(T)) carried from phi0 at t0 with the
+    intrinsic density n_i^2 ~ T^3 exp(-Eg(T) / kT)."""
+    vt, vt0 = K_B * t / Q_E, K_B * t0 / Q_E
+    ln_n_over_ni0 = phi0 / (2 * vt0)
+    ln_ni_ratio = 1.5 * math.log(t / t0) + _eg_si(t0) / (2 * vt0) - _eg_si(t) / (2 * vt)
+    return 2 * vt * (ln_n_over_ni0 - ln_ni_ratio)
+
+
+def _sat_beta_vto(temp, vb=0.0, gamma=0.5, phi=0.7):
+    """beta and threshold of one saturated NMOS (LAMBDA = 0) at `temp`, from its
+    drain current at three gate voltages: I = beta/2 (vg - vto)^2 - i_bd, the
+    reverse drain-bulk diode term i_bd the same at every vg."""
+    vd, vgs = 1.6, (1.3, 1.5, 1.7)
+    cur = []
+    for vg in vgs:
+        c, nodes = _one_mos(vd=vd, vg=vg, vs=0.0, vb=vb, GAMMA=gamma, PHI=phi)
+        o = Oracle(c, temp=temp)
+        o, p, r, _ = _eval(c, nodes, (vd, vg, 0.0, vb), o=o)
+        cur.append(_drain_current(p, r, nodes[0]))
+    d = vgs[1] - vgs[0]
+    beta = (cur[0] - 2 * cur[1] + cur[2]) / (d * d)
+    # I2 - I0 = beta/2 (vg2 - vg0)(vg2 + vg0 - 2 vto)
+    vto = 0.5 * (vgs[2] + vgs[0] - (cur[2] - cur[0]) / (beta / 2 * (vgs[2] - vgs[0])))
+    return beta, vto
+
+
+@pytest.mark.parametrize("temp", [250.0, 350.0, 400.0])
+def test_temperature_scales_mobility_threshold_and_surface_potential(temp):
+    """TEMP != TNOM (Instance::updateTemperature, N_DEV_MOSFET1.C:3487): against
+    device physics, not the update formulas: KP ~ T^-1.5 (mobility); the surface
+    potential 2 (kT/q) ln(N/n_i(T)) (found from the body effect between
+    Vbs = 0 and -1 V); the threshold moving with the flat-band voltage's
+    midgap work-function term -(Eg(T) - Eg(Tnom))/2 - (phi(T) - phi0)/2 plus
+    phi(T) + gamma sqrt(phi(T)) (Vto = Vfb + phi + gamma sqrt(phi))."""
+    gamma, phi0, vto0 = 0.5, 0.7, 0.7
+    b0, v0 = _sat_beta_vto(T_NOM)
+    assert v0 == pytest.approx(vto0, abs=1e-9)
+    b, v = _sat_beta_vto(temp)
+    assert b / b0 == pytest.approx((temp / T_NOM) ** -1.5, rel=1e-9)
+    phi_t = _phi_of_t(phi0, temp)
+    vto_t = (vto0 - (_eg_si(temp) - _eg_si(T_NOM)) / 2 - (phi_t - phi0) / 2 + (phi_t - phi0)
+             + gamma * (math.sqrt(phi_t) - math.sqrt(phi0)))
+    assert v == pytest.approx(vto_t, abs=1e-9)
+    # body effect: vto(vbs) = vto + gamma (sqrt(phi(T) - vbs) - sqrt(phi(T)))
+    _, v_b = _sat_beta_vto(temp, vb=-1.0)
+    assert v_b - v == pytest.approx(gamma * (math.sqrt(phi_t + 1.0) - math.sqrt(phi_t)), abs=1e-9)
+
+
+@pytest.mark.parametrize("temp", [250.0, 400.0])
+def test_junction_current_has_the_thermal_voltage_of_its_temperature(temp):
+    """A forward-biased bulk junction pair (drain, gate and source at 0 V): the
+    bulk current 2 (Is (exp(V/vt) - 1) + gmin V) at two biases fixes vt, which
+    must be k T / q."""
+    from scipy.optimize import brentq
+    vals = []
+    for vb in (0.55, 0.62):
+        c, nodes = _one_mos(vd=0.0, vg=0.0, vs=0.0, vb=vb)
+        o = Oracle(c, temp=temp)
+        o, p, r, _ = _eval(c, nodes, (0.0, 0.0, 0.0, vb), o=o)
+        ib = r["F"][p["node_gid"][nodes[3]]]
+        vals.append((vb, ib / 2 - 1e-12 * vb))
+    (v1, i1), (v2, i2) = vals
+    f = lambda vt: (math.expm1(v2 / vt) / math.expm1(v1 / vt)) - i2 / i1
+    vt = brentq(f, 0.005, 0.1, xtol=1e-16)
+    assert vt == pytest.approx(K_B * temp / Q_E, rel=1e-9)
+
+
 def test_pmos_mirror():
     cn, nodes_n = _one_mos(+1, 0.7, 50e-6, vd=0.6, vg=1.3, vs=0.0, vb=0.0, LAMBDA=0.02)
     cp, nodes_p = _one_mos(-1, -0.7, 50e-6, vd=-0.6, vg=-1.3, vs=0.0, vb=0.0, LAMBDA=0.02)

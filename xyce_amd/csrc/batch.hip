@@ -158,7 +158,7 @@ struct BatchArgs {
   // learned predictor (mode 0 off, 1 collect training pairs, 2 serve)
   int pred_mode;
   int pair_cap;            // training pairs per instance
-  double* xP;              // last 4 accepted solutions, planes of total_n
+  double* xP;              // last 4 accepted solutions and x_cubic, planes of total_n
   long long total_n;
   unsigned short* feat;    // [windows][XG_PRED_FEAT] bf16 features of the current step
   const float* pred_out;   // [windows][XG_PRED_WIN] network output
@@ -276,9 +276,10 @@ __device__ __forceinline__ unsigned short d_f2bf(float f) {
 //   g((x_poly - x_n) / w), g((x_cubic - x_poly) / w), x_n,
 // g(v) = sign(v) log2(1 + |v|) (BF16 keeps 3 digits of every magnitude,
 // so nodes near 0 V are resolved as well as the rails), then the step-size
-// ratios.  The network's output y gives x = x_poly + w y: a proposal within
-// 1/3 of a weight of the solution on every unknown converges in one Newton
-// iteration.
+// ratios.  The network corrects the cubic, computed here in FP64: its output
+// y gives x = x_cubic + w y (y = 0 proposes the cubic itself); a proposal
+// within 1/3 of a weight of the solution on every unknown converges in one
+// Newton iteration.  x_cubic is kept in the fifth history plane.
 __device__ __forceinline__ double pred_weight(double xn, double xp) { return kNlRel * fmax(fabs(xn), fabs(xp)) + kNlAbs; }
 __device__ __forceinline__ float pred_slog(double v) { return (float)copysign(log2(1.0 + fabs(v)), v); }
 
@@ -308,6 +309,7 @@ __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const Class
         const double xp = A.xn0[off + r];
         const double xc = L[0] * x0 + L[1] * x1 + L[2] * x2 + L[3] * x3;
         const double wr = pred_weight(x0, xp);
+        A.xP[4 * TN + off + r] = xc;
         v[0] = pred_slog((x0 - x1) / wr); v[1] = pred_slog((x1 - x2) / wr); v[2] = pred_slog((x2 - x3) / wr);
         v[3] = pred_slog((xp - x0) / wr); v[4] = pred_slog((xc - xp) / wr); v[5] = (float)x0;
       }
@@ -365,7 +367,7 @@ __device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDe
         double tv = 0.0;
         if (ok) {
           const double xp = A.xn0[off + r];
-          tv = (xs[r] - xp) / pred_weight(A.xP[off + r], xp);
+          tv = (xs[r] - A.xP[4 * TN + off + r]) / pred_weight(A.xP[off + r], xp);
           tv = fmin(fmax(tv, -kPredClip), kPredClip);
         }
         A.pair_t[rec * XG_PRED_WIN + j] = (float)tv;
@@ -1313,7 +1315,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
 }
 
 // Resumes the instances waiting for the network's proposal (mode 3):
-// x = x_poly + w y (w the Newton weight of each unknown) when every
+// x = x_cubic + w y (w the Newton weight of each unknown) when every
 // |y| <= kPredTrust and finite, else x = x_poly (the polynomial predictor is
 // the fallback).
 __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
@@ -1338,7 +1340,7 @@ __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
     const int w = r / XG_PRED_WIN;
     const double y = A.pred_out[(size_t)(I.win_off + w) * XG_PRED_WIN + r % XG_PRED_WIN];
     const double xp = A.xn0[off + r];
-    A.x[off + r] = use ? xp + pred_weight(A.xP[off + r], xp) * y : xp;
+    A.x[off + r] = use ? A.xP[4 * A.total_n + off + r] + pred_weight(A.xP[off + r], xp) * y : xp;
   }
   if (threadIdx.x == 0) {
     InstState& S = A.st[inst];
@@ -2005,7 +2007,7 @@ int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity) {
   b->pred_mode = mode; b->net = net; b->pred_rows = rows; b->total_windows = (int)win;
   b->pair_cap = mode == 1 ? pair_capacity : 0; b->total_n = tn; b->total_records = rec;
   if (mode != 0) {
-    HIPCHK(b->d_xP.zeros((size_t)4 * tn));
+    HIPCHK(b->d_xP.zeros((size_t)5 * tn));   // x_n .. x_{n-3}, x_cubic
     HIPCHK(b->d_feat.zeros((size_t)rows * XG_PRED_FEAT));
     HIPCHK(b->d_pred_out.zeros((size_t)rows * XG_PRED_WIN));
   }

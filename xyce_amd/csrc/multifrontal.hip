@@ -168,7 +168,8 @@ struct MfDev {
 };
 
 constexpr int kMfTiny = 64;     // fronts up to this order: one wavefront, the front in LDS
-constexpr int kMfPanel = 32;    // panel width of the blocked factorisation
+constexpr int kMfPanel = 32;    // panel width of the blocked factorisation (64 spills the register panel)
+constexpr int kMfRowsPerLoad = 64 / kMfPanel;   // L21 rows a 64-lane load covers
 constexpr int kMfSolveBlock = 1024;
 
 template <class T>
@@ -437,11 +438,11 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
       T v[kMfPanel];
 #pragma unroll
       for (int it = 0; it < kMfPanel; ++it) {
-        const int rr = (t >> 5) + 2 * it, q = t & 31;
+        const int rr = t / kMfPanel + kMfRowsPerLoad * it, q = t % kMfPanel;
         v[it] = (r0 + rr < ms && q < pw) ? Fs[(long long)(r0 + rr) * ms + p0 + q] : (T)0;
       }
 #pragma unroll
-      for (int it = 0; it < kMfPanel; ++it) Tl[(t >> 5) + 2 * it][t & 31] = v[it];
+      for (int it = 0; it < kMfPanel; ++it) Tl[t / kMfPanel + kMfRowsPerLoad * it][t % kMfPanel] = v[it];
     }
     __syncthreads();
     T a[kMfPanel];
@@ -463,7 +464,7 @@ __global__ void __launch_bounds__(64) k_mf_panel(MfDev d, const int4* ent, int p
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kMfPanel; ++it) {
-      const int rr = (t >> 5) + 2 * it, q = t & 31;
+      const int rr = t / kMfPanel + kMfRowsPerLoad * it, q = t % kMfPanel;
       if (r0 + rr < ms && q < pw) Fs[(long long)(r0 + rr) * ms + p0 + q] = Tl[rr][q];
     }
   } else if (rest > 0) {
@@ -532,7 +533,7 @@ __global__ void __launch_bounds__(256) k_mf_update(MfDev d, const int4* ent, int
   if (e.y <= 0 && e.z <= 0) {
     const T* db = dbuf + (long long)e.w * kMfPanel * kMfPanel;
     for (int q = threadIdx.x; q < pw * kMfPanel; q += blockDim.x) {
-      const int i = q >> 5, j = q & 31;
+      const int i = q / kMfPanel, j = q % kMfPanel;
       if (j < pw) Fs[(long long)(p0 + i) * ms + p0 + j] = db[q];
     }
   }
