@@ -82,6 +82,9 @@ typedef struct xg_solver_state {
   double h;             /* current time step */
   double time;          /* t_{n+1}: time at which sources are evaluated */
   double tstep, tstop;  /* .TRAN step / stop (PULSE defaults) */
+  int tranop;           /* the operating point of a .TRAN (tranopFlag, N_DEV_SolverState.C:334;
+                         * N_ANP_AnalysisManager.C:1170): the capacitances are stamped into dQdx and
+                         * the charge limiter terms as in a transient (N_DEV_MOSFET1.C:4504,4829) */
 } xg_solver_state;
 
 typedef struct xg_circuit xg_circuit;
@@ -174,7 +177,11 @@ int xg_lu_destroy(xg_lu* lu);
  * corrections, to ir_tol); from then on the handle keeps such factors
  * without choosing pivots again. */
 int xg_lu_numeric(xg_lu* lu, const double* vals);
-/* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed. */
+/* Solve A x = b with the last factors (:396); NaN -> XG_E_NAN, x zeroed.
+ * transpose != 0 solves A^T x = b (doSolve(reuse_factors, transpose),
+ * N_LAS_Solver.h:95-107; Amesos SetUseTranspose): level-scheduled method
+ * with FP64 factors only (XG_E_ARG otherwise; Xyce's transient path does not
+ * use it). */
 int xg_lu_solve(xg_lu* lu, const double* b, double* x, int transpose);
 /* xg_lu_numeric / xg_lu_solve on device arrays (HIP device memory); same
  * semantics; each call returns when its outputs are complete. */
@@ -340,12 +347,19 @@ int xg_batch_autotune(xg_batch* b, double* result);
  * mode 0 off; 1 collect training pairs, up to pair_capacity accepted steps
  * per instance; 2 serve `net` (in_dim XG_PRED_FEAT, out_dim XG_PRED_WIN):
  * after each launch the network proposes the Newton initial guess of every
- * instance that starts a step, x = x_poly + w * y per unknown, w its Newton
- * weight (features: see xg_batch_pairs).  A proposal with a non-finite or
+ * instance that starts a step, x = x_base + w * y per unknown, w its Newton
+ * weight (base: xg_batch_predictor_options; features: see xg_batch_pairs).  A proposal with a non-finite or
  * |y| > 512 entry is not used, and a step whose Newton solve
  * fails from a proposal is retried from the polynomial predictor, so the
  * step control and tolerances are those of the run without it. */
 int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity);
+/* No counterpart in the reference.  Options of the learned predictor (before
+ * the first xg_batch_advance; a trained network goes with the options it was
+ * trained under): base 0 = the network corrects the polynomial predictor,
+ * 1 = the cubic through the last four accepted points (both computed in FP64;
+ * the targets of collected pairs are taken from the same base); deadzone =
+ * corrections |y| <= deadzone Newton weights are not applied (default 0). */
+int xg_batch_predictor_options(xg_batch* b, int base, double deadzone);
 /* Collected pairs (mode 1), one record per window of XG_PRED_WIN unknowns
  * of an accepted step, in units of each unknown's Newton weight
  * w = 1e-2 max(|x_n|, |x_poly|) + 1e-6 (the transient Newton weight of the
@@ -355,7 +369,7 @@ int xg_batch_predictor(xg_batch* b, int mode, xg_mlp* net, int pair_capacity);
  * x_n (XG_PRED_WIN each; g(v) = sign(v) log2(1 + |v|), x_cubic the cubic
  * through x_n..x_{n-3} at t_{n+1}), then h_{n+1}/h_n, h_n/h_{n-1},
  * h_{n-1}/h_{n-2}, order, (log10 h_{n+1} + 12) / 4, 1, zeros;
- * target t [XG_PRED_WIN] = (x_{n+1} - x_poly) / w clipped to +-256; mask
+ * target t [XG_PRED_WIN] = (x_{n+1} - x_base) / w clipped to +-256; mask
  * m = 1 where the window row exists.  Copies up to max_records into the device buffers
  * (NULL: count only); *count = records copied (or available). */
 int xg_batch_pairs(xg_batch* b, long long max_records, unsigned short* x_dev, float* t_dev, float* m_dev,

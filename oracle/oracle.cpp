@@ -593,7 +593,7 @@ static void mos1_load_vectors(const Mos1& mi, const SolverState& ss, const Optio
     dFdxdVp[mi.li_dp] += j5 * M;
     dFdxdVp[mi.li_sp] += j6 * M;
     double gcgd = 0, gcgs = 0, gcgb = 0, gcbs = 0, gcbd = 0;
-    if (ss.tran) { gcgd = mi.Capgd; gcgs = mi.Capgs; gcgb = mi.Capgb; gcbs = mi.capbs; gcbd = mi.capbd; }
+    if (ss.tran || ss.tranop) { gcgd = mi.Capgd; gcgs = mi.Capgs; gcgb = mi.Capgb; gcbs = mi.capbs; gcbd = mi.capbd; }
     double q2 = Dtype * (gcgd * (mi.vgd - mi.vgd_orig) + gcgs * (mi.vgs - mi.vgs_orig) +
                          gcgb * (mi.vgs - mi.vgs_orig - mi.vbs + mi.vbs_orig));
     double q4 = Dtype * (-(gcgb) * (mi.vgs - mi.vgs_orig - mi.vbs + mi.vbs_orig) + (gcgb) * (mi.vbd - mi.vbd_orig) +
@@ -633,7 +633,7 @@ static void mos1_load_matrices(const Mos1& mi, const SolverState& ss, double* dF
   addv(dFdx, f[15], -((mi.gds + mi.revsum) * M));
   addv(dFdx, f[16], (gs + mi.gds + mi.gbs + mi.nrmsum) * M);
   double gcgd = 0, gcgs = 0, gcgb = 0, gcbs = 0, gcbd = 0;
-  if (ss.tran) { gcgd = mi.Capgd; gcgs = mi.Capgs; gcgb = mi.Capgb; gcbs = mi.capbs; gcbd = mi.capbd; }
+  if (ss.tran || ss.tranop) { gcgd = mi.Capgd; gcgs = mi.Capgs; gcgb = mi.Capgb; gcbs = mi.capbs; gcbd = mi.capbd; }
   const int* q = mi.qslot;
   addv(dQdx, q[0], (gcgd + gcgs + gcgb) * M);
   addv(dQdx, q[1], -(gcgb * M));

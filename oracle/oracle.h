@@ -40,6 +40,7 @@ enum IP {
 
 struct SolverState {
   int dcop = 0, tran = 0, initJct = 0, initFix = 0;
+  int tranop = 0;        // the operating point of a .TRAN (SolverState::tranopFlag)
   int newtonIter = 0;
   int order = 1;
   double h = 0.0;        // current time step

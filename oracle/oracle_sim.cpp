@@ -201,6 +201,7 @@ struct Work {
   int dc_strategy = 0;               // 0 plain Newton, 1 gmin stepping, 2 source stepping (the last DC solve)
   int dc_first_stage = 0;            // tests: start the strategy at this stage (0 = the reference's order)
   int uic = 0;                       // .TRAN ... UIC: no operating point, the run starts from x = 0
+  int tranop = 0;                    // the operating point being solved is a .TRAN's (tranopFlag)
   std::vector<int> vrow, vdiag;      // node-voltage rows and their diagonal CSR slots
   explicit Work(Circuit* cc) : c(cc) {
     n = c->n; nnz = (int)c->colind.size();
@@ -236,7 +237,7 @@ static void load_rhs(Work& w, const SolverState& ss) {
   const int n = w.n;
   if (w.load_hook) {
     // the caller's loads: vectors and matrices at this x, as orc_load returns them
-    const int ssi[6] = {ss.dcop, ss.tran, ss.initJct, ss.initFix, ss.newtonIter, ss.order};
+    const int ssi[7] = {ss.dcop, ss.tran, ss.initJct, ss.initFix, ss.newtonIter, ss.order, ss.tranop};
     const double ssd[4] = {ss.h, ss.time, w.tstep, w.tstop};
     w.x[n] = 0.0;
     w.limited = w.load_hook(w.hook_ctx, ssi, ssd, w.x.data(), w.staCur.data(), w.stoCur.data(), w.staNext.data(),
@@ -532,7 +533,7 @@ template <class SetParam>
 static int dc_continuation(Work& w, long* iters, double p0, double p1, double step, double min_step, double max_step,
                            double aggressiveness, bool init_jct, SetParam set_param) {
   const int n = w.n;
-  SolverState base; base.dcop = 1; base.tran = 0; base.time = 0.0;
+  SolverState base; base.dcop = 1; base.tran = 0; base.tranop = w.tranop; base.time = 0.0;
   NLP p{1e-3, 1e-12, 1.0, 1e-6, 1e-6, 20, false};
   std::vector<double> xc(n, 0.0);
   std::fill(w.x.begin(), w.x.end(), 0.0);
@@ -574,7 +575,7 @@ static int dc_continuation(Work& w, long* iters, double p0, double p1, double st
 // (all independent sources scaled from 0 to 1).
 static int dcop(Work& w, long* iters) {
   const int n = w.n;
-  SolverState base; base.dcop = 1; base.tran = 0; base.time = 0.0;
+  SolverState base; base.dcop = 1; base.tran = 0; base.tranop = w.tranop; base.time = 0.0;
   NLP p{1e-3, 1e-12, 1.0, 1e-6, 1e-6, 200, false};
   std::vector<double> xc(n, 0.0);
   std::fill(w.x.begin(), w.x.end(), 0.0);
@@ -659,7 +660,9 @@ static TranResult tran(Work& w, double tstop, double tstep, double hmax, bool re
     load_rhs(w, ss);
     w.dc_strategy = -1;
   } else {
+    w.tranop = 1;   // tranopFlag: a transient with no time integration yet (N_ANP_AnalysisManager.C:1170-1174)
     int st = dcop(w, &dc_iters);
+    w.tranop = 0;
     if (st <= 0) { R.status = -100; return R; }
   }
   // tranop -> transient
@@ -948,13 +951,13 @@ void orc_pattern(OrcHandle* h, int* rowptr, int* colind, int* node_gid, int* mos
 }
 
 // One device load at x: updateState + loadDAEVectors + loadDAEMatrices.
-// ssi: dcop tran initJct initFix newtonIter order ; ssd: h time tstep tstop
+// ssi: dcop tran initJct initFix newtonIter order tranop ; ssd: h time tstep tstop
 int orc_load(OrcHandle* h, const int* ssi, const double* ssd, const double* x,
              const double* sta_cur, const double* sto_cur, double* sta_next, double* sto_next,
              double* F, double* Q, double* B, double* dFdxdVp, double* dQdxdVp, double* dFdx, double* dQdx) {
   Work& w = *h->w;
   SolverState ss; ss.dcop = ssi[0]; ss.tran = ssi[1]; ss.initJct = ssi[2]; ss.initFix = ssi[3];
-  ss.newtonIter = ssi[4]; ss.order = ssi[5]; ss.h = ssd[0]; ss.time = ssd[1];
+  ss.newtonIter = ssi[4]; ss.order = ssi[5]; ss.tranop = ssi[6]; ss.h = ssd[0]; ss.time = ssd[1];
   w.tstep = ssd[2]; w.tstop = ssd[3];
   std::copy(x, x + w.n, w.x.begin()); w.x[w.n] = 0.0;
   std::copy(sta_cur, sta_cur + w.c->nsta, w.staCur.begin());

@@ -106,8 +106,8 @@ class Oracle:
                     mos_stasto=ss[: 2 * self.n_mos].reshape(-1, 2))
 
     def load(self, x, sta_cur, sto_cur, sta_next=None, sto_next=None, *, dcop=0, tran=1, init_jct=0,
-             newton_iter=1, order=1, h=1e-11, time=0.0, tstep=1e-12, tstop=1e-9):
-        ssi = np.array([dcop, tran, init_jct, 0, newton_iter, order], np.int32)
+             newton_iter=1, order=1, h=1e-11, time=0.0, tstep=1e-12, tstop=1e-9, tranop=0):
+        ssi = np.array([dcop, tran, init_jct, 0, newton_iter, order, tranop], np.int32)
         ssd = np.array([h, time, tstep, tstop], np.float64)
         x = np.ascontiguousarray(x, np.float64)
         sta_next = np.array(sta_cur if sta_next is None else sta_next, np.float64)

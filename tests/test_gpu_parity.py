@@ -108,8 +108,9 @@ def test_load_parity_with_caps(gpu, newton_iter, order):
 
 
 @pytest.mark.parametrize("seed", [0, 1])
-@pytest.mark.parametrize("dcop,init_jct,newton_iter", [(1, 1, 0), (1, 0, 1), (0, 0, 0), (0, 0, 2)])
-def test_device_zoo_load_parity(gpu, seed, dcop, init_jct, newton_iter):
+@pytest.mark.parametrize("dcop,init_jct,newton_iter,tranop", [(1, 1, 0, 0), (1, 0, 1, 0), (1, 1, 0, 1), (1, 0, 1, 1),
+                                                               (0, 0, 0, 0), (0, 0, 2, 0)])
+def test_device_zoo_load_parity(gpu, seed, dcop, init_jct, newton_iter, tranop):
     """200 independent devices over four model cards (every depletion-charge
     form) at random terminal voltages: every operating branch of
     updateIntermediateVars occurs (checked on the oracle's output, as in
@@ -121,8 +122,11 @@ def test_device_zoo_load_parity(gpu, seed, dcop, init_jct, newton_iter):
     o = Oracle(c)
     x, sta0, sto0 = zoo_states(o, seed)
     kw = dict(dcop=dcop, tran=1 - dcop, init_jct=init_jct, newton_iter=newton_iter, order=2, h=3e-12,
-              time=0.4e-9, tstep=1e-12, tstop=5e-9)
+              time=0.4e-9, tstep=1e-12, tstop=5e-9, tranop=tranop)
     ref = o.load(x, sta0, sto0, sta_next=sta0, sto_next=sto0, **kw)
+    if tranop:
+        # a transient's operating point stamps the capacitances (the zoo has Meyer and junction caps)
+        assert np.max(np.abs(ref["dQdx"])) > 0
     if not init_jct:
         assert ALL_BRANCHES <= branch_classes(c, o.pattern()["mos_stasto"], ref["sto_next"])
     d.set_solver_state(**kw)
@@ -208,6 +212,37 @@ def test_lu_solve_parity(gpu, stages):
     res = np.linalg.norm(A @ xg - rhs) / np.linalg.norm(rhs)
     assert res <= 1e-12, res
     np.testing.assert_allclose(xg, xn, rtol=1e-9, atol=1e-12 * np.max(np.abs(xn)))
+
+
+@pytest.mark.parametrize("stages", [5, 101])
+def test_lu_transpose_solve(gpu, stages):
+    """doSolve(reuse_factors, transpose) (N_LAS_Solver.h:95-107): A^T x = b
+    from the factors of A, against numpy, and again after a refactor with new
+    values; the multifrontal method and FP32 factors refuse it."""
+    c = C.ring(stages, seed=stages, mc_seed=stages)
+    o = Oracle(c)
+    x, sta0, sto0 = _loaded_state(o, stages)
+    ref = o.load(x, sta0, sto0, h=5e-12, order=2)
+    rhs, J = o.combine(ref, ref["Q"], ref["F"] - ref["B"])
+    p = o.pattern()
+    s = X.DirectSolver(o.n, p["rowptr"], p["colind"], method=1)
+    for vals in (J, J * (1.0 + 0.1 * np.random.default_rng(stages).uniform(size=J.size))):
+        A = np.zeros((o.n, o.n))
+        for r in range(o.n):
+            for k in range(p["rowptr"][r], p["rowptr"][r + 1]):
+                A[r, p["colind"][k]] += vals[k]
+        st, xt = s.doSolve(vals, rhs, transpose=True)
+        assert st == 0
+        res = np.linalg.norm(A.T @ xt - rhs) / np.linalg.norm(rhs)
+        assert res <= 1e-12, res
+        np.testing.assert_allclose(xt, np.linalg.solve(A.T, rhs), rtol=1e-9, atol=1e-12 * np.max(np.abs(xt)))
+        # the plain solve from the same factors still solves A x = b
+        st, xn = s.solve(rhs)
+        assert np.linalg.norm(A @ xn - rhs) / np.linalg.norm(rhs) <= 1e-12
+    s2 = X.DirectSolver(o.n, p["rowptr"], p["colind"], method=2)
+    assert s2.numeric(J) == 0
+    with pytest.raises(X.XgError):
+        s2.solve(rhs, transpose=True)
 
 
 def test_lu_singular_returns_status(gpu):
@@ -423,10 +458,10 @@ def test_reference_driver_drives_the_dropin(gpu):
 
     def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
         try:
-            si, sd = arr(ssi, 6), arr(ssd, 4)
+            si, sd = arr(ssi, 7), arr(ssd, 4)
             dm.set_solver_state(dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
                                 order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]),
-                                tstop=float(sd[3]))
+                                tstop=float(sd[3]), tranop=int(si[6]))
             sta_n, sto_n, lim = dm.updateState(arr(x, n)[:n].copy(), arr(sc, nsta)[:nsta].copy(),
                                                arr(oc, nsto)[:nsto].copy(), sto_next=arr(on, nsto)[:nsto].copy())
             arr(sn, nsta)[:nsta] = sta_n

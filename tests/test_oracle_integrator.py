@@ -147,11 +147,12 @@ def test_driver_hooks_with_oracle_loads_reproduce_the_run():
     p = inner.pattern()
 
     def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
-        si, sd = arr(ssi, 6), arr(ssd, 4)
+        si, sd = arr(ssi, 7), arr(ssd, 4)
         out = inner.load(arr(x, n)[:n].copy(), arr(sc, nsta)[:nsta].copy(), arr(oc, nsto)[:nsto].copy(),
                          sta_next=arr(sn, nsta)[:nsta].copy(), sto_next=arr(on, nsto)[:nsto].copy(),
                          dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
-                         order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]), tstop=float(sd[3]))
+                         order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]), tstop=float(sd[3]),
+                         tranop=int(si[6]))
         arr(sn, nsta)[:nsta] = out["sta_next"][:nsta]
         arr(on, nsto)[:nsto] = out["sto_next"][:nsto]
         for name, ptr in (("F", F), ("Q", Q), ("B", B), ("dFdxdVp", dFv), ("dQdxdVp", dQv)):

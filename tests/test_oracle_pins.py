@@ -197,12 +197,13 @@ def _record_run(c, tstop, tstep, hmax):
     log = []
 
     def load(ctx, ssi, ssd, x, sc, oc, sn, on, F, Q, B, dFv, dQv, dFdx, dQdx):
-        si, sd = arr(ssi, 6), arr(ssd, 4)
+        si, sd = arr(ssi, 7), arr(ssd, 4)
         xv = arr(x, n)[:n].copy()
         out = inner.load(xv, arr(sc, nsta)[:nsta].copy(), arr(oc, nsto)[:nsto].copy(),
                          sta_next=arr(sn, nsta)[:nsta].copy(), sto_next=arr(on, nsto)[:nsto].copy(),
                          dcop=int(si[0]), tran=int(si[1]), init_jct=int(si[2]), newton_iter=int(si[4]),
-                         order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]), tstop=float(sd[3]))
+                         order=int(si[5]), h=float(sd[0]), time=float(sd[1]), tstep=float(sd[2]), tstop=float(sd[3]),
+                         tranop=int(si[6]))
         arr(sn, nsta)[:nsta] = out["sta_next"][:nsta]
         arr(on, nsto)[:nsto] = out["sto_next"][:nsto]
         for name, ptr in (("F", F), ("Q", Q), ("B", B), ("dFdxdVp", dFv), ("dQdxdVp", dQv)):
@@ -273,3 +274,31 @@ def test_newton_stops_where_the_weighted_update_rule_says():
     # so a different tolerance would change the iteration counts
     assert in_window >= 10
     assert sum(len(s["x"]) - 1 for s in solves) == r["newton_iters"]
+
+
+# ---------------------------------------------------------------- tranop
+def test_tranop_stamps_the_operating_point_capacitances():
+    """At the operating point of a .TRAN (tranopFlag, N_DEV_SolverState.C:334)
+    the reference stamps the gate capacitances into dQdx
+    (N_DEV_MOSFET1.C:4829-4836); at a plain operating point they are zero.
+    In saturation the DC Meyer values are the textbook ones: Cgs = 2/3 Cox W L
+    plus overlap, Cgd and Cgb the overlaps alone."""
+    vto, phi, tox, w, l = 0.7, 0.6, 2e-8, 2e-6, 0.5e-6
+    cgso, cgdo, cgbo = 3e-10, 2e-10, 1e-10
+    c, nodes = _mos_circuit(+1, vto, TOX=tox, CGSO=cgso, CGDO=cgdo, CGBO=cgbo, PHI=phi)
+    o = Oracle(c, gmin=GMIN)
+    p = o.pattern()
+    d, g, s, b = (int(p["node_gid"][nd]) for nd in nodes)
+    x = np.zeros(o.n)
+    x[d], x[g] = 1.0, 1.5             # vgs - vto = 0.8 < vds: saturation
+    sta, sto = np.zeros(o.nsta), np.zeros(o.nsto)
+    cox = EPSOX / tox * w * l
+    cgs, cgd, cgb = 2.0 / 3.0 * cox + cgso * w, cgdo * w, cgbo * l
+    for tranop in (0, 1):
+        r = o.load(x, sta, sto, dcop=1, tran=0, newton_iter=0, tranop=tranop)
+        A = _dense(o, p, r["dQdx"])
+        if not tranop:
+            assert np.all(A == 0.0)
+            continue
+        np.testing.assert_allclose([A[g, g], A[g, s], A[g, d], A[g, b]], [cgs + cgd + cgb, -cgs, -cgd, -cgb],
+                                   rtol=1e-12)

@@ -45,7 +45,7 @@ EXPORTS = [
     "xg_batch_create", "xg_batch_destroy", "xg_batch_advance", "xg_batch_counters", "xg_batch_instance",
     "xg_batch_points", "xg_batch_autotune", "xg_batch_config", "xg_batch_set_method", "xg_batch_dc_strategy", "xg_batch_dc_stage", "xg_batch_group_info", "xg_batch_profile", "xg_host_pattern", "xg_host_lu_analyse", "xg_host_mf_plan",
     "xg_mlp_create", "xg_mlp_destroy", "xg_mlp_params", "xg_mlp_forward", "xg_mlp_loss_backward", "xg_mlp_adam",
-    "xg_mlp_sync", "xg_batch_predictor", "xg_batch_pairs", "xg_batch_pred_counters",
+    "xg_mlp_sync", "xg_batch_predictor", "xg_batch_predictor_options", "xg_batch_pairs", "xg_batch_pred_counters",
     "xg_tran_create", "xg_tran_destroy", "xg_tran_set_precision", "xg_tran_set_tolerances", "xg_tran_set_record",
     "xg_tran_run", "xg_tran_points", "xg_tran_solution", "xg_tran_set_progress", "xg_tran_set_uic",
 ]
@@ -76,7 +76,7 @@ class CircuitDesc(C.Structure):
 class SolverState(C.Structure):
     _fields_ = [("dcop", C.c_int), ("tran", C.c_int), ("init_jct", C.c_int), ("init_fix", C.c_int),
                 ("newton_iter", C.c_int), ("order", C.c_int), ("h", C.c_double), ("time", C.c_double),
-                ("tstep", C.c_double), ("tstop", C.c_double)]
+                ("tstep", C.c_double), ("tstop", C.c_double), ("tranop", C.c_int)]
 
 
 def lib():
@@ -190,8 +190,8 @@ class DeviceMgr:
                     mos_stasto=ss[: 2 * self.n_mos].reshape(-1, 2))
 
     def set_solver_state(self, *, dcop=0, tran=1, init_jct=0, newton_iter=1, order=1, h=1e-11, time=0.0,
-                         tstep=1e-12, tstop=1e-9):
-        ss = SolverState(dcop, tran, init_jct, 0, newton_iter, order, h, time, tstep, tstop)
+                         tstep=1e-12, tstop=1e-9, tranop=0):
+        ss = SolverState(dcop, tran, init_jct, 0, newton_iter, order, h, time, tstep, tstop, tranop)
         _check(lib().xg_set_solver_state(self.h, C.byref(ss)), "set_solver_state")
         self._ss = ss
 
@@ -277,22 +277,23 @@ class DirectSolver:
         v = np.ascontiguousarray(values, np.float64)
         return _check(lib().xg_lu_numeric(self.h, _p(v)), "xg_lu_numeric")
 
-    def solve(self, b):
+    def solve(self, b, transpose=False):
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self.n)
-        st = lib().xg_lu_solve(self.h, _p(b), _p(x), 0)
+        st = lib().xg_lu_solve(self.h, _p(b), _p(x), int(bool(transpose)))
         if st == -99:
             return st, x
         _check(st, "xg_lu_solve")
         return st, x
 
-    def doSolve(self, values, b, reuse_factors=False):
-        """Linear::Solver::doSolve: returns (status, x); status != 0 on failure."""
+    def doSolve(self, values, b, reuse_factors=False, transpose=False):
+        """Linear::Solver::doSolve: returns (status, x); status != 0 on failure;
+        transpose solves A^T x = b (level-scheduled method, FP64 factors)."""
         if not reuse_factors:
             st = self.numeric(values)
             if st > 0:
                 return st, np.zeros(self.n)
-        return self.solve(b)
+        return self.solve(b, transpose)
 
 
 class Batch:
@@ -375,9 +376,14 @@ class Batch:
                 break
         return self.counters()
 
-    def set_predictor(self, mode, net=None, pair_capacity=0):
-        """Learned predictor: mode 0 off, 1 collect training pairs, 2 serve `net` (an MLP)."""
+    def set_predictor(self, mode, net=None, pair_capacity=0, base="poly", deadzone=0.0):
+        """Learned predictor: mode 0 off, 1 collect training pairs, 2 serve `net` (an MLP).
+        The network corrects ``base``: 'poly' (the polynomial predictor) or 'cubic'
+        (the cubic through the last four points); corrections of at most
+        ``deadzone`` Newton weights are not applied."""
         self._net = net
+        _check(lib().xg_batch_predictor_options(self.h, {"poly": 0, "cubic": 1}[base], C.c_double(deadzone)),
+               "xg_batch_predictor_options")
         _check(lib().xg_batch_predictor(self.h, int(mode), net.h if net is not None else None, int(pair_capacity)),
                "xg_batch_predictor")
 

@@ -157,6 +157,8 @@ struct BatchArgs {
   int* lim;
   // learned predictor (mode 0 off, 1 collect training pairs, 2 serve)
   int pred_mode;
+  int pred_base;           // the network corrects 0: the polynomial predictor, 1: the cubic extrapolation
+  double pred_dead;        // corrections |y| <= pred_dead (Newton weights) are not applied
   int pair_cap;            // training pairs per instance
   double* xP;              // last 4 accepted solutions and x_cubic, planes of total_n
   long long total_n;
@@ -169,10 +171,10 @@ struct BatchArgs {
 
 constexpr double kRelTol = 1e-3, kAbsTol = 1e-6;                       // TIA defaults
 constexpr double kNlRel = 1e-2, kNlAbs = 1e-6, kDxTol = 0.33, kRhsTol = 1e-2;  // transient Newton
-// learned predictor: training targets clipped to +-kPredClip Newton weights;
-// proposals with an entry beyond kPredTrust weights are not used
+// learned predictor: training targets clipped to +-kPredClip Newton weights
+// before compression; proposals with an output beyond g(512) = 9 are not used
 constexpr double kPredClip = 256.0;
-constexpr float kPredTrust = 512.0f;
+constexpr float kPredTrustLog = 9.0f;
 constexpr int kMaxNewton = 20;
 constexpr double kDcRel = 1e-3, kDcAbs = 1e-12, kDcDxTol = 1.0, kDcRhsTol = 1e-6;
 constexpr int kDcMaxNewton = 200;
@@ -276,12 +278,16 @@ __device__ __forceinline__ unsigned short d_f2bf(float f) {
 //   g((x_poly - x_n) / w), g((x_cubic - x_poly) / w), x_n,
 // g(v) = sign(v) log2(1 + |v|) (BF16 keeps 3 digits of every magnitude,
 // so nodes near 0 V are resolved as well as the rails), then the step-size
-// ratios.  The network corrects the cubic, computed here in FP64: its output
-// y gives x = x_cubic + w y (y = 0 proposes the cubic itself); a proposal
-// within 1/3 of a weight of the solution on every unknown converges in one
-// Newton iteration.  x_cubic is kept in the fifth history plane.
+// ratios.  The network corrects a base extrapolation computed here in FP64,
+// the polynomial predictor or the cubic (pred_base), in the same compressed
+// units: its output g gives y = sign(g)(2^|g| - 1) and x = x_base + w y
+// (g = 0 proposes the base itself), so that a correction of a fraction of a
+// weight is learned as precisely, relative to its size, as one of hundreds; a
+// proposal within 1/3 of a weight of the solution on every unknown converges
+// in one Newton iteration.  x_base is kept in the fifth history plane.
 __device__ __forceinline__ double pred_weight(double xn, double xp) { return kNlRel * fmax(fabs(xn), fabs(xp)) + kNlAbs; }
 __device__ __forceinline__ float pred_slog(double v) { return (float)copysign(log2(1.0 + fabs(v)), v); }
+__device__ __forceinline__ double pred_sexp(double g) { return copysign(exp2(fabs(g)) - 1.0, g); }
 
 __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const ClassDesc& C, InstState& S) {
   const int n = C.n, off = I.vec_off, tid = threadIdx.x, nth = blockDim.x;
@@ -309,7 +315,7 @@ __device__ void pred_features(const BatchArgs& A, const InstDesc& I, const Class
         const double xp = A.xn0[off + r];
         const double xc = L[0] * x0 + L[1] * x1 + L[2] * x2 + L[3] * x3;
         const double wr = pred_weight(x0, xp);
-        A.xP[4 * TN + off + r] = xc;
+        A.xP[4 * TN + off + r] = A.pred_base ? xc : xp;
         v[0] = pred_slog((x0 - x1) / wr); v[1] = pred_slog((x1 - x2) / wr); v[2] = pred_slog((x2 - x3) / wr);
         v[3] = pred_slog((xp - x0) / wr); v[4] = pred_slog((xc - xp) / wr); v[5] = (float)x0;
       }
@@ -368,7 +374,7 @@ __device__ void pred_accept(const BatchArgs& A, const InstDesc& I, const ClassDe
         if (ok) {
           const double xp = A.xn0[off + r];
           tv = (xs[r] - A.xP[4 * TN + off + r]) / pred_weight(A.xP[off + r], xp);
-          tv = fmin(fmax(tv, -kPredClip), kPredClip);
+          tv = pred_slog(fmin(fmax(tv, -kPredClip), kPredClip));
         }
         A.pair_t[rec * XG_PRED_WIN + j] = (float)tv;
         A.pair_m[rec * XG_PRED_WIN + j] = ok ? 1.f : 0.f;
@@ -671,6 +677,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   if (FUSED) {
     MosFlags fl;
     fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0 && S.cont_first != 0.0;
+    fl.tranop = dc;   // every batch runs a transient: its DC point is a transient's operating point
     fl.newton_iter = S.newton_iter;
     const int* lid = ci + C.o_lid;
     const int* tpos = ci + C.o_tpos;
@@ -1296,6 +1303,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const bool dc = mode == 0;
   MosFlags fl;
   fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0 && first_solve; fl.newton_iter = newton_iter;
+  fl.tranop = dc;
   const int* lid = A.ci + C.o_lid;
   const int* tpos = A.ci + C.o_tpos;
   double* T = A.terms + I.term_off;
@@ -1315,9 +1323,10 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
 }
 
 // Resumes the instances waiting for the network's proposal (mode 3):
-// x = x_cubic + w y (w the Newton weight of each unknown) when every
-// |y| <= kPredTrust and finite, else x = x_poly (the polynomial predictor is
-// the fallback).
+// x = x_base + w y, y = sign(g)(2^|g| - 1) from the network's output g (w the
+// Newton weight of each unknown; y taken as 0 where |y| <= pred_dead) when
+// every |g| <= kPredTrustLog and finite, else x = x_poly (the polynomial
+// predictor is the fallback).
 __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
   const int inst = blockIdx.x;
   if (inst >= A.n_inst) return;
@@ -1331,16 +1340,17 @@ __global__ void __launch_bounds__(256) k_pred_apply(BatchArgs A) {
   int b = 0;
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
     const float y = A.pred_out[(size_t)(I.win_off + r / XG_PRED_WIN) * XG_PRED_WIN + r % XG_PRED_WIN];
-    if (!(fabsf(y) <= kPredTrust)) b = 1;
+    if (!(fabsf(y) <= kPredTrustLog)) b = 1;
   }
   if (b) atomicOr(&bad, 1);
   __syncthreads();
   const bool use = bad == 0;
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
     const int w = r / XG_PRED_WIN;
-    const double y = A.pred_out[(size_t)(I.win_off + w) * XG_PRED_WIN + r % XG_PRED_WIN];
+    const double y = pred_sexp(A.pred_out[(size_t)(I.win_off + w) * XG_PRED_WIN + r % XG_PRED_WIN]);
     const double xp = A.xn0[off + r];
-    A.x[off + r] = use ? A.xP[4 * A.total_n + off + r] + pred_weight(A.xP[off + r], xp) * y : xp;
+    const double yc = fabs(y) > A.pred_dead ? y : 0.0;
+    A.x[off + r] = use ? A.xP[4 * A.total_n + off + r] + pred_weight(A.xP[off + r], xp) * yc : xp;
   }
   if (threadIdx.x == 0) {
     InstState& S = A.st[inst];
@@ -1921,6 +1931,8 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.n_kcls = (int)(kcls.size() / KSTRIDE); A.iters = 1; A.prof = nullptr;
   A.dense = b->d_dense.p;
   A.pred_mode = 0;
+  A.pred_base = 0;
+  A.pred_dead = 0.0;
   A.dev_inst = b->d_dev_inst.p; A.lim = b->d_lim.p;
   b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
   for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu, C.blob_len));
@@ -1975,6 +1987,15 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   b->last_seconds = ms * 1e-3;
   b->launches += launches;
+  return XG_OK;
+}
+
+int xg_batch_predictor_options(xg_batch* b, int base, double deadzone) {
+  if (!b || base < 0 || base > 1 || !(deadzone >= 0.0)) return XG_E_ARG;
+  if (b->launches != 0) return XG_E_ARG;
+  b->args.pred_base = base;
+  b->args.pred_dead = deadzone;
+  b->drop_graph();
   return XG_OK;
 }
 

@@ -421,6 +421,30 @@ __global__ void __launch_bounds__(1024) k_lu_solve_1wg(int n, const int* __restr
   if (bad) atomicOr(nan_flag, 1);
 }
 
+// transpose solves B^T z = c = U^T L^T z: U^T by the backward levels in reverse
+// (w_j = (c_j - sum_i U_ij w_i) / U_jj over column j of U), then L^T by the
+// forward levels in reverse (z_i = w_i - sum_j L_ji z_j over column i of L);
+// a column's entries are the CSR positions listed in cptr/cpos with their rows
+// in crow
+__global__ void k_solve_ut_level(int t0, int t1, const int* rows, const int* cptr, const int* cpos, const int* crow,
+                                 const int* diag, const double* lu, double* y) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const int j = rows[t];
+  double v = y[j];
+  for (int k = cptr[j]; k < cptr[j + 1]; ++k) v -= lu[cpos[k]] * y[crow[k]];
+  y[j] = v / lu[diag[j]];
+}
+__global__ void k_solve_lt_level(int t0, int t1, const int* rows, const int* cptr, const int* cpos, const int* crow,
+                                 const double* lu, double* y) {
+  const int t = t0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const int i = rows[t];
+  double v = y[i];
+  for (int k = cptr[i]; k < cptr[i + 1]; ++k) v -= lu[cpos[k]] * y[crow[k]];
+  y[i] = v;
+}
+
 template <class T>
 __global__ void k_solve_fwd_level(int t0, int t1, const int* fl_rows, const int* lu_rowptr, const int* lu_colind,
                                   const int* diag, const T* lu, T* y) {
@@ -577,6 +601,9 @@ struct xg_lu {
   bool own_stream = false;
   DBuf<int> a2lu, lu_rowptr, lu_colind, lu_diag, f_lvl_ptr, f_tgt, f_op_ptr, f_ops, f_div;
   DBuf<int> fs_ptr, fs_rows, bs_ptr, bs_rows, prow, pcol, flag, arp, aci;
+  // column lists of U (above the diagonal) and L (below it) for transpose solves, built on first use
+  DBuf<int> ut_ptr, ut_pos, ut_row, lt_ptr, lt_pos, lt_row;
+  bool have_t = false;
   DBuf<int> long_rows, lcb, lce, lrc;   // rows of A with many entries: chunks for the residual
   DBuf<double> lpart;
   int n_long = 0, n_lchunks = 0;
@@ -673,6 +700,7 @@ struct xg_lu {
     HIPCHK(x.alloc(s.n)); HIPCHK(r.alloc(s.n)); HIPCHK(d.alloc(s.n));
     if (mode == 1) { HIPCHK(lu32.alloc(s.nlu)); HIPCHK(y32.alloc(s.n)); }
     h_flvl = s.f_lvl_ptr; h_fs = s.fs_lvl_ptr; h_bs = s.bs_lvl_ptr;
+    have_t = false;   // a new pattern: the transpose column lists are rebuilt when next needed
     const int nlev = (int)s.f_lvl_ptr.size() - 1;
     one_wg = s.nlu <= (1 << 20) && nlev <= 4096 && s.n <= (1 << 16);
     have64 = false;
@@ -706,6 +734,52 @@ struct xg_lu {
     HIPCHK(hipStreamSynchronize(stream));
     *bad_out = res[0] == 0x7fffffff ? 0 : res[0];
     *hard_out = res[1];
+    return XG_OK;
+  }
+  // column lists of the factors' pattern for transpose solves (level path)
+  int setup_transpose() {
+    const int n = s.n;
+    std::vector<int> uptr(n + 1, 0), lptr(n + 1, 0);
+    for (int i = 0; i < n; ++i)
+      for (int p = s.lu_rowptr[i]; p < s.lu_rowptr[i + 1]; ++p) {
+        const int j = s.lu_colind[p];
+        if (j > i) uptr[j + 1]++;
+        else if (j < i) lptr[j + 1]++;
+      }
+    for (int j = 0; j < n; ++j) { uptr[j + 1] += uptr[j]; lptr[j + 1] += lptr[j]; }
+    std::vector<int> upos(uptr[n]), urow(uptr[n]), lpos(lptr[n]), lrow(lptr[n]);
+    std::vector<int> uf(uptr.begin(), uptr.end() - 1), lf(lptr.begin(), lptr.end() - 1);
+    for (int i = 0; i < n; ++i)
+      for (int p = s.lu_rowptr[i]; p < s.lu_rowptr[i + 1]; ++p) {
+        const int j = s.lu_colind[p];
+        if (j > i) { upos[uf[j]] = p; urow[uf[j]++] = i; }
+        else if (j < i) { lpos[lf[j]] = p; lrow[lf[j]++] = i; }
+      }
+    hipStream_t st = stream;
+    HIPCHK(ut_ptr.upload(uptr, st)); HIPCHK(ut_pos.upload(upos, st)); HIPCHK(ut_row.upload(urow, st));
+    HIPCHK(lt_ptr.upload(lptr, st)); HIPCHK(lt_pos.upload(lpos, st)); HIPCHK(lt_row.upload(lrow, st));
+    HIPCHK(hipStreamSynchronize(st));
+    have_t = true;
+    return XG_OK;
+  }
+  // A^T x = b with the FP64 factors of B = P_r A P_c: B^T (P_r x) = P_c^T b
+  int solve_transpose(const double* db, double* dx, int* nan_flag) {
+    const int n = s.n;
+    if (!have_t) {
+      const int rc = setup_transpose();
+      if (rc != XG_OK) return rc;
+    }
+    k_permute_in<double><<<nblk(n), 256, 0, stream>>>(n, pcol.p, db, y.p);
+    for (size_t l = h_bs.size() - 1; l-- > 0;)
+      if (h_bs[l + 1] > h_bs[l])
+        k_solve_ut_level<<<nblk(h_bs[l + 1] - h_bs[l]), 256, 0, stream>>>(h_bs[l], h_bs[l + 1], bs_rows.p, ut_ptr.p,
+                                                                          ut_pos.p, ut_row.p, lu_diag.p, lu.p, y.p);
+    for (size_t l = h_fs.size() - 1; l-- > 0;)
+      if (h_fs[l + 1] > h_fs[l])
+        k_solve_lt_level<<<nblk(h_fs[l + 1] - h_fs[l]), 256, 0, stream>>>(h_fs[l], h_fs[l + 1], fs_rows.p, lt_ptr.p,
+                                                                          lt_pos.p, lt_row.p, lu.p, y.p);
+    k_permute_out<double><<<nblk(n), 256, 0, stream>>>(n, prow.p, y.p, dx, nan_flag);
+    HIPCHK(hipGetLastError());
     return XG_OK;
   }
   // solve with the factors f and device vectors db -> dx (arithmetic in T),
@@ -989,6 +1063,7 @@ int circuit_load(xg_circuit* c) {
     a.sta_cur = c->sta_cur.p; a.sto_out = c->sto_nxt.p; a.sta_out = c->sta_nxt.p; a.rec = c->rec.p;
     a.limited = c->limited.p;
     a.fl.dcop = c->ss.dcop; a.fl.tran = c->ss.tran; a.fl.init_jct = c->ss.init_jct; a.fl.newton_iter = c->ss.newton_iter;
+    a.fl.tranop = c->ss.tranop;
     k_mos1_eval<<<nblk(N), 256, 0, s>>>(a);
   }
   gather_vectors(c);
@@ -1129,6 +1204,7 @@ static int update_state_impl(xg_circuit* c, const double* x, const double* sta_c
     a.sta_cur = c->sta_cur.p; a.sto_out = c->sto_nxt.p; a.sta_out = c->sta_nxt.p; a.rec = c->rec.p;
     a.limited = c->limited.p;
     a.fl.dcop = c->ss.dcop; a.fl.tran = c->ss.tran; a.fl.init_jct = c->ss.init_jct; a.fl.newton_iter = c->ss.newton_iter;
+    a.fl.tranop = c->ss.tranop;
     k_mos1_eval<<<nblk(N), 256, 0, s>>>(a);
     HIPCHK(hipGetLastError());
     // in-place read of sto_nxt_in vs write of sto_out: each thread reads its
@@ -1387,13 +1463,15 @@ int xg_lu_numeric_dev(xg_lu* lu, const double* vals) { return lu_numeric_impl(lu
 
 static int lu_solve_impl(xg_lu* lu, const double* b, double* x, int transpose, bool dev) {
   if (!lu || !b || !x) return XG_E_ARG;
-  if (transpose) return XG_E_ARG;  // Xyce's transient path never requests a transpose solve
+  // transpose solves: the level-scheduled method with plain FP64 factors
+  if (transpose && (lu->method != 1 || lu->mode == 1 || !lu->have64 || lu->soft64)) return XG_E_ARG;
   hipStream_t s = lu->stream;
   const int n = lu->s.n;
   HIPCHK(hipMemcpyAsync(lu->b.p, b, sizeof(double) * n, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
   k_fill<<<1, 1, 0, s>>>(lu->flag.p + 1, 1, 0);
   int st;
-  if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
+  if (transpose) st = lu->solve_transpose(lu->b.p, lu->x.p, lu->flag.p + 1);
+  else if (lu->mode == 1 && !lu->have64) st = lu->solve_refined(lu->b.p, lu->x.p);
   else if (lu->soft64) st = lu->solve_refined64(lu->b.p, lu->x.p);
   else st = lu->solve_any(false, lu->b.p, lu->x.p);
   if (st != XG_OK) return st;

@@ -107,6 +107,7 @@ __device__ __forceinline__ int d_rec_vec_id(int plane) {
 
 struct MosFlags {
   int dcop, tran, init_jct, newton_iter;
+  int tranop;   // the operating point of a transient
 };
 
 // Evaluates one device.  volt[6] = V(d g s b d' s'); sto_cur/sto_nxt/sta_cur
@@ -298,7 +299,7 @@ __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tv
   emit(R_Q_DP, (-(Qeqbd + Qeqgd)) * M);
   emit(R_Q_SP, (-(Qeqbs + Qeqgs)) * M);
   double gcgd = 0, gcgs = 0, gcgb = 0, gcbs = 0, gcbd = 0;
-  if (fl.tran) { gcgd = Capgd; gcgs = Capgs; gcgb = Capgb; gcbs = capbs; gcbd = capbd; }
+  if (fl.tran || fl.tranop) { gcgd = Capgd; gcgs = Capgs; gcgb = Capgb; gcbs = capbs; gcbd = capbd; }
   if (!orig) {
     const double dvgx = mode > 0 ? (vgs - vgs_o) : (vgd - vgd_o);
     const double dvbx = mode > 0 ? (vbs - vbs_o) : (vbd - vbd_o);

@@ -299,6 +299,7 @@ void reduce(xg_tran* T, int n, const double* a, const double* b, double s, int s
 int set_state(xg_tran* T, int dcop, int newton_iter, int init_jct, double hh, double time) {
   xg_solver_state ss{};
   ss.dcop = dcop; ss.tran = dcop ? 0 : 1; ss.init_jct = init_jct; ss.init_fix = 0;
+  ss.tranop = dcop;   // the engine's operating points are a transient's
   ss.newton_iter = newton_iter; ss.order = T->order; ss.h = hh; ss.time = time;
   ss.tstep = T->tstep; ss.tstop = T->tstop;
   return xg_set_solver_state(T->c, &ss);

@@ -21,11 +21,12 @@ import numpy as np
 from . import MLP, PRED_FEAT, PRED_WIN, Batch
 
 
-def collect_pairs(circuits, tstop, tstep, hmax, capacity, chunk=512, max_launches=2_000_000):
+def collect_pairs(circuits, tstop, tstep, hmax, capacity, chunk=512, max_launches=2_000_000, base="poly"):
     """Runs the batch to the end with pair collection on; returns (x, t, m,
-    counters) with x bf16 [r][PRED_FEAT], t/m float32 [r][PRED_WIN] on the GPU."""
+    counters) with x bf16 [r][PRED_FEAT], t/m float32 [r][PRED_WIN] on the GPU
+    (targets relative to ``base``, 'poly' or 'cubic')."""
     b = Batch(circuits, tstop, tstep, hmax)
-    b.set_predictor(1, pair_capacity=capacity)
+    b.set_predictor(1, pair_capacity=capacity, base=base)
     c = b.run_to_end(chunk=chunk, max_launches=max_launches)
     x, t, m = b.pairs()
     c.update(b.pred_counters())
@@ -63,12 +64,14 @@ def train(net: MLP, x, t, m, epochs=4, batch=2048, lr=1e-3, seed=0, log=None):
     return dict(loss=hist, poly_loss=base, seconds=time.time() - t0)
 
 
-def run(circuits, tstop, tstep, hmax, net=None, record_max=0, chunk=512, max_launches=2_000_000):
+def run(circuits, tstop, tstep, hmax, net=None, record_max=0, chunk=512, max_launches=2_000_000, base="poly",
+        deadzone=0.0):
     """Runs a batch to the end, with the network serving predictions when
-    given; returns (counters, batch)."""
+    given (as corrections of ``base`` above ``deadzone`` Newton weights);
+    returns (counters, batch)."""
     b = Batch(circuits, tstop, tstep, hmax, record_max=record_max)
     if net is not None:
-        b.set_predictor(2, net)
+        b.set_predictor(2, net, base=base, deadzone=deadzone)
     c = b.run_to_end(chunk=chunk, max_launches=max_launches)
     if net is not None:
         c.update(b.pred_counters())
