@@ -4,7 +4,7 @@
   (AmesosSolver::doSolve, N_LAS_AmesosSolver.C:415-424) with either LU
   method and with FP32 factors;
 * bad arguments are rejected with XG_E_ARG before any device work
-  (transpose solves, which the transient path never asks for; empty batches;
+  (transpose solves on the multifrontal method; empty batches;
   an instance whose class does not fit a workgroup's LDS; launch
   configurations out of range);
 * an instance that fails (its step size collapses) stops on its own: the
@@ -68,8 +68,8 @@ def test_bad_arguments_are_rejected(gpu):
     bad = ci.copy()
     bad[5] = n + 3
     assert L.xg_lu_symbolic(n, rp.ctypes.data_as(P), bad.ctypes.data_as(P), None, ctypes.byref(h)) < 0
-    # transpose solves are not part of the transient path
-    s = X.DirectSolver(n, rp, ci)
+    # transpose solves: the level-scheduled method only
+    s = X.DirectSolver(n, rp, ci, method=2)
     assert s.numeric(v) == 0
     b = np.ones(n)
     x = np.zeros(n)
