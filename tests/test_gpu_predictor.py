@@ -120,11 +120,11 @@ def _evaluate(net, test_set, tstop, hmax, oscillator=False, check=(0, 3), base="
 def test_trained_predictor_on_power_grids(gpu):
     """The power-grid family: linear, so a proposal within the Newton
     weights saves the second solve of a step.  Trained on 2M step pairs
-    (5800 grids over 2 ns: about 357 accepted steps each)."""
+    (5800 grids over 2 ns: about 357 accepted steps each), 8 epochs."""
     x, t, m, c = P.collect_pairs(_grids(5800, 100), 2e-9, 1e-12, 0.0, capacity=420)
     assert x.shape[0] >= 2_000_000
     net = P.new_net(max_rows=2048, seed=5)
-    hist = P.train(net, x, t, m, epochs=3, batch=2048, lr=1e-3)
+    hist = P.train(net, x, t, m, epochs=8, batch=2048, lr=1e-3)
     print(f"{x.shape[0]} pairs; loss {hist['loss']} (polynomial predictor {hist['poly_loss']:.3e})")
     assert hist["loss"][-1] < hist["poly_loss"]
     poly, learned, ipb, ipl, _ = _evaluate(net, _grids(16, 5000), 2e-9, 0.0)
@@ -135,11 +135,11 @@ def test_trained_predictor_on_power_grids(gpu):
 def test_trained_predictor_on_rings(gpu):
     """Ring oscillators (nonlinear, about 2.07 Newton iterations per step with
     the polynomial predictor): the network corrects the cubic extrapolation,
-    trained on 0.93 M step pairs of 1024 rings; the iterations per step must
+    trained on 0.93 M step pairs of 1024 rings for 32 epochs; the iterations per step must
     fall below both the polynomial predictor's and the cubic's alone."""
     x, t, m, c = P.collect_pairs(_rings(1024, 100), 2e-9, 1e-12, 1e-11, capacity=1200, base="cubic")
     net = P.new_net(max_rows=2048, seed=5)
-    hist = P.train(net, x, t, m, epochs=16, batch=2048, lr=1e-3)
+    hist = P.train(net, x, t, m, epochs=32, batch=2048, lr=1e-3)
     print(f"{x.shape[0]} pairs; loss {hist['loss'][-1]:.4g} (zero correction {hist['poly_loss']:.4g})")
     assert hist["loss"][-1] < hist["poly_loss"]
     poly, learned, ipb, ipl, ipc = _evaluate(net, _rings(16, 900), 2e-9, 1e-11, oscillator=True, check=(0, 7),
