@@ -563,7 +563,7 @@ def _accurate_edge_shift(c, tstop, cols, t_o, x_o):
     return edge_shift(t_o, x_o[:, cols], ra["t"], ra["x"][:, cols])
 
 
-def test_config1_ring101_50ns_matches_oracle(gpu):
+def test_ring101_50ns_matches_oracle(gpu):
     """The reference's own CPU-runnable case: 101-stage ring oscillator,
     .TRAN 0-50 ns with a 10 ps maximum step (about 22k steps, 49k Newton
     iterations), on the GPU engine against the oracle over the whole window:
@@ -605,7 +605,7 @@ def test_config1_ring101_50ns_matches_oracle(gpu):
     col = int(o.pattern()["node_gid"][2])
     pg, ng = _period(t_g, x_g[:, col])
     po, no = _period(r["t"], r["x"][:, col])
-    print(f"config 1: steps gpu {st['steps']:.0f} oracle {r['steps']}, Newton iterations gpu {st['newton_iters']:.0f} "
+    print(f"101-stage ring, 0-50 ns: steps gpu {st['steps']:.0f} oracle {r['steps']}, Newton iterations gpu {st['newton_iters']:.0f} "
           f"oracle {r['newton_iters']}; 0-5 ns max |dV| {early:.2e} V; crossing shift gpu-oracle "
           f"{shift * 1e12:.3f} ps, oracle-accurate {acc_shift * 1e12:.2f} ps; 0-50 ns max |dV| within that shift "
           f"{tube:.2e} V; period gpu {pg * 1e9:.6f} ns oracle {po * 1e9:.6f} ns ({ng} periods)")

@@ -139,7 +139,7 @@ def test_power_grid_512_transient_full_size(gpu):
 
 
 def test_power_grid_512_on_the_device_resident_engine(gpu):
-    """The config-2 workload on the single-circuit engine (every vector in
+    """The 512x512 power-grid workload on the single-circuit engine (every vector in
     HBM, the multifrontal LU, no host round trips of vectors): the 512x512
     mesh with its 4096 PWL sinks, .TRAN with a 1 ps step, by default over
     0-50 ps (about 1000 steps); with XG_GRID_WINDOW=2e-9 in the environment,
