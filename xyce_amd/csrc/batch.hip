@@ -52,9 +52,14 @@ namespace xg {
 // ---- per structure class (offsets into the concatenated int arrays)
 struct ClassDesc {
   int n, nnz, nlu, n_mos, n_src, nops, nterm;
+  // term array of an instance: tsize = kterm * n_mos doubles, the term of
+  // device d with rank r among its nonzero planes at r * n_mos + d
+  int kterm, tsize;
   int nflev, nfs, nbs;
-  // int32 arrays (global): device node indices, term positions, term ranges, sources
-  int o_lid, o_tpos, o_vptr, o_mptr, o_bgp, o_bge;
+  // int32 arrays (global): device node indices, per-device plane masks
+  // (low words, then high words), the term array index of every slot-ordered
+  // term (o_tix, nterm entries), term ranges, sources
+  int o_lid, o_tmask, o_tix, o_vptr, o_mptr, o_bgp, o_bge;
   // groups of >= kLongSeg terms, summed by a whole wavefront: vector groups
   // (4*row + vector) then matrix groups (2*slot + F/Q), at o_long
   int n_vlong, n_mlong, o_long;
@@ -220,6 +225,41 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 
 // J = a*dQdx + b*dFdx (EpetraMatrix::linearCombo rounding)
 __device__ __forceinline__ double jac_combine(double a, double dq, double b, double df) { return a * dq + b * df; }
+
+// A device's nonzero record planes are a 48-bit mask (lo: planes 0-31, hi:
+// planes 32-47); plane p's term sits at its rank among them.  With p a
+// constant at every call site both reduce to a mask test and a bit count.
+__device__ __forceinline__ bool term_on(unsigned lo, unsigned hi, int p) {
+  return p < 32 ? (lo >> p) & 1u : (hi >> (p - 32)) & 1u;
+}
+__device__ __forceinline__ int term_rank(unsigned lo, unsigned hi, int p) {
+  return p < 32 ? __popc(lo & ((1u << p) - 1u)) : __popc(lo) + __popc(hi & ((1u << (p - 32)) - 1u));
+}
+
+// Sums the term groups [b[0], b[1]), ..., [b[G-1], b[G]) (consecutive in slot
+// order; tix maps a slot to its place in the term array) into out[0..G-1],
+// each in slot order from 0.0.  The span's indices are loaded 8 at a time
+// before any of their values, so a span costs one round trip per 8 terms.
+template <int G>
+__device__ __forceinline__ void gather_groups(const double* T, const int* tix, const int (&b)[G + 1], double (&out)[G]) {
+#pragma unroll
+  for (int v = 0; v < G; ++v) out[v] = 0.0;
+  for (int c = b[0]; c < b[G]; c += 8) {
+    int ix[8];
+    double tv[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ix[q] = c + q < b[G] ? tix[c + q] : 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) tv[q] = c + q < b[G] ? T[ix[q]] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = c + q;
+#pragma unroll
+      for (int v = 0; v < G; ++v)
+        if (e >= b[v] && e < b[v + 1]) out[v] += tv[q];
+    }
+  }
+}
 // residual combine (OneStep::obtainResidual / NoTimeIntegration::obtainResidual rounding)
 __device__ __forceinline__ double residual_combine(bool dc, int order, double h, double f, double q, double b,
                                                    double fl, double ql, double q0, double q2) {
@@ -500,13 +540,14 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
   const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
   const unsigned short* mptr = prog + C.b_mptr;
   const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+  const int* tix = ci + C.o_tix;
   for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
   __syncthreads();
   for (int r = tid; r < n; r += nth)
     for (int s = rp[r]; s < rp[r + 1]; ++s) {
       double df = 0.0, dq = 0.0;
-      for (int e = mptr[2 * s]; e < mptr[2 * s + 1]; ++e) df += T[e];
-      for (int e = mptr[2 * s + 1]; e < mptr[2 * s + 2]; ++e) dq += T[e];
+      for (int e = mptr[2 * s]; e < mptr[2 * s + 1]; ++e) df += T[tix[e]];
+      for (int e = mptr[2 * s + 1]; e < mptr[2 * s + 2]; ++e) dq += T[tix[e]];
       if (linF[s] != 0.0) df += linF[s];
       if (linQ[s] != 0.0) dq += linQ[s];
       D[(size_t)r * n + cc[s]] = jac_combine(ja, dq, jb, df);
@@ -672,15 +713,17 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   }
   __syncthreads();
   XG_PHASE(0);
-  // ---- 2. device evaluation (one thread per MOSFET); each term goes to its
-  //         slot-ordered position in the instance's term array
+  // ---- 2. device evaluation (one thread per MOSFET); each nonzero term goes
+  //         to the device's column of the instance's term array, at its rank
+  //         among the device's nonzero planes (an index computed in registers:
+  //         no load stands between two term stores)
   if (FUSED) {
     MosFlags fl;
     fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && S.newton_iter == 0 && S.cont_first != 0.0;
     fl.tranop = dc;   // every batch runs a transient: its DC point is a transient's operating point
     fl.newton_iter = S.newton_iter;
     const int* lid = ci + C.o_lid;
-    const int* tpos = ci + C.o_tpos;
+    const int* tmask = ci + C.o_tmask;
     int lim_any = 0;
     for (int d = tid; d < C.n_mos; d += nth) {
       const int g = I.dev_off + d;
@@ -690,7 +733,12 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
       for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
       // class arrays are plane-major ([plane][device]): a wavefront's loads are coalesced
-      auto emit = [&](int plane, double v) { const int p = tpos[(size_t)plane * C.n_mos + d]; if (p >= 0) T[p] = v; };
+      const unsigned mlo = (unsigned)tmask[d], mhi = (unsigned)tmask[C.n_mos + d];
+      double* Td = T + d;
+      const int nm = C.n_mos;
+      auto emit = [&](int plane, double v) {
+        if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
+      };
       lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
       for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
       for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
@@ -709,13 +757,14 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     const int* lg = ci + C.o_long;
     const unsigned short* vptr = prog + C.b_vptr;
     const unsigned short* mptr = prog + C.b_mptr;
+    const int* tix = ci + C.o_tix;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
     for (int j = wave; j < nl; j += nwaves) {
       const int g = lg[j];
       const int beg = j < C.n_vlong ? vptr[g] : mptr[g];
       const int end = j < C.n_vlong ? vptr[g + 1] : mptr[g + 1];
       double sacc = 0.0;
-      for (int e = beg + lane; e < end; e += 64) sacc += T[e];
+      for (int e = beg + lane; e < end; e += 64) sacc += T[tix[e]];
       for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
       if (lane == 0) lsum[j] = sacc;
     }
@@ -731,21 +780,28 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     const unsigned short* bgp = prog + C.b_bgp; const unsigned short* bge = prog + C.b_bge;
     const unsigned short* lrp = prog + C.b_lrp; const unsigned short* lk = prog + C.b_lk;
     const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
+    const int* tix = ci + C.o_tix;
     for (int base = tid & ~63; base < n; base += nth) {
       const int r = base + (tid & 63);
       double res2 = 0.0;
       if (r < n) {
       double acc[4];
-      for (int v = 0; v < 4; ++v) {
-        const int beg = vptr[4 * r + v], end = vptr[4 * r + v + 1];
-        double s = 0.0;
-        if (end - beg >= kLongSeg) {
-          const int* lg = ci + C.o_long;
-          for (int j = 0; j < C.n_vlong; ++j) if (lg[j] == 4 * r + v) s = lsum[j];
-        } else {
-          for (int e = beg; e < end; ++e) s += T[e];
+      const int gb[5] = {vptr[4 * r], vptr[4 * r + 1], vptr[4 * r + 2], vptr[4 * r + 3], vptr[4 * r + 4]};
+      bool lng = false;
+      for (int v = 0; v < 4; ++v) lng |= gb[v + 1] - gb[v] >= kLongSeg;
+      if (!lng) {
+        gather_groups<4>(T, tix, gb, acc);
+      } else {
+        for (int v = 0; v < 4; ++v) {
+          double s = 0.0;
+          if (gb[v + 1] - gb[v] >= kLongSeg) {
+            const int* lg = ci + C.o_long;
+            for (int j = 0; j < C.n_vlong; ++j) if (lg[j] == 4 * r + v) s = lsum[j];
+          } else {
+            for (int e = gb[v]; e < gb[v + 1]; ++e) s += T[tix[e]];
+          }
+          acc[v] = s;
         }
-        acc[v] = s;
       }
       double f = acc[0], q = acc[1];
       const double fl = acc[2], ql = acc[3];
@@ -839,17 +895,23 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
         const unsigned short* a2lu = prog + C.b_a2lu;
         const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
         const int* lg = ci + C.o_long;
+        const int* tix = ci + C.o_tix;
         for (int s = tid; s < C.nnz; s += nth) {
           double part[2];
-          for (int fq = 0; fq < 2; ++fq) {
-            const int g = 2 * s + fq, beg = mptr[g], end = mptr[g + 1];
-            double acc = 0.0;
-            if (end - beg >= kLongSeg) {
-              for (int j = 0; j < C.n_mlong; ++j) if (lg[C.n_vlong + j] == g) acc = lsum[C.n_vlong + j];
-            } else {
-              for (int e = beg; e < end; ++e) acc += T[e];
+          const int b[3] = {mptr[2 * s], mptr[2 * s + 1], mptr[2 * s + 2]};
+          if (b[1] - b[0] < kLongSeg && b[2] - b[1] < kLongSeg) {
+            gather_groups<2>(T, tix, b, part);
+          } else {
+            for (int fq = 0; fq < 2; ++fq) {
+              const int g = 2 * s + fq;
+              double acc = 0.0;
+              if (b[fq + 1] - b[fq] >= kLongSeg) {
+                for (int j = 0; j < C.n_mlong; ++j) if (lg[C.n_vlong + j] == g) acc = lsum[C.n_vlong + j];
+              } else {
+                for (int e = b[fq]; e < b[fq + 1]; ++e) acc += T[tix[e]];
+              }
+              part[fq] = acc;
             }
-            part[fq] = acc;
           }
           double df = part[0], dq = part[1];
           if (linF[s] != 0.0) df += linF[s];
@@ -1232,7 +1294,11 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
 #undef XG_PHASE
 }
 
-template <bool FUSED>
+// KLDS: the MOSFET class table is staged in LDS (at most 16 entries); TLDS:
+// the term array lives in LDS.  Both are template constants so that every
+// access compiles to its own memory's instructions (LDS reads wait only on
+// LDS, not on the term stores in flight).
+template <bool FUSED, bool KLDS, bool TLDS>
 __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   if ((int)blockIdx.x >= A.n_inst) return;
   const int inst = A.order[blockIdx.x];
@@ -1243,7 +1309,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   __shared__ int red_i[16];
   __shared__ int sh_int[8];
   __shared__ double sh_d[8];
-  __shared__ double kc_s[FUSED ? 16 * KSTRIDE : 1];
+  __shared__ double kc_s[KLDS ? 16 * KSTRIDE : 1];
   __shared__ double srcv[kMaxSrc];
   extern __shared__ double dyn[];
   double* xs = dyn;                         // n + 1
@@ -1264,15 +1330,14 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
     for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
   }
   const double* kcls = A.kcls;
-  if (FUSED && A.n_kcls <= 16) {
+  if (KLDS) {
     for (int i = tid; i < A.n_kcls * KSTRIDE; i += nth) kc_s[i] = A.kcls[i];
     kcls = kc_s;
   }
   __syncthreads();
-  // fused form with lds_terms: the term array lives in LDS after the program
-  // (8-byte aligned); it is written and read within each Newton iteration
-  double* T = (FUSED && A.lds_terms) ? reinterpret_cast<double*>(prog + ((A.lds_blob + 3) & ~3))
-                                      : A.terms + I.term_off;
+  // fused form with the term array in LDS: after the program (8-byte
+  // aligned); it is written and read within each Newton iteration
+  double* T = TLDS ? reinterpret_cast<double*>(prog + ((A.lds_blob + 3) & ~3)) : A.terms + I.term_off;
   const int iters = FUSED ? A.iters : 1;
   for (int it = 0; it < iters; ++it) {
     batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv, T);
@@ -1305,7 +1370,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   fl.dcop = dc; fl.tran = !dc; fl.init_jct = dc && newton_iter == 0 && first_solve; fl.newton_iter = newton_iter;
   fl.tranop = dc;
   const int* lid = A.ci + C.o_lid;
-  const int* tpos = A.ci + C.o_tpos;
+  const int* tmask = A.ci + C.o_tmask;
   double* T = A.terms + I.term_off;
   const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
   double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
@@ -1315,7 +1380,12 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   }
   for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
   for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
-  auto emit = [&](int plane, double v) { const int p = tpos[(size_t)plane * C.n_mos + d]; if (p >= 0) T[p] = v; };
+  const unsigned mlo = (unsigned)tmask[d], mhi = (unsigned)tmask[C.n_mos + d];
+  double* Td = T + d;
+  const int nm = C.n_mos;
+  auto emit = [&](int plane, double v) {
+    if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
+  };
   const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
   for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
   for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
@@ -1516,7 +1586,7 @@ struct xg_batch {
         for (int j = s0; j < s1; ++j) {
           const ClassDesc& C = classes[inst_cls[ord[j]]];
           bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu); blob = std::max(blob, C.blob_len);
-          nterm = std::max(nterm, C.nterm);
+          nterm = std::max(nterm, C.tsize);
         }
         bk.shmem = lds_bytes(bk.n, bk.nlu, blob);
         bk.blob = blob; bk.nterm = nterm;
@@ -1525,7 +1595,7 @@ struct xg_batch {
       }
     } else {
       Bucket bk{0, n_inst, args.max_n, args.max_nlu, shmem, 0};
-      for (const ClassDesc& C : classes) { bk.blob = std::max(bk.blob, C.blob_len); bk.nterm = std::max(bk.nterm, C.nterm); }
+      for (const ClassDesc& C : classes) { bk.blob = std::max(bk.blob, C.blob_len); bk.nterm = std::max(bk.nterm, C.tsize); }
       bk.shmem_terms = lds_bytes(bk.n, bk.nlu, ((bk.blob + 3) & ~3)) + sizeof(double) * (size_t)bk.nterm;
       buckets.push_back(bk);
     }
@@ -1558,8 +1628,10 @@ struct xg_batch {
         s = gstream[j];
         if (hipStreamWaitEvent(s, fork_ev, 0) != hipSuccess) return XG_E_HIP;
       }
-      if (split) hipLaunchKernelGGL(k_batch_step<false>, dim3(bk.count), dim3(bs), bk.shmem, s, a);
-      else hipLaunchKernelGGL(k_batch_step<true>, dim3(bk.count), dim3(bs), shm, s, a);
+      if (split) hipLaunchKernelGGL((k_batch_step<false, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      else if (a.n_kcls > 16) hipLaunchKernelGGL((k_batch_step<true, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      else if (lt) hipLaunchKernelGGL((k_batch_step<true, true, true>), dim3(bk.count), dim3(bs), shm, s, a);
+      else hipLaunchKernelGGL((k_batch_step<true, true, false>), dim3(bk.count), dim3(bs), shm, s, a);
       if (side) {
         if (hipEventRecord(join_ev[j], s) != hipSuccess) return XG_E_HIP;
         if (hipStreamWaitEvent(stream, join_ev[j], 0) != hipSuccess) return XG_E_HIP;
@@ -1674,6 +1746,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       for (int g = 0; g < 2 * hc.nnz; ++g) mptr[g + 1] = mptr[g] + mcnt[g];
       C.nterm = mptr[2 * hc.nnz];
       std::vector<int> tpos((size_t)hc.n_mos * R_COUNT, -1);
+      bool tix_ok = false;
       {
         std::vector<int> vf(vptr.begin(), vptr.end() - 1), mf(mptr.begin(), mptr.end() - 1);
         for (int d = 0; d < hc.n_mos; ++d)
@@ -1683,11 +1756,31 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           }
       }
       {
-        // plane-major copies for the device evaluation's coalesced loads
-        std::vector<int> tposT((size_t)hc.n_mos * R_COUNT);
-        for (int d = 0; d < hc.n_mos; ++d)
-          for (int p = 0; p < R_COUNT; ++p) tposT[(size_t)p * hc.n_mos + d] = tpos[(size_t)d * R_COUNT + p];
-        C.o_tpos = push(tposT);
+        // term array: device d's nonzero planes at rank * n_mos + d; the
+        // slot-ordered lists reach them through tix
+        std::vector<int> tm((size_t)2 * hc.n_mos);
+        int kt = 0;
+        for (int d = 0; d < hc.n_mos; ++d) {
+          tm[d] = (int)(unsigned)(emask[d] & 0xffffffffull);
+          tm[(size_t)hc.n_mos + d] = (int)(unsigned)(emask[d] >> 32);
+          kt = std::max(kt, __builtin_popcountll(emask[d]));
+        }
+        C.kterm = kt;
+        C.tsize = std::max(1, kt * hc.n_mos);
+        std::vector<int> tix(C.nterm, -1);
+        for (int d = 0; d < hc.n_mos; ++d) {
+          int rank = 0;
+          for (int p = 0; p < R_COUNT; ++p) {
+            if (!(emask[d] >> p & 1ull)) continue;
+            const int e = tpos[(size_t)d * R_COUNT + p];
+            if (e >= 0 && e < C.nterm) tix[e] = rank * hc.n_mos + d;
+            ++rank;
+          }
+        }
+        C.o_tmask = push(tm);
+        C.o_tix = push(tix);
+        tix_ok = (long long)kt * hc.n_mos < (1ll << 30) && kt <= R_COUNT;
+        for (int x : tix) tix_ok = tix_ok && x >= 0 && x < C.tsize;
       }
       C.o_vptr = push(vptr); C.o_mptr = push(mptr);
       {
@@ -1817,7 +1910,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
                   in(sym.lu_colind, 0, C.n) && in(sym.lu_diag, 0, C.nlu) && in(sym.f_tgt, 0, C.nlu) &&
                   in(sym.f_ops, 0, C.nlu) && in(sym.f_div, -1, C.nlu) && in(sym.fs_rows, 0, C.n) &&
                   in(sym.bs_rows, 0, C.n) && in(sym.perm_row, 0, C.n) && in(sym.perm_col, 0, C.n) &&
-                  in(tpos, -1, C.nterm);
+                  in(tpos, -1, C.nterm) && tix_ok;
         // every value of the uint16 program must fit (0xFFFF is the "no division" sentinel)
         ok = ok && C.nlu < 0xFFFF && (int)sym.f_ops.size() / 2 < 0xFFFF && C.nnz < 0xFFFF && C.n < 0xFFFF;
         // term ranges are uint16 in the program; source values are staged in LDS
@@ -1842,7 +1935,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
     D.dense_off = dense_total;
     dense_total += (long long)hc.n * hc.n;
     D.term_off = term_total;
-    term_total += b->classes[cid].nterm;
+    term_total += b->classes[cid].tsize;
     b->insts.push_back(D);
     b->cls_n.push_back(hc.n_mos + hc.n);
     // devices
@@ -2184,7 +2277,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
         for (int bs : gblocks) {
           for (int lt = 0; lt < 2; ++lt) {
             // the term arrays in LDS where they fit a workgroup's LDS
-            if (lt && b->buckets[j].shmem_terms > kMaxLdsBytes) continue;
+            if (lt && (b->buckets[j].shmem_terms > kMaxLdsBytes || b->args.n_kcls > 16)) continue;
             b->buckets[j].block = bs;
             b->buckets[j].terms = lt != 0;
             b->only_bucket = (int)j;
@@ -2290,7 +2383,7 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
   if (split_eval >= 0) {
     b->split = split_eval == 1 ? 1 : 0;
     // 2: fused, each group's term arrays in LDS where they fit; 0: in HBM
-    for (auto& bk : b->buckets) bk.terms = split_eval == 2 && bk.shmem_terms <= kMaxLdsBytes;
+    for (auto& bk : b->buckets) bk.terms = split_eval == 2 && bk.shmem_terms <= kMaxLdsBytes && b->args.n_kcls <= 16;
   }
   if (iters_per_launch) b->args.iters = iters_per_launch;
   if (b->split) b->args.iters = 1;

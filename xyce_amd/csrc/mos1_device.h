@@ -77,28 +77,33 @@ __device__ __forceinline__ double d_fetlim(double vnew, double vold, double vto)
 
 __device__ __forceinline__ void d_qmeyer(double vgs, double vgd, double von, double vdsat, double& capgs,
                                          double& capgd, double& capgb, double phi, double cox) {
+  // branch results in locals, written through the references once (stores
+  // through the references in every branch get merged into stores through a
+  // selected pointer, which keeps the caller's outputs in scratch memory)
+  double gs, gd, gb;
   const double vgst = vgs - von;
   if (vgst <= -phi) {
-    capgb = cox / 2; capgs = 0; capgd = 0;
+    gb = cox / 2; gs = 0; gd = 0;
   } else if (vgst <= -phi / 2) {
-    capgb = -vgst * cox / (2 * phi); capgs = 0; capgd = 0;
+    gb = -vgst * cox / (2 * phi); gs = 0; gd = 0;
   } else if (vgst <= 0) {
-    capgb = -vgst * cox / (2 * phi);
-    capgs = vgst * cox / (1.5 * phi) + cox / 3;
-    capgd = 0;
+    gb = -vgst * cox / (2 * phi);
+    gs = vgst * cox / (1.5 * phi) + cox / 3;
+    gd = 0;
   } else {
     const double vds = vgs - vgd;
     if (vdsat <= vds) {
-      capgs = cox / 3; capgd = 0; capgb = 0;
+      gs = cox / 3; gd = 0; gb = 0;
     } else {
       const double vddif = 2.0 * vdsat - vds;
       const double vddif1 = vdsat - vds;
       const double vddif2 = vddif * vddif;
-      capgd = cox * (1.0 - vdsat * vdsat / vddif2) / 3;
-      capgs = cox * (1.0 - vddif1 * vddif1 / vddif2) / 3;
-      capgb = 0;
+      gd = cox * (1.0 - vdsat * vdsat / vddif2) / 3;
+      gs = cox * (1.0 - vddif1 * vddif1 / vddif2) / 3;
+      gb = 0;
     }
   }
+  capgs = gs; capgd = gd; capgb = gb;
 }
 
 __device__ __forceinline__ int d_rec_vec_id(int plane) {
@@ -248,8 +253,15 @@ __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tv
   // Meyer capacitances with back-averaging
   double capgs, capgd, capgb;
   const double cox = k[K_OXCAP];
-  if (mode > 0) d_qmeyer(vgs, vgd, Von, Vdsat, capgs, capgd, capgb, tphi, cox);
-  else d_qmeyer(vgd, vgs, Von, Vdsat, capgd, capgs, capgb, tphi, cox);
+  {
+    // one call with the terminals swapped in reverse mode (two calls writing
+    // through swapped references would make the compiler keep the three
+    // outputs in scratch memory)
+    double c_s, c_d;
+    d_qmeyer(mode > 0 ? vgs : vgd, mode > 0 ? vgd : vgs, Von, Vdsat, c_s, c_d, capgb, tphi, cox);
+    capgs = mode > 0 ? c_s : c_d;
+    capgd = mode > 0 ? c_d : c_s;
+  }
   double Capgs, Capgd, Capgb;
   if (fl.dcop) {
     Capgs = 2.0 * capgs + k[K_GSO]; Capgd = 2.0 * capgd + k[K_GDO]; Capgb = 2.0 * capgb + k[K_GBO];
