@@ -55,7 +55,7 @@ struct ClassDesc {
   // term array of an instance: tsize = kterm * n_mos doubles, the term of
   // device d with rank r among its nonzero planes at r * n_mos + d
   int kterm, tsize;
-  int nflev, nfs, nbs;
+  int nflev, nbs;
   // int32 arrays (global): device node indices, per-device plane masks
   // (low words, then high words), the term array index of every slot-ordered
   // term (o_tix, nterm entries), term ranges, sources
@@ -65,11 +65,11 @@ struct ClassDesc {
   int n_vlong, n_mlong, o_long;
   // uint16 program blob (staged into LDS each launch); offsets in uint16 units
   int blob_off, blob_len;
-  int b_flp, b_ftg, b_fop, b_fdiv, b_fops, b_lurp, b_luci, b_diag, b_fsp, b_fsr, b_bsp, b_bsr, b_prow, b_pcol,
+  int b_flp, b_fmeta, b_fops, b_lurp, b_luci, b_diag, b_bsp, b_bsr, b_prow, b_pcol,
       b_rp, b_cc, b_a2lu;
   // long LU targets / solve rows (>= kLongOps terms, one wavefront each) come
   // last in their level: first long index per level
-  int b_fls, b_fss, b_bss;
+  int b_fls, b_bss;
   int b_vrow;        // n flags: 1 = node-voltage row (gmin stepping adds its conductance there)
   // term ranges, linear entries (CSR slots with a nonzero linear value, per
   // row) and per-row source lists, all uint16
@@ -670,7 +670,7 @@ __device__ __noinline__ int dc_strategy_step(InstState& S, int status) {
 template <bool FUSED>
 __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I, const ClassDesc& C, InstState& S,
                                 double* red, int* red_i, int* sh_int, double* sh_d, double* xs, double* rhs,
-                                double* y, double* lu, const unsigned short* prog, const double* kcls,
+                                double* lu, const unsigned short* prog, const double* kcls,
                                 double* srcv, double* T) {
   const int tid = threadIdx.x, nth = blockDim.x;
   const int n = C.n, off = I.vec_off;
@@ -728,7 +728,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     for (int d = tid; d < C.n_mos; d += nth) {
       const int g = I.dev_off + d;
       const double* k = kcls + (size_t)A.mcls[g] * KSTRIDE;
-      double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+      double volt[6], sc[6], sn[6], ac[8];
       for (int j = 0; j < 6; ++j) volt[j] = xs[lid[j * C.n_mos + d]];
       for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
       for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
@@ -739,9 +739,11 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       auto emit = [&](int plane, double v) {
         if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
       };
-      lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
-      for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
-      for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
+      auto keep = [&](int j, double v) {
+        if (j < 6) A.sto_nxt[(size_t)j * NT + g] = v;
+        else A.sta_nxt[(size_t)(j - 6) * NT + g] = v;
+      };
+      lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, keep, emit);
     }
     if (lim_any) atomicOr(&sh_int[0], 1);
   }
@@ -764,7 +766,18 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const int beg = j < C.n_vlong ? vptr[g] : mptr[g];
       const int end = j < C.n_vlong ? vptr[g + 1] : mptr[g + 1];
       double sacc = 0.0;
-      for (int e = beg + lane; e < end; e += 64) sacc += T[tix[e]];
+      // four strides at a time: indices, then values, then the sums in order
+      for (int e0 = beg + lane; e0 < end; e0 += 256) {
+        int ix[4];
+        double tv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ix[q] = e0 + 64 * q < end ? tix[e0 + 64 * q] : 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tv[q] = e0 + 64 * q < end ? T[ix[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (e0 + 64 * q < end) sacc += tv[q];
+      }
       for (int o = 32; o > 0; o >>= 1) sacc += __shfl_xor(sacc, o, 64);
       if (lane == 0) lsum[j] = sacc;
     }
@@ -889,6 +902,12 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     bool use_dense = dc;
     if (!dc) {
       for (int p = tid; p < nlu; p += nth) lu[p] = 0.0;
+      // the forward solve runs inside the refactor: its n entries after the
+      // factors start from the permuted right-hand side
+      {
+        const unsigned short* prow = prog + C.b_prow;
+        for (int i = tid; i < n; i += nth) lu[nlu + i] = rhs[prow[i]];
+      }
       __syncthreads();
       {
         const unsigned short* mptr = prog + C.b_mptr;
@@ -924,33 +943,46 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       {
         // level-scheduled refactor; KLU's threshold test (|l_ij| <= 1/0.001,
         // finite nonzero pivots) is evaluated as the entries are produced
-        const unsigned short* flp = prog + C.b_flp; const unsigned short* ftg = prog + C.b_ftg;
-        const unsigned short* fop = prog + C.b_fop; const unsigned short* fops = prog + C.b_fops;
-        const unsigned short* fdiv = prog + C.b_fdiv; const unsigned short* fls = prog + C.b_fls;
+        const unsigned short* flp = prog + C.b_flp; const unsigned short* fls = prog + C.b_fls;
+        const uint2* fmeta = reinterpret_cast<const uint2*>(prog + C.b_fmeta);
+        const unsigned* fops = reinterpret_cast<const unsigned*>(prog + C.b_fops);
         const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
         int bad = 0;
         for (int l = 0; l < C.nflev; ++l) {
           const int t0 = flp[l], ts = fls[l], t1 = flp[l + 1];
           for (int t = t0 + tid; t < ts; t += nth) {
-            const int p = ftg[t];
+            const uint2 m = fmeta[t];
+            const int p = m.x & 0xFFFF, dv = m.x >> 16, o0 = m.y & 0xFFFF, no = m.y >> 16;
             double v = lu[p];
-            for (int o = fop[t]; o < fop[t + 1]; ++o) v -= lu[fops[2 * o]] * lu[fops[2 * o + 1]];
-            const int dv = fdiv[t];
+            const double d = dv != 0xFFFF ? lu[dv] : 1.0;
+            for (int o = 0; o < no; o += 4) {
+              unsigned pr[4];
+              double fa[4], fb[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) pr[q] = o + q < no ? fops[o0 + o + q] : 0u;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) { fa[q] = lu[pr[q] & 0xFFFF]; fb[q] = lu[pr[q] >> 16]; }
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (o + q < no) v -= fa[q] * fb[q];
+            }
             if (dv != 0xFFFF) {
-              const double d = lu[dv];
               v /= d;
               if (!(fabs(v) <= 1.0e3) || !(fabs(d) > 0.0) || !isfinite(d)) bad = 1;
             }
             lu[p] = v;
           }
           for (int j = ts + wave; j < t1; j += nwaves) {
+            const uint2 m = fmeta[j];
+            const int p = m.x & 0xFFFF, dv = m.x >> 16, o0 = m.y & 0xFFFF, no = m.y >> 16;
             double acc = 0.0;
-            for (int o = fop[j] + lane; o < fop[j + 1]; o += 64) acc += lu[fops[2 * o]] * lu[fops[2 * o + 1]];
+            for (int o = lane; o < no; o += 64) {
+              const unsigned pr = fops[o0 + o];
+              acc += lu[pr & 0xFFFF] * lu[pr >> 16];
+            }
             for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
             if (lane == 0) {
-              const int p = ftg[j];
               double v = lu[p] - acc;
-              const int dv = fdiv[j];
               if (dv != 0xFFFF) {
                 const double d = lu[dv];
                 v /= d;
@@ -974,31 +1006,14 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       }
       XG_PHASE(7);
       if (!use_dense) {
-        const unsigned short* prow = prog + C.b_prow; const unsigned short* pcol = prog + C.b_pcol;
+        // L y = P b ran with the refactor (y after the factors); U x = y here
+        const unsigned short* pcol = prog + C.b_pcol;
         const unsigned short* lurp = prog + C.b_lurp; const unsigned short* luci = prog + C.b_luci;
         const unsigned short* dg = prog + C.b_diag;
-        const unsigned short* fsp = prog + C.b_fsp; const unsigned short* fsr = prog + C.b_fsr;
         const unsigned short* bsp = prog + C.b_bsp; const unsigned short* bsr = prog + C.b_bsr;
-        for (int i = tid; i < n; i += nth) y[i] = rhs[prow[i]];
-        __syncthreads();
-        const unsigned short* fss = prog + C.b_fss; const unsigned short* bss = prog + C.b_bss;
+        const unsigned short* bss = prog + C.b_bss;
         const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
-        for (int l = 0; l < C.nfs; ++l) {
-          for (int t = fsp[l] + tid; t < fss[l]; t += nth) {
-            const int i = fsr[t];
-            double v = y[i];
-            for (int p = lurp[i]; p < dg[i]; ++p) v -= lu[p] * y[luci[p]];
-            y[i] = v;
-          }
-          for (int j = fss[l] + wave; j < fsp[l + 1]; j += nwaves) {
-            const int i = fsr[j];
-            double acc = 0.0;
-            for (int p = lurp[i] + lane; p < dg[i]; p += 64) acc += lu[p] * y[luci[p]];
-            for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
-            if (lane == 0) y[i] -= acc;
-          }
-          __syncthreads();
-        }
+        double* y = lu + nlu;
         for (int l = 0; l < C.nbs; ++l) {
           for (int t = bsp[l] + tid; t < bss[l]; t += nth) {
             const int i = bsr[t];
@@ -1314,10 +1329,9 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   extern __shared__ double dyn[];
   double* xs = dyn;                         // n + 1
   double* rhs = xs + (A.lds_n + 1);         // n
-  double* y = rhs + A.lds_n;                // n
-  double* lu = y + A.lds_n;                 // nlu
+  double* lu = rhs + (A.lds_n + 1);         // nlu factor entries, then the n forward-solve entries
   // class program after the doubles, on a 16-byte boundary (uint4 copies)
-  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((3 * (A.lds_n + 1) + A.lds_nlu + 1) & ~1));
+  unsigned short* prog = reinterpret_cast<unsigned short*>(dyn + ((2 * (A.lds_n + 1) + A.lds_nlu + 1) & ~1));
   const int tid = threadIdx.x, nth = blockDim.x;
   if (tid == 0) S = A.st[inst];
   __syncthreads();
@@ -1340,7 +1354,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   double* T = TLDS ? reinterpret_cast<double*>(prog + ((A.lds_blob + 3) & ~3)) : A.terms + I.term_off;
   const int iters = FUSED ? A.iters : 1;
   for (int it = 0; it < iters; ++it) {
-    batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, y, lu, prog, kcls, srcv, T);
+    batch_iteration<FUSED>(A, inst, I, C, S, red, red_i, sh_int, sh_d, xs, rhs, lu, prog, kcls, srcv, T);
     __syncthreads();
     if (S.mode >= 2 || S.mode < 0) break;
   }
@@ -1373,7 +1387,7 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const int* tmask = A.ci + C.o_tmask;
   double* T = A.terms + I.term_off;
   const double* k = A.kcls + (size_t)A.mcls[g] * KSTRIDE;
-  double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+  double volt[6], sc[6], sn[6], ac[8];
   for (int j = 0; j < 6; ++j) {
     const int l = lid[j * C.n_mos + d];
     volt[j] = l < n ? A.x[off + l] : 0.0;
@@ -1386,9 +1400,11 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   auto emit = [&](int plane, double v) {
     if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
   };
-  const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, so, ao, emit);
-  for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = so[j];
-  for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = ao[j];
+  auto keep = [&](int j, double v) {
+    if (j < 6) A.sto_nxt[(size_t)j * NT + g] = v;
+    else A.sta_nxt[(size_t)(j - 6) * NT + g] = v;
+  };
+  const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, keep, emit);
   if (lim) atomicOr(&A.lim[inst], 1);
 }
 
@@ -1564,7 +1580,7 @@ struct xg_batch {
     if (fork_ev) (void)hipEventDestroy(fork_ev);
   }
   static size_t lds_bytes(int n, int nlu, int blob) {
-    return sizeof(double) * (((size_t)3 * (n + 1) + nlu + 1) & ~(size_t)1) + sizeof(unsigned short) * (size_t)blob;
+    return sizeof(double) * (((size_t)2 * (n + 1) + nlu + 1) & ~(size_t)1) + sizeof(unsigned short) * (size_t)blob;
   }
   // mode 0: given order, one launch sized for the largest class; mode 1:
   // instances by LDS need (largest first), up to 4 launches of equal size,
@@ -1585,7 +1601,7 @@ struct xg_batch {
         int blob = 0, nterm = 0;
         for (int j = s0; j < s1; ++j) {
           const ClassDesc& C = classes[inst_cls[ord[j]]];
-          bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu); blob = std::max(blob, C.blob_len);
+          bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu + C.n); blob = std::max(blob, C.blob_len);
           nterm = std::max(nterm, C.tsize);
         }
         bk.shmem = lds_bytes(bk.n, bk.nlu, blob);
@@ -1723,7 +1739,6 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       }
       C.nlu = sym.nlu;
       C.nflev = (int)sym.f_lvl_ptr.size() - 1;
-      C.nfs = (int)sym.fs_lvl_ptr.size() - 1;
       C.nbs = (int)sym.bs_lvl_ptr.size() - 1;
       auto push = [&](const std::vector<int>& v) { int o = (int)ci.size(); ci.insert(ci.end(), v.begin(), v.end()); return o; };
       const int n = hc.n;
@@ -1837,25 +1852,58 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           for (int x : v) blob.push_back((unsigned short)x);
           return o;
         };
-        // LU targets: within each level the ones with >= kLongOps operations
-        // go last (a wavefront each); fls[l] = first of them
-        std::vector<int> ftg, fop{0}, fops, fdv, fls;
-        for (size_t l = 0; l + 1 < sym.f_lvl_ptr.size(); ++l) {
-          std::vector<int> shrt, lng;
-          for (int t = sym.f_lvl_ptr[l]; t < sym.f_lvl_ptr[l + 1]; ++t)
-            (sym.f_op_ptr[t + 1] - sym.f_op_ptr[t] >= kLongOps ? lng : shrt).push_back(t);
-          auto add = [&](int t) {
-            ftg.push_back(sym.f_tgt[t]);
-            for (int o = sym.f_op_ptr[t]; o < sym.f_op_ptr[t + 1]; ++o) {
-              fops.push_back(sym.f_ops[2 * o]); fops.push_back(sym.f_ops[2 * o + 1]);
+        // Refactor program with the forward solve L y = P b folded in: y_i
+        // (entry nlu + i) is a target with the operations y_i -= L_ik y_k,
+        // scheduled one level after the last L_ik and y_k it reads, so the
+        // refactor's level sweep also produces y.  Within each level the
+        // targets with >= kLongOps operations go last (a wavefront each);
+        // fls[l] = first of them.
+        const int nlu0 = C.nlu;
+        std::vector<int> plev(nlu0, 0);
+        for (size_t l = 0; l + 1 < sym.f_lvl_ptr.size(); ++l)
+          for (int t = sym.f_lvl_ptr[l]; t < sym.f_lvl_ptr[l + 1]; ++t) plev[sym.f_tgt[t]] = (int)l;
+        int nlev = (int)sym.f_lvl_ptr.size() - 1;
+        std::vector<int> ylev(n, 0);
+        for (int i = 0; i < n; ++i) {
+          int lv = 0;
+          for (int q = sym.lu_rowptr[i]; q < sym.lu_diag[i]; ++q)
+            lv = std::max(lv, std::max(plev[q], ylev[sym.lu_colind[q]]) + 1);
+          ylev[i] = lv;
+          nlev = std::max(nlev, lv + 1);
+        }
+        std::vector<std::vector<int>> ylist(nlev);
+        for (int i = 0; i < n; ++i) ylist[ylev[i]].push_back(i);
+        std::vector<int> ftg, fop{0}, fops, fdv, fls, flp{0};
+        for (int l = 0; l < nlev; ++l) {
+          std::vector<int> shrt, lng;   // >= 0: factor target index t; < 0: forward-solve row -1 - i
+          if (l + 1 < (int)sym.f_lvl_ptr.size())
+            for (int t = sym.f_lvl_ptr[l]; t < sym.f_lvl_ptr[l + 1]; ++t)
+              (sym.f_op_ptr[t + 1] - sym.f_op_ptr[t] >= kLongOps ? lng : shrt).push_back(t);
+          for (int i : ylist[l]) (sym.lu_diag[i] - sym.lu_rowptr[i] >= kLongOps ? lng : shrt).push_back(-1 - i);
+          auto add = [&](int e) {
+            if (e >= 0) {
+              ftg.push_back(sym.f_tgt[e]);
+              for (int o = sym.f_op_ptr[e]; o < sym.f_op_ptr[e + 1]; ++o) {
+                fops.push_back(sym.f_ops[2 * o]); fops.push_back(sym.f_ops[2 * o + 1]);
+              }
+              fdv.push_back(fdiv16[e]);
+            } else {
+              const int i = -1 - e;
+              ftg.push_back(nlu0 + i);
+              for (int q = sym.lu_rowptr[i]; q < sym.lu_diag[i]; ++q) {
+                fops.push_back(q); fops.push_back(nlu0 + sym.lu_colind[q]);
+              }
+              fdv.push_back(0xFFFF);
             }
             fop.push_back((int)fops.size() / 2);
-            fdv.push_back(fdiv16[t]);
           };
-          for (int t : shrt) add(t);
+          for (int e : shrt) add(e);
           fls.push_back((int)ftg.size());
-          for (int t : lng) add(t);
+          for (int e : lng) add(e);
+          flp.push_back((int)ftg.size());
         }
+        C.nflev = nlev;
+        C.nops = (int)fops.size() / 2;
         // solve rows: long L (forward) / U (backward) parts last in their level
         auto split_rows = [&](const std::vector<int>& lp, const std::vector<int>& rows, bool fwd,
                               std::vector<int>& out, std::vector<int>& split) {
@@ -1870,8 +1918,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
             out.insert(out.end(), lng.begin(), lng.end());
           }
         };
-        std::vector<int> fsr, fss, bsr, bss;
-        split_rows(sym.fs_lvl_ptr, sym.fs_rows, true, fsr, fss);
+        std::vector<int> bsr, bss;
         split_rows(sym.bs_lvl_ptr, sym.bs_rows, false, bsr, bss);
         // linear entries per row: CSR slots with a nonzero linear value
         std::vector<int> lrp{0}, lk;
@@ -1880,10 +1927,19 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
             if (hc.linF[k] != 0.0 || hc.linQ[k] != 0.0) lk.push_back(k);
           lrp.push_back((int)lk.size());
         }
-        C.b_flp = put(sym.f_lvl_ptr); C.b_ftg = put(ftg); C.b_fop = put(fop); C.b_fdiv = put(fdv);
-        C.b_fops = put(fops); C.b_fls = put(fls);
+        std::vector<int> fmeta;
+        for (size_t t = 0; t < ftg.size(); ++t) {
+          fmeta.push_back(ftg[t]); fmeta.push_back(fdv[t]);
+          fmeta.push_back(fop[t]); fmeta.push_back(fop[t + 1] - fop[t]);
+        }
+        auto align = [&](size_t a) { while ((blob.size() - b0) % a) blob.push_back(0); };
+        align(4);
+        C.b_fmeta = put(fmeta);
+        align(2);
+        C.b_fops = put(fops);
+        C.b_flp = put(flp); C.b_fls = put(fls);
         C.b_lurp = put(sym.lu_rowptr); C.b_luci = put(sym.lu_colind);
-        C.b_diag = put(sym.lu_diag); C.b_fsp = put(sym.fs_lvl_ptr); C.b_fsr = put(fsr); C.b_fss = put(fss);
+        C.b_diag = put(sym.lu_diag);
         C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(bsr); C.b_bss = put(bss); C.b_prow = put(sym.perm_row);
         C.b_pcol = put(sym.perm_col); C.b_rp = put(hc.rowptr); C.b_cc = put(hc.colind); C.b_a2lu = put(sym.a2lu);
         C.b_vptr = put(vptr); C.b_mptr = put(mptr); C.b_lrp = put(lrp); C.b_lk = put(lk);
@@ -1893,9 +1949,11 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           for (int r = 0; r < hc.n; ++r) vflag[r] = hc.var_type[r] == 'V' ? 1 : 0;
           C.b_vrow = put(vflag);
         }
-        blob_ok = (int)ftg.size() == C.nlu && (int)fls.size() == C.nflev && (int)fsr.size() == n &&
-                  (int)bsr.size() == n && (int)fss.size() == C.nfs && (int)bss.size() == C.nbs &&
-                  fop.back() == C.nops && (int)fops.size() == 2 * C.nops;
+        bool ops_in = true;
+        for (int x : fops) ops_in = ops_in && x >= 0 && x < nlu0 + n;
+        blob_ok = (int)ftg.size() == nlu0 + n && (int)fls.size() == C.nflev && (int)flp.size() == C.nflev + 1 &&
+                  (int)bsr.size() == n && (int)bss.size() == C.nbs && fop.back() == C.nops &&
+                  (int)fops.size() == 2 * C.nops && ops_in && nlu0 + n < 0xFFFF && C.nops < 0xFFFF;
         while ((blob.size() - b0) % 8) blob.push_back(0);
         C.blob_off = (int)b0;
         C.blob_len = (int)(blob.size() - b0);
@@ -1925,7 +1983,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       max_blob = std::max(max_blob, C.blob_len);
       b->classes.push_back(C);
       max_n = std::max(max_n, C.n);
-      max_nlu = std::max(max_nlu, C.nlu);
+      max_nlu = std::max(max_nlu, C.nlu + C.n);
     } else {
       cid = it->second;
     }
@@ -2028,7 +2086,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.pred_dead = 0.0;
   A.dev_inst = b->d_dev_inst.p; A.lim = b->d_lim.p;
   b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
-  for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu, C.blob_len));
+  for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu + C.n, C.blob_len));
   for (const InstDesc& D : b->insts) b->inst_cls.push_back(D.cls);
   if (b->shmem > kMaxLdsBytes || max_n > 64 * kMaxChunks) return XG_E_ARG;
   b->set_order(1);

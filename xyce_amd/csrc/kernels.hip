@@ -67,16 +67,20 @@ __global__ void __launch_bounds__(256) k_mos1_eval(EvalArgs a) {
   if (i >= a.n_mos) return;
   const int N = a.n_mos;
   const double* k = a.cls + (size_t)a.mcls[i] * KSTRIDE;
-  double volt[6], sc[6], sn[6], ac[8], so[6], ao[8];
+  double volt[6], sc[6], sn[6], ac[8];
   for (int j = 0; j < 6; ++j) volt[j] = a.x[a.lid[6 * i + j]];
   for (int j = 0; j < 6; ++j) { sc[j] = a.sto_cur[(size_t)j * N + i]; sn[j] = a.sto_nxt_in[(size_t)j * N + i]; }
   for (int j = 0; j < 8; ++j) ac[j] = a.sta_cur[(size_t)j * N + i];
   const unsigned long long m = a.mask[i];
   double* rec = a.rec;
   auto emit = [&](int plane, double v) { if (m >> plane & 1ull) rec[(size_t)plane * N + i] = v; };
-  const int lim = mos1_eval(k, a.tvbi[i], a.beta[i], a.tvto[i], volt, sc, sn, ac, a.fl, so, ao, emit);
-  for (int j = 0; j < 6; ++j) a.sto_out[(size_t)j * N + i] = so[j];
-  for (int j = 0; j < 8; ++j) a.sta_out[(size_t)j * N + i] = ao[j];
+  // the store and state inputs are all in registers before the first output
+  // is written (the outputs may be the input arrays)
+  auto keep = [&](int j, double v) {
+    if (j < 6) a.sto_out[(size_t)j * N + i] = v;
+    else a.sta_out[(size_t)(j - 6) * N + i] = v;
+  };
+  const int lim = mos1_eval(k, a.tvbi[i], a.beta[i], a.tvto[i], volt, sc, sn, ac, a.fl, keep, emit);
   if (lim) atomicOr(a.limited, 1);
 }
 

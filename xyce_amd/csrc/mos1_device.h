@@ -116,13 +116,14 @@ struct MosFlags {
 };
 
 // Evaluates one device.  volt[6] = V(d g s b d' s'); sto_cur/sto_nxt/sta_cur
-// are the device's 6 store / 8 state values; outputs the new store/state and
-// the record terms through emit(plane, value).  Returns the limiter flag.
-template <class Emit>
+// are the device's 6 store / 8 state values; outputs the new store values
+// through keep(O_*, value) and state values through keep(6 + S_*, value) as
+// soon as they are known, and the record terms through emit(plane, value).
+// Returns the limiter flag.
+template <class Keep, class Emit>
 __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tvbi, double beta, double tvto,
                                          const double volt[6], const double sto_cur[6], const double sto_nxt[6],
-                                         const double sta_cur[8], const MosFlags fl, double sto_out[6],
-                                         double sta_out[8], Emit emit) {
+                                         const double sta_cur[8], const MosFlags fl, Keep keep, Emit emit) {
   const int dtype = (int)k[K_DTYPE];
   const double gamma = k[K_GAMMA], lambda = k[K_LAMBDA], tphi = k[K_TPHI], vt = k[K_VT];
   const double gmin = k[K_GMIN], M = k[K_M], gd = k[K_GD], gs = k[K_GS];
@@ -279,8 +280,8 @@ __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tv
   else { Gm = -gm; Gmbs = -gmbs; nrmsum = 0; revsum = -(Gm + Gmbs); cdreq = -(dtype)*cdrain; }
 
   // store / state (Master::updateState)
-  sto_out[O_VBD] = vbd; sto_out[O_VBS] = vbs; sto_out[O_VGS] = vgs; sto_out[O_VDS] = vds;
-  sto_out[O_VON] = von; sto_out[O_GM] = Gm;
+  keep(O_VBD, vbd); keep(O_VBS, vbs); keep(O_VGS, vgs); keep(O_VDS, vds);
+  keep(O_VON, von); keep(O_GM, Gm);
   double qgs, qgd, qgb;
   if (fl.dcop) {
     qgs = Capgs * vgs; qgd = Capgd * vgd; qgb = Capgb * Vgb;
@@ -292,9 +293,9 @@ __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tv
     qgd += Capgd * (vgd - vgd1);
     qgb += Capgb * ((vgs - vbs) - vgb1);
   }
-  sta_out[S_QGS] = qgs; sta_out[S_QGD] = qgd; sta_out[S_QGB] = qgb;
-  sta_out[S_CAPGS] = capgs; sta_out[S_CAPGD] = capgd; sta_out[S_CAPGB] = capgb;
-  sta_out[S_QBD] = qbd; sta_out[S_QBS] = qbs;
+  keep(6 + S_QGS, qgs); keep(6 + S_QGD, qgd); keep(6 + S_QGB, qgb);
+  keep(6 + S_CAPGS, capgs); keep(6 + S_CAPGD, capgd); keep(6 + S_CAPGB, capgb);
+  keep(6 + S_QBD, qbd); keep(6 + S_QBS, qbs);
 
   // ---- vector terms (loadDAEVectors)
   const double ceqbs = dtype * (cbs), ceqbd = dtype * (cbd), ceqgb = 0.0, ceqgs = 0.0, ceqgd = 0.0;
