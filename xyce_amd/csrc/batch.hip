@@ -726,6 +726,10 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     const int* tmask = ci + C.o_tmask;
     int lim_any = 0;
     for (int d = tid; d < C.n_mos; d += nth) {
+      // one device at a time: without this fence the compiler starts the
+      // next device's loads inside the current one and holds both devices'
+      // values at once (a hundred more registers)
+      asm volatile("" ::: "memory");
       const int g = I.dev_off + d;
       const double* k = kcls + (size_t)A.mcls[g] * KSTRIDE;
       double volt[6], sc[6], sn[6], ac[8];
