@@ -64,7 +64,7 @@ struct ClassDesc {
   // (4*row + vector) then matrix groups (2*slot + F/Q), at o_long
   int n_vlong, n_mlong, o_long;
   // uint16 program blob (staged into LDS each launch); offsets in uint16 units
-  int blob_off, blob_len;
+  int blob_off, blob_len, blob_lds;   // blob_lds: the leading part staged into LDS (the rest is read in HBM)
   int b_flp, b_fmeta, b_fops, b_lurp, b_luci, b_diag, b_bsp, b_bsr, b_prow, b_pcol,
       b_rp, b_cc, b_a2lu;
   // long LU targets / solve rows (>= kLongOps terms, one wavefront each) come
@@ -188,6 +188,7 @@ constexpr double kHmin = 1e-18;
 constexpr int kLongSeg = 32;     // term groups this long are reduced by a wavefront
 constexpr int kMaxLong = 16;     // per class
 constexpr int kLongOps = 16;     // LU targets / solve rows this long are reduced by a wavefront
+constexpr int kMaxKclsLds = 4;   // MOSFET class records the fused kernel stages in LDS
 constexpr int kMaxSrc = 64;      // sources per instance (their values are staged in LDS)
 
 // Sum over rows 0..n-1 whose value does not depend on the workgroup size:
@@ -537,8 +538,9 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
                                  double* rhs, double* red, int* sh_i) {
   const int n = C.n, tid = threadIdx.x, nth = blockDim.x;
   double* D = A.dense + I.dense_off;
-  const unsigned short* rp = prog + C.b_rp; const unsigned short* cc = prog + C.b_cc;
-  const unsigned short* mptr = prog + C.b_mptr;
+  const unsigned short* gprog = A.blob + C.blob_off;
+  const unsigned short* rp = gprog + C.b_rp; const unsigned short* cc = gprog + C.b_cc;
+  const unsigned short* mptr = gprog + C.b_mptr;
   const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
   const int* tix = ci + C.o_tix;
   for (int e = tid; e < n * n; e += nth) D[e] = 0.0;
@@ -556,7 +558,7 @@ __device__ int dense_pivot_solve(const BatchArgs& A, const ClassDesc& C, const I
   if (gdiag != 0.0) {
     // gmin stepping: the continuation conductance on every node-voltage row
     // (GStepping::augmentJacobian)
-    const unsigned short* vr = prog + C.b_vrow;
+    const unsigned short* vr = gprog + C.b_vrow;
     for (int r = tid; r < n; r += nth)
       if (vr[r]) D[(size_t)r * n + r] += gdiag;
     __syncthreads();
@@ -675,6 +677,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   const int tid = threadIdx.x, nth = blockDim.x;
   const int n = C.n, off = I.vec_off;
   const int* ci = A.ci;
+  const unsigned short* gprog = A.blob + C.blob_off;   // the program's part read in HBM
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
   // T: the instance's slot-ordered term array (HBM, or LDS in the fused form)
@@ -761,8 +764,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   {
     const int nl = C.n_vlong + C.n_mlong;
     const int* lg = ci + C.o_long;
-    const unsigned short* vptr = prog + C.b_vptr;
-    const unsigned short* mptr = prog + C.b_mptr;
+    const unsigned short* vptr = gprog + C.b_vptr;
+    const unsigned short* mptr = gprog + C.b_mptr;
     const int* tix = ci + C.o_tix;
     const int wave = tid >> 6, lane = tid & 63, nwaves = nth >> 6;
     for (int j = wave; j < nl; j += nwaves) {
@@ -787,15 +790,15 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }
     if (nl) __syncthreads();
   }
-  const unsigned short* cc = prog + C.b_cc;
-  const unsigned short* vrow = prog + C.b_vrow;
+  const unsigned short* cc = gprog + C.b_cc;
+  const unsigned short* vrow = gprog + C.b_vrow;
   // ---- 3. vectors per row (contiguous term ranges) and the residual
   const double h = S.h;
   double n2 = 0.0, ninf = 0.0;
   {
-    const unsigned short* vptr = prog + C.b_vptr;
-    const unsigned short* bgp = prog + C.b_bgp; const unsigned short* bge = prog + C.b_bge;
-    const unsigned short* lrp = prog + C.b_lrp; const unsigned short* lk = prog + C.b_lk;
+    const unsigned short* vptr = gprog + C.b_vptr;
+    const unsigned short* bgp = gprog + C.b_bgp; const unsigned short* bge = gprog + C.b_bge;
+    const unsigned short* lrp = gprog + C.b_lrp; const unsigned short* lk = gprog + C.b_lk;
     const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
     const int* tix = ci + C.o_tix;
     for (int base = tid & ~63; base < n; base += nth) {
@@ -909,13 +912,13 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       // the forward solve runs inside the refactor: its n entries after the
       // factors start from the permuted right-hand side
       {
-        const unsigned short* prow = prog + C.b_prow;
+        const unsigned short* prow = gprog + C.b_prow;
         for (int i = tid; i < n; i += nth) lu[nlu + i] = rhs[prow[i]];
       }
       __syncthreads();
       {
-        const unsigned short* mptr = prog + C.b_mptr;
-        const unsigned short* a2lu = prog + C.b_a2lu;
+        const unsigned short* mptr = gprog + C.b_mptr;
+        const unsigned short* a2lu = gprog + C.b_a2lu;
         const double* linF = A.linF + I.nnz_off; const double* linQ = A.linQ + I.nnz_off;
         const int* lg = ci + C.o_long;
         const int* tix = ci + C.o_tix;
@@ -1011,7 +1014,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       XG_PHASE(7);
       if (!use_dense) {
         // L y = P b ran with the refactor (y after the factors); U x = y here
-        const unsigned short* pcol = prog + C.b_pcol;
+        const unsigned short* pcol = gprog + C.b_pcol;
         const unsigned short* lurp = prog + C.b_lurp; const unsigned short* luci = prog + C.b_luci;
         const unsigned short* dg = prog + C.b_diag;
         const unsigned short* bsp = prog + C.b_bsp; const unsigned short* bsr = prog + C.b_bsr;
@@ -1328,7 +1331,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   __shared__ int red_i[16];
   __shared__ int sh_int[8];
   __shared__ double sh_d[8];
-  __shared__ double kc_s[KLDS ? 16 * KSTRIDE : 1];
+  __shared__ double kc_s[KLDS ? kMaxKclsLds * KSTRIDE : 1];
   __shared__ double srcv[kMaxSrc];
   extern __shared__ double dyn[];
   double* xs = dyn;                         // n + 1
@@ -1344,7 +1347,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
   {
     const uint4* src = reinterpret_cast<const uint4*>(A.blob + C.blob_off);
     uint4* dst = reinterpret_cast<uint4*>(prog);
-    const int chunks = C.blob_len / 8;
+    const int chunks = C.blob_lds / 8;
     for (int i = tid; i < chunks; i += nth) dst[i] = src[i];
   }
   const double* kcls = A.kcls;
@@ -1605,7 +1608,7 @@ struct xg_batch {
         int blob = 0, nterm = 0;
         for (int j = s0; j < s1; ++j) {
           const ClassDesc& C = classes[inst_cls[ord[j]]];
-          bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu + C.n); blob = std::max(blob, C.blob_len);
+          bk.n = std::max(bk.n, C.n); bk.nlu = std::max(bk.nlu, C.nlu + C.n); blob = std::max(blob, C.blob_lds);
           nterm = std::max(nterm, C.tsize);
         }
         bk.shmem = lds_bytes(bk.n, bk.nlu, blob);
@@ -1615,7 +1618,7 @@ struct xg_batch {
       }
     } else {
       Bucket bk{0, n_inst, args.max_n, args.max_nlu, shmem, 0};
-      for (const ClassDesc& C : classes) { bk.blob = std::max(bk.blob, C.blob_len); bk.nterm = std::max(bk.nterm, C.tsize); }
+      for (const ClassDesc& C : classes) { bk.blob = std::max(bk.blob, C.blob_lds); bk.nterm = std::max(bk.nterm, C.tsize); }
       bk.shmem_terms = lds_bytes(bk.n, bk.nlu, ((bk.blob + 3) & ~3)) + sizeof(double) * (size_t)bk.nterm;
       buckets.push_back(bk);
     }
@@ -1649,7 +1652,7 @@ struct xg_batch {
         if (hipStreamWaitEvent(s, fork_ev, 0) != hipSuccess) return XG_E_HIP;
       }
       if (split) hipLaunchKernelGGL((k_batch_step<false, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
-      else if (a.n_kcls > 16) hipLaunchKernelGGL((k_batch_step<true, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
+      else if (a.n_kcls > kMaxKclsLds) hipLaunchKernelGGL((k_batch_step<true, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
       else if (lt) hipLaunchKernelGGL((k_batch_step<true, true, true>), dim3(bk.count), dim3(bs), shm, s, a);
       else hipLaunchKernelGGL((k_batch_step<true, true, false>), dim3(bk.count), dim3(bs), shm, s, a);
       if (side) {
@@ -1944,7 +1947,11 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         C.b_flp = put(flp); C.b_fls = put(fls);
         C.b_lurp = put(sym.lu_rowptr); C.b_luci = put(sym.lu_colind);
         C.b_diag = put(sym.lu_diag);
-        C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(bsr); C.b_bss = put(bss); C.b_prow = put(sym.perm_row);
+        C.b_bsp = put(sym.bs_lvl_ptr); C.b_bsr = put(bsr); C.b_bss = put(bss);
+        // the arrays read once per iteration stay in HBM
+        align(8);
+        C.blob_lds = (int)(blob.size() - b0);
+        C.b_prow = put(sym.perm_row);
         C.b_pcol = put(sym.perm_col); C.b_rp = put(hc.rowptr); C.b_cc = put(hc.colind); C.b_a2lu = put(sym.a2lu);
         C.b_vptr = put(vptr); C.b_mptr = put(mptr); C.b_lrp = put(lrp); C.b_lk = put(lk);
         C.b_bgp = put(bgp_h); C.b_bge = put(bge_h);
@@ -1984,7 +1991,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         for (auto& e : hc.srcs) ok = ok && e.li_p <= C.n && e.li_n <= C.n && e.wave >= 0 && e.wave < (int)hc.wave_type.size();
         if (!ok) return XG_E_SETUP;
       }
-      max_blob = std::max(max_blob, C.blob_len);
+      max_blob = std::max(max_blob, C.blob_lds);
       b->classes.push_back(C);
       max_n = std::max(max_n, C.n);
       max_nlu = std::max(max_nlu, C.nlu + C.n);
@@ -2090,7 +2097,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
   A.pred_dead = 0.0;
   A.dev_inst = b->d_dev_inst.p; A.lim = b->d_lim.p;
   b->shmem = xg_batch::lds_bytes(max_n, max_nlu, max_blob);
-  for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu + C.n, C.blob_len));
+  for (const ClassDesc& C : b->classes) b->cls_lds.push_back(xg_batch::lds_bytes(C.n, C.nlu + C.n, C.blob_lds));
   for (const InstDesc& D : b->insts) b->inst_cls.push_back(D.cls);
   if (b->shmem > kMaxLdsBytes || max_n > 64 * kMaxChunks) return XG_E_ARG;
   b->set_order(1);
@@ -2339,7 +2346,7 @@ int xg_batch_autotune(xg_batch* b, double* result) {
         for (int bs : gblocks) {
           for (int lt = 0; lt < 2; ++lt) {
             // the term arrays in LDS where they fit a workgroup's LDS
-            if (lt && (b->buckets[j].shmem_terms > kMaxLdsBytes || b->args.n_kcls > 16)) continue;
+            if (lt && (b->buckets[j].shmem_terms > kMaxLdsBytes || b->args.n_kcls > kMaxKclsLds)) continue;
             b->buckets[j].block = bs;
             b->buckets[j].terms = lt != 0;
             b->only_bucket = (int)j;
@@ -2445,7 +2452,7 @@ int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval
   if (split_eval >= 0) {
     b->split = split_eval == 1 ? 1 : 0;
     // 2: fused, each group's term arrays in LDS where they fit; 0: in HBM
-    for (auto& bk : b->buckets) bk.terms = split_eval == 2 && bk.shmem_terms <= kMaxLdsBytes && b->args.n_kcls <= 16;
+    for (auto& bk : b->buckets) bk.terms = split_eval == 2 && bk.shmem_terms <= kMaxLdsBytes && b->args.n_kcls <= kMaxKclsLds;
   }
   if (iters_per_launch) b->args.iters = iters_per_launch;
   if (b->split) b->args.iters = 1;
