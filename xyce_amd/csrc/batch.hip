@@ -983,9 +983,17 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
             const uint2 m = fmeta[j];
             const int p = m.x & 0xFFFF, dv = m.x >> 16, o0 = m.y & 0xFFFF, no = m.y >> 16;
             double acc = 0.0;
-            for (int o = lane; o < no; o += 64) {
-              const unsigned pr = fops[o0 + o];
-              acc += lu[pr & 0xFFFF] * lu[pr >> 16];
+            // four strides at a time: index pairs, then operands, then the sums in order
+            for (int o = lane; o < no; o += 256) {
+              unsigned pr[4];
+              double fa[4], fb[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) pr[q] = o + 64 * q < no ? fops[o0 + o + 64 * q] : 0u;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) { fa[q] = lu[pr[q] & 0xFFFF]; fb[q] = lu[pr[q] >> 16]; }
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (o + 64 * q < no) acc += fa[q] * fb[q];
             }
             for (int sh = 32; sh > 0; sh >>= 1) acc += __shfl_xor(acc, sh, 64);
             if (lane == 0) {
@@ -1887,6 +1895,13 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
             for (int t = sym.f_lvl_ptr[l]; t < sym.f_lvl_ptr[l + 1]; ++t)
               (sym.f_op_ptr[t + 1] - sym.f_op_ptr[t] >= kLongOps ? lng : shrt).push_back(t);
           for (int i : ylist[l]) (sym.lu_diag[i] - sym.lu_rowptr[i] >= kLongOps ? lng : shrt).push_back(-1 - i);
+          // targets with no operation and no division keep the value the
+          // Jacobian scatter (or the right-hand side) put there: no work
+          auto noop = [&](int e) {
+            return e >= 0 ? sym.f_op_ptr[e + 1] == sym.f_op_ptr[e] && fdiv16[e] == 0xFFFF
+                          : sym.lu_diag[-1 - e] == sym.lu_rowptr[-1 - e];
+          };
+          shrt.erase(std::remove_if(shrt.begin(), shrt.end(), noop), shrt.end());
           auto add = [&](int e) {
             if (e >= 0) {
               ftg.push_back(sym.f_tgt[e]);
@@ -1908,8 +1923,9 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
           fls.push_back((int)ftg.size());
           for (int e : lng) add(e);
           flp.push_back((int)ftg.size());
+          if (flp.back() == flp[flp.size() - 2]) { flp.pop_back(); fls.pop_back(); }   // a level of copies only
         }
-        C.nflev = nlev;
+        C.nflev = (int)flp.size() - 1;
         C.nops = (int)fops.size() / 2;
         // solve rows: long L (forward) / U (backward) parts last in their level
         auto split_rows = [&](const std::vector<int>& lp, const std::vector<int>& rows, bool fwd,
@@ -1962,7 +1978,7 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
         }
         bool ops_in = true;
         for (int x : fops) ops_in = ops_in && x >= 0 && x < nlu0 + n;
-        blob_ok = (int)ftg.size() == nlu0 + n && (int)fls.size() == C.nflev && (int)flp.size() == C.nflev + 1 &&
+        blob_ok = (int)ftg.size() <= nlu0 + n && (int)fls.size() == C.nflev && (int)flp.size() == C.nflev + 1 &&
                   (int)bsr.size() == n && (int)bss.size() == C.nbs && fop.back() == C.nops &&
                   (int)fops.size() == 2 * C.nops && ops_in && nlu0 + n < 0xFFFF && C.nops < 0xFFFF;
         while ((blob.size() - b0) % 8) blob.push_back(0);
