@@ -108,7 +108,10 @@ struct InstState {
   // converged value and step, the conductance and source scale in force,
   // whether the next solve starts from initJct, and the steps taken
   double dc_stage, cont_par, cont_ok, cont_step, gcond, src_scale, cont_first, cont_k;
-  double pad[10];
+  // which of the two store/state buffers holds the last accepted values (the
+  // other takes the iterate's)
+  int sbuf, pad_i;
+  double pad[9];
 };
 
 static_assert(sizeof(InstState) == 512, "InstState is one 512-byte record");
@@ -133,6 +136,8 @@ struct BatchArgs {
   int nt_mos;
   const double* kcls; const int* mcls; const double* tvbi; const double* beta; const double* tvto;
   const unsigned long long* mask;
+  // store / state values, two buffers each: an instance's accepted values are
+  // in buffer sbuf, its iterate's in the other (accepting a step flips sbuf)
   double* sto_cur; double* sto_nxt; double* sta_cur; double* sta_nxt;
   double* terms;               // per-instance term arrays (slot order)
   const unsigned short* blob;  // per-class uint16 programs
@@ -199,6 +204,17 @@ constexpr int kMaxSrc = 64;      // sources per instance (their values are stage
 // wave_tree_sum, store it in chs[base / 64] from lane 0 and then call
 // chunk_total.
 constexpr int kMaxChunks = 128;          // rows per instance <= 64 * kMaxChunks
+// An instance's accepted (cur) and iterate (nxt) store / state planes.
+struct DevState {
+  double *sto_cur, *sto_nxt, *sta_cur, *sta_nxt;
+};
+__device__ __forceinline__ DevState dev_state(const BatchArgs& A, int sbuf) {
+  DevState d;
+  d.sto_cur = sbuf ? A.sto_nxt : A.sto_cur; d.sto_nxt = sbuf ? A.sto_cur : A.sto_nxt;
+  d.sta_cur = sbuf ? A.sta_nxt : A.sta_cur; d.sta_nxt = sbuf ? A.sta_cur : A.sta_nxt;
+  return d;
+}
+
 __device__ __forceinline__ double wave_tree_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
@@ -680,6 +696,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   const unsigned short* gprog = A.blob + C.blob_off;   // the program's part read in HBM
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
+  const DevState DS = dev_state(A, S.sbuf);
   // T: the instance's slot-ordered term array (HBM, or LDS in the fused form)
   // phase timers (one instance, thread 0, after each phase's barrier)
   const bool prof_on = A.prof != nullptr && blockIdx.x == 0 && tid == 0;
@@ -737,8 +754,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const double* k = kcls + (size_t)A.mcls[g] * KSTRIDE;
       double volt[6], sc[6], sn[6], ac[8];
       for (int j = 0; j < 6; ++j) volt[j] = xs[lid[j * C.n_mos + d]];
-      for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
-      for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
+      for (int j = 0; j < 6; ++j) { sc[j] = DS.sto_cur[(size_t)j * NT + g]; sn[j] = DS.sto_nxt[(size_t)j * NT + g]; }
+      for (int j = 0; j < 8; ++j) ac[j] = DS.sta_cur[(size_t)j * NT + g];
       // class arrays are plane-major ([plane][device]): a wavefront's loads are coalesced
       const unsigned mlo = (unsigned)tmask[d], mhi = (unsigned)tmask[C.n_mos + d];
       double* Td = T + d;
@@ -747,8 +764,8 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
         if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
       };
       auto keep = [&](int j, double v) {
-        if (j < 6) A.sto_nxt[(size_t)j * NT + g] = v;
-        else A.sta_nxt[(size_t)(j - 6) * NT + g] = v;
+        if (j < 6) DS.sto_nxt[(size_t)j * NT + g] = v;
+        else DS.sta_nxt[(size_t)(j - 6) * NT + g] = v;
       };
       lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, keep, emit);
     }
@@ -1100,11 +1117,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     if (act == 1) {
       // converged continuation step: it becomes the restart point (xH1 is free during DC)
       for (int i = tid; i < n; i += nth) A.xH1[off + i] = xs[i];
-      for (int d = tid; d < C.n_mos; d += nth) {
-        const int g = I.dev_off + d;
-        for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
-        for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
-      }
+      if (tid == 0) S.sbuf ^= 1;   // the iterate's store/state become the accepted ones
       return;
     }
     if (act == 3) {
@@ -1121,17 +1134,14 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       for (int i = tid; i < n; i += nth) A.x[off + i] = A.xH1[off + i];
       for (int d = tid; d < C.n_mos; d += nth) {
         const int g = I.dev_off + d;
-        for (int j = 0; j < 6; ++j) A.sto_nxt[(size_t)j * NT + g] = A.sto_cur[(size_t)j * NT + g];
-        for (int j = 0; j < 8; ++j) A.sta_nxt[(size_t)j * NT + g] = A.sta_cur[(size_t)j * NT + g];
+        for (int j = 0; j < 6; ++j) DS.sto_nxt[(size_t)j * NT + g] = DS.sto_cur[(size_t)j * NT + g];
+        for (int j = 0; j < 8; ++j) DS.sta_nxt[(size_t)j * NT + g] = DS.sta_cur[(size_t)j * NT + g];
       }
       return;
     }
-    // tranop -> transient: OneStep::initialize (N_TIA_OneStep.C:1684)
-    for (int d = tid; d < C.n_mos; d += nth) {
-      const int g = I.dev_off + d;
-      for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
-      for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
-    }
+    // tranop -> transient: OneStep::initialize (N_TIA_OneStep.C:1684); the
+    // operating point's store/state become the accepted ones
+    if (tid == 0) S.sbuf ^= 1;
     for (int i = tid; i < n; i += nth) {
       A.xH0[off + i] = xs[i];
       A.qH0[off + i] = A.Q[off + i];
@@ -1233,11 +1243,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
         A.qH0[off + i] = qv;
       }
     }
-    for (int d = tid; d < C.n_mos; d += nth) {
-      const int g = I.dev_off + d;
-      for (int j = 0; j < 6; ++j) A.sto_cur[(size_t)j * NT + g] = A.sto_nxt[(size_t)j * NT + g];
-      for (int j = 0; j < 8; ++j) A.sta_cur[(size_t)j * NT + g] = A.sta_nxt[(size_t)j * NT + g];
-    }
+    if (tid == 0) S.sbuf ^= 1;   // the iterate's store/state become the accepted ones
     if (A.pred_mode != 0) pred_accept(A, I, C, S, xs);
     if (tid == 0) S.t = S.t + S.h;
     __syncthreads();
@@ -1407,8 +1413,9 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
     const int l = lid[j * C.n_mos + d];
     volt[j] = l < n ? A.x[off + l] : 0.0;
   }
-  for (int j = 0; j < 6; ++j) { sc[j] = A.sto_cur[(size_t)j * NT + g]; sn[j] = A.sto_nxt[(size_t)j * NT + g]; }
-  for (int j = 0; j < 8; ++j) ac[j] = A.sta_cur[(size_t)j * NT + g];
+  const DevState DS = dev_state(A, A.st[inst].sbuf);
+  for (int j = 0; j < 6; ++j) { sc[j] = DS.sto_cur[(size_t)j * NT + g]; sn[j] = DS.sto_nxt[(size_t)j * NT + g]; }
+  for (int j = 0; j < 8; ++j) ac[j] = DS.sta_cur[(size_t)j * NT + g];
   const unsigned mlo = (unsigned)tmask[d], mhi = (unsigned)tmask[C.n_mos + d];
   double* Td = T + d;
   const int nm = C.n_mos;
@@ -1416,8 +1423,8 @@ __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
     if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
   };
   auto keep = [&](int j, double v) {
-    if (j < 6) A.sto_nxt[(size_t)j * NT + g] = v;
-    else A.sta_nxt[(size_t)(j - 6) * NT + g] = v;
+    if (j < 6) DS.sto_nxt[(size_t)j * NT + g] = v;
+    else DS.sta_nxt[(size_t)(j - 6) * NT + g] = v;
   };
   const int lim = mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, keep, emit);
   if (lim) atomicOr(&A.lim[inst], 1);
