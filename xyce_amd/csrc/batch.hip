@@ -758,10 +758,13 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       for (int j = 0; j < 8; ++j) ac[j] = DS.sta_cur[(size_t)j * NT + g];
       // class arrays are plane-major ([plane][device]): a wavefront's loads are coalesced
       const unsigned mlo = (unsigned)tmask[d], mhi = (unsigned)tmask[C.n_mos + d];
-      double* Td = T + d;
+      // mos1_eval emits the planes in increasing order, so a plane's rank
+      // among the device's nonzero planes is the count of stores before it:
+      // the term pointer advances by one device column after each store
+      double* Tq = T + d;
       const int nm = C.n_mos;
       auto emit = [&](int plane, double v) {
-        if (term_on(mlo, mhi, plane)) Td[(size_t)term_rank(mlo, mhi, plane) * nm] = v;
+        if (term_on(mlo, mhi, plane)) { *Tq = v; Tq += nm; }
       };
       auto keep = [&](int j, double v) {
         if (j < 6) DS.sto_nxt[(size_t)j * NT + g] = v;

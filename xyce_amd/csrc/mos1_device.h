@@ -118,8 +118,9 @@ struct MosFlags {
 // Evaluates one device.  volt[6] = V(d g s b d' s'); sto_cur/sto_nxt/sta_cur
 // are the device's 6 store / 8 state values; outputs the new store values
 // through keep(O_*, value) and state values through keep(6 + S_*, value) as
-// soon as they are known, and the record terms through emit(plane, value).
-// Returns the limiter flag.
+// soon as they are known, and the record terms through emit(plane, value),
+// every plane once and in increasing plane order (the batch engine's term
+// store advances one position per nonzero plane).  Returns the limiter flag.
 template <class Keep, class Emit>
 __device__ __forceinline__ int mos1_eval(const double* __restrict__ k, double tvbi, double beta, double tvto,
                                          const double volt[6], const double sto_cur[6], const double sto_nxt[6],
