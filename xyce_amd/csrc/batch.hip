@@ -710,26 +710,15 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
     }                                                            \
   } while (0)
   if (prof_on) A.prof[15] += 1;
+  // (the loop's barrier after the previous iteration has retired every
+  // reader of these; their first use in this one follows the next barrier)
   if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
-  __syncthreads();
   for (int i = tid; i < n; i += nth) xs[i] = A.x[off + i];
   if (tid == 0) xs[n] = 0.0;
   if (!FUSED && tid == 0) {
     // the evaluation kernel ran just before this launch
     sh_int[0] = A.lim[inst];
     A.lim[inst] = 0;
-  }
-  {
-    // source values at this iteration's time, staged in LDS (read after the
-    // barrier that ends the device evaluation)
-    const double ts = dc ? 0.0 : S.t + S.h;
-    const double sscale = dc ? S.src_scale : 1.0;     // source stepping (VSRCSCALE)
-    for (int k = tid; k < C.n_src; k += nth) {
-      const int w = A.src_wave[I.src_off + k];
-      const int wo = A.wave_off[w];
-      srcv[k] = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, ts, A.tstep, A.tstop);
-      if (sscale != 1.0) srcv[k] *= sscale;
-    }
   }
   __syncthreads();
   XG_PHASE(0);
@@ -773,6 +762,19 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       lim_any |= mos1_eval(k, A.tvbi[g], A.beta[g], A.tvto[g], volt, sc, sn, ac, fl, keep, emit);
     }
     if (lim_any) atomicOr(&sh_int[0], 1);
+  }
+  {
+    // source values at this iteration's time, staged in LDS for the vector
+    // gather; the last threads (one evaluation pass at most) compute them
+    const double ts = dc ? 0.0 : S.t + S.h;
+    const double sscale = dc ? S.src_scale : 1.0;     // source stepping (VSRCSCALE)
+    for (int k = nth - 1 - tid; k < C.n_src; k += nth) {
+      if (k < 0) break;
+      const int w = A.src_wave[I.src_off + k];
+      const int wo = A.wave_off[w];
+      srcv[k] = wave_value(A.wave_type[w], A.wave_data + wo, A.wave_off[w + 1] - wo, ts, A.tstep, A.tstop);
+      if (sscale != 1.0) srcv[k] *= sscale;
+    }
   }
   __syncthreads();
   XG_PHASE(1);
@@ -825,6 +827,10 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       const int r = base + (tid & 63);
       double res2 = 0.0;
       if (r < n) {
+      // loads that do not wait for the term gather go out first
+      const int lk0 = lrp[r], lk1 = lrp[r + 1], bg0 = bgp[r], bg1 = bgp[r + 1];
+      const double qh0 = A.qH0[off + r];
+      const double qh12 = (!dc && A.method == 1) ? A.qH1[off + r] : A.qH2[off + r];
       double acc[4];
       const int gb[5] = {vptr[4 * r], vptr[4 * r + 1], vptr[4 * r + 2], vptr[4 * r + 3], vptr[4 * r + 4]};
       bool lng = false;
@@ -846,7 +852,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       double f = acc[0], q = acc[1];
       const double fl = acc[2], ql = acc[3];
       double sf = 0.0, sq = 0.0;
-      for (int k = lrp[r]; k < lrp[r + 1]; ++k) {
+      for (int k = lk0; k < lk1; ++k) {
         const int sl = lk[k];
         const double xv = xs[cc[sl]];
         sf += linF[sl] * xv;
@@ -854,15 +860,15 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
       }
       f += sf; q += sq;
       double b = 0.0;
-      for (int e = bgp[r]; e < bgp[r + 1]; ++e) {
+      for (int e = bg0; e < bg1; ++e) {
         const int ent = bge[e];
         const double v = srcv[ent >> 1];
         if (ent & 1) b -= v; else b += v;
       }
       A.F[off + r] = f; A.Q[off + r] = q; A.B[off + r] = b;
       const double res = (!dc && A.method == 1)
-                             ? residual_gear(S.order, h, S.a0, S.a1, S.a2, f, q, b, fl, ql, A.qH0[off + r], A.qH1[off + r])
-                             : residual_combine(dc, S.order, h, f, q, b, fl, ql, A.qH0[off + r], A.qH2[off + r]);
+                             ? residual_gear(S.order, h, S.a0, S.a1, S.a2, f, q, b, fl, ql, qh0, qh12)
+                             : residual_combine(dc, S.order, h, f, q, b, fl, ql, qh0, qh12);
       rhs[r] = (dc && S.gcond != 0.0 && vrow[r]) ? res - S.gcond * xs[r] : res;
       res2 = res * res;
       ninf = fmax(ninf, fabs(res));
@@ -943,6 +949,9 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
         const int* lg = ci + C.o_long;
         const int* tix = ci + C.o_tix;
         for (int s = tid; s < C.nnz; s += nth) {
+          // loads that do not wait for the term gather go out first
+          const double lf = linF[s], lq = linQ[s];
+          const int dst = a2lu[s];
           double part[2];
           const int b[3] = {mptr[2 * s], mptr[2 * s + 1], mptr[2 * s + 2]};
           if (b[1] - b[0] < kLongSeg && b[2] - b[1] < kLongSeg) {
@@ -960,9 +969,9 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
             }
           }
           double df = part[0], dq = part[1];
-          if (linF[s] != 0.0) df += linF[s];
-          if (linQ[s] != 0.0) dq += linQ[s];
-          lu[a2lu[s]] = jac_combine(ja, dq, jb, df);
+          if (lf != 0.0) df += lf;
+          if (lq != 0.0) dq += lq;
+          lu[dst] = jac_combine(ja, dq, jb, df);
         }
       }
       __syncthreads();
