@@ -411,7 +411,9 @@ int xg_mlp_sync(xg_mlp* net);
  * for checking the pattern on machines without a GPU. */
 int xg_host_pattern(const xg_circuit_desc* desc, int* sizes, int* rowptr, int* colind, int* node_gid, int* mos_lid,
                     int* mos_fslot, int* mos_qslot, int* mos_stasto);
-/* The symbolic analysis of xg_lu_symbolic on the host: info as xg_lu_info;
+/* The symbolic analysis of xg_lu_symbolic on the host: info[0..3] as the
+ * first four entries of xg_lu_info (nnz(L+U), factor levels, solve levels,
+ * flops per factorization);
  * B = A(perm_row, perm_col); L+U pattern of B (lu_colind filled if it fits in
  * max_lu entries).  No reference counterpart (KLU's analyse is internal). */
 int xg_host_lu_analyse(int n, const int* rowptr, const int* colind, const double* values, double* info,
