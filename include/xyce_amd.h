@@ -298,7 +298,7 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
  * iters_per_launch iterations (at most 1024) back to back with the
  * instance's state and program kept in LDS (0 = keep); 2: fused, and each
  * group's term arrays
- * (the slot-ordered device contributions) kept in LDS too where they fit;
+ * (the device contributions of one iteration) kept in LDS too where they fit;
  * -1 = keep.  concurrent_groups 1: the
  * instances' LDS-sized groups run side by side on streams of their own, 0:
  * one after another, -1 = keep. */

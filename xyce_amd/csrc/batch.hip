@@ -119,7 +119,7 @@ static_assert(sizeof(InstState) == 512, "InstState is one 512-byte record");
 struct InstDesc {
   int cls, dev_off, vec_off, nnz_off, src_off, rec_off;
   long long dense_off;   // n*n scratch for the partial-pivoting fallback
-  long long term_off;    // this instance's slot-ordered term array
+  long long term_off;    // this instance's term array (device-major, kterm x n_mos)
   int bp_off, n_bp;      // this instance's breakpoints (sorted, ending at tstop)
   int win_off;           // first predictor window (rows of 256 unknowns)
   long long pair_off;    // first training-pair record (capacity * windows)
@@ -139,7 +139,7 @@ struct BatchArgs {
   // store / state values, two buffers each: an instance's accepted values are
   // in buffer sbuf, its iterate's in the other (accepting a step flips sbuf)
   double* sto_cur; double* sto_nxt; double* sta_cur; double* sta_nxt;
-  double* terms;               // per-instance term arrays (slot order)
+  double* terms;               // per-instance term arrays (device-major)
   const unsigned short* blob;  // per-class uint16 programs
   // per-instance vectors (offset vec_off, length n)
   double* x; double* xn0; double* xH0; double* xH1; double* xH2; double* qH0; double* qH1; double* qH2;
@@ -697,7 +697,7 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   const int NT = A.nt_mos;
   const bool dc = S.mode == 0;
   const DevState DS = dev_state(A, S.sbuf);
-  // T: the instance's slot-ordered term array (HBM, or LDS in the fused form)
+  // T: the instance's device-major term array (HBM, or LDS in the fused form)
   // phase timers (one instance, thread 0, after each phase's barrier)
   const bool prof_on = A.prof != nullptr && blockIdx.x == 0 && tid == 0;
   unsigned long long prof_t = prof_on ? wall_clock64() : 0;
@@ -1397,7 +1397,7 @@ __global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
 // Device evaluation of every MOSFET of every instance that is iterating
 // (DC or transient), one thread per device over the structure-of-arrays
 // state: MOSFET1 updateIntermediateVars + the vector and matrix terms,
-// written to their slots of the instance's term array.  Runs before the
+// written to the device's column of the instance's term array.  Runs before the
 // per-instance kernel of the same launch.
 __global__ void __launch_bounds__(256, 4) k_batch_eval(BatchArgs A) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1776,8 +1776,9 @@ int xg_batch_create(int n_inst, const xg_circuit_desc* descs, double tstop, doub
       C.nbs = (int)sym.bs_lvl_ptr.size() - 1;
       auto push = [&](const std::vector<int>& v) { int o = (int)ci.size(); ci.insert(ci.end(), v.begin(), v.end()); return o; };
       const int n = hc.n;
-      // ---- slot-ordered term layout: vector terms grouped by (row, vector),
-      //      matrix terms by (slot, dFdx/dQdx), each group in device order
+      // ---- slot order of the terms (the order the gathers sum them in):
+      //      vector terms grouped by (row, vector), matrix terms by (slot,
+      //      dFdx/dQdx), each group in device order
       static const int vrow[R_JF0] = {0, 1, 2, 3, 4, 5, 1, 3, 4, 5, 3, 4, 5, 1, 3, 4, 5};
       std::vector<int> vcnt(4 * n + 1, 0), mcnt(2 * hc.nnz + 1, 0);
       auto vgroup = [&](int d, int p) { return 4 * hc.mos_lid[(size_t)6 * d + vrow[p]] + rec_vec_id(p); };
