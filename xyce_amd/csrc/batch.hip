@@ -698,18 +698,20 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
   const bool dc = S.mode == 0;
   const DevState DS = dev_state(A, S.sbuf);
   // T: the instance's device-major term array (HBM, or LDS in the fused form)
-  // phase timers (one instance, thread 0, after each phase's barrier)
+  // phase timers (one instance, thread 0, after each phase's barrier); the
+  // launch groups may run side by side on their own streams, so the sums are
+  // atomic
   const bool prof_on = A.prof != nullptr && blockIdx.x == 0 && tid == 0;
   unsigned long long prof_t = prof_on ? wall_clock64() : 0;
 #define XG_PHASE(k)                                              \
   do {                                                           \
     if (prof_on) {                                               \
       const unsigned long long t_ = wall_clock64();              \
-      A.prof[k] += t_ - prof_t;                                  \
+      atomicAdd(&A.prof[k], t_ - prof_t);                        \
       prof_t = t_;                                               \
     }                                                            \
   } while (0)
-  if (prof_on) A.prof[15] += 1;
+  if (prof_on) atomicAdd(&A.prof[15], 1ull);
   // (the loop's barrier after the previous iteration has retired every
   // reader of these; their first use in this one follows the next barrier)
   if (tid < 8) { sh_int[tid] = 0; sh_d[tid] = 0.0; }
@@ -1345,9 +1347,13 @@ __device__ void batch_iteration(const BatchArgs& A, int inst, const InstDesc& I,
 // KLDS: the MOSFET class table is staged in LDS (at most 16 entries); TLDS:
 // the term array lives in LDS.  Both are template constants so that every
 // access compiles to its own memory's instructions (LDS reads wait only on
-// LDS, not on the term stores in flight).
+// LDS, not on the term stores in flight).  Registers are bounded for 3
+// wavefronts per SIMD (168 VGPRs, about 34 spilled): each instance's iteration
+// is a chain of dependent LDS and HBM round trips, and a third wavefront hides
+// more of them than the spills cost (the 4096-ring batch runs about 5 %
+// faster than at 2 wavefronts per SIMD with 202 VGPRs and no spills).
 template <bool FUSED, bool KLDS, bool TLDS>
-__global__ void __launch_bounds__(512) k_batch_step(BatchArgs A) {
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(3))) k_batch_step(BatchArgs A) {
   if ((int)blockIdx.x >= A.n_inst) return;
   const int inst = A.order[blockIdx.x];
   const InstDesc I = A.inst[inst];
