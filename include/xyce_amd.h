@@ -300,8 +300,11 @@ int xg_batch_points(xg_batch* b, int inst, int max_points, double* t, double* x)
  * group's term arrays
  * (the device contributions of one iteration) kept in LDS too where they fit;
  * -1 = keep.  concurrent_groups 1: the
- * instances' LDS-sized groups run side by side on streams of their own, 0:
- * one after another, -1 = keep. */
+ * instances' LDS-sized groups run side by side on streams of their own, each
+ * group its launches of one xg_batch_advance (of one captured graph) back to
+ * back, the groups joined at the end (not after every launch, unless the
+ * split form or the learned predictor needs all groups between launches);
+ * 0: one after another, -1 = keep.  The trajectories are the same. */
 int xg_batch_config(xg_batch* b, int block, int iters_per_launch, int split_eval, int concurrent_groups);
 /* Time-integration method of every instance: 0 = trapezoid (OneStep, the
  * default), 1 = Gear12 (variable-step BDF1/BDF2), as .OPTIONS TIMEINT

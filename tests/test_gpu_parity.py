@@ -372,19 +372,24 @@ def test_workgroup_size_does_not_change_results(gpu):
 def test_graph_replays_match_eager_launches(gpu):
     """Launches captured in a hipGraph and replayed (with counter reads and
     synchronisation between replays) give the same trajectory as eager
-    launches."""
-    cs = [C.ring(s, seed=3 + s, mc_seed=3 + s) for s in (13, 29)]
-    res = []
-    for chunk in (0, 16):
-        b = X.Batch(cs, 0.3e-9, 1e-12, 1e-11, record_max=2000)
-        for _ in range(40):
-            b.advance(16, graph_chunk=chunk)
-            b.counters()
-        n0 = X.host_pattern(cs[0])["n"]; n1 = X.host_pattern(cs[1])["n"]
-        res.append((b.points(0, n0), b.points(1, n1), b.counters()["newton_iters"]))
-    assert res[0][2] == res[1][2]
-    for a, c in zip(res[0][:2], res[1][:2]):
-        assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
+    launches, also with the LDS-sized groups side by side, where each group
+    runs its launches of a graph or an advance back to back on its own
+    stream."""
+    cs = [C.ring(s, seed=3 + s, mc_seed=3 + s) for s in (13, 29, 9, 21)]
+    ns = [X.host_pattern(c)["n"] for c in cs]
+    for split, k in ((1, 1), (0, 2)):
+        res = []
+        for chunk, conc in ((0, 0), (16, 0), (16, 1), (0, 1)):
+            b = X.Batch(cs, 0.3e-9, 1e-12, 1e-11, record_max=2000)
+            b.config(split_eval=split, iters_per_launch=k, concurrent_groups=conc)
+            for _ in range(40):
+                b.advance(16, graph_chunk=chunk)
+                b.counters()
+            res.append((tuple(b.points(i, ns[i]) for i in range(len(cs))), b.counters()["newton_iters"]))
+        for other in res[1:]:
+            assert other[1] == res[0][1], (split, other[1], res[0][1])
+            for a, c in zip(res[0][0], other[0]):
+                assert np.array_equal(a[0], c[0]) and np.array_equal(a[1], c[1])
 
 
 def test_batch_power_grids_match_oracle(gpu):

@@ -410,7 +410,8 @@ class Batch:
     def config(self, block=0, iters_per_launch=0, split_eval=-1, concurrent_groups=-1):
         """block: workgroup size (0 keep); split_eval 1: separate evaluation kernel, one iteration per
         launch; 0: evaluation fused, iters_per_launch iterations per launch; concurrent_groups 1: the
-        LDS-sized instance groups side by side on their own streams; -1 keep."""
+        LDS-sized instance groups side by side on their own streams, each running the launches of one
+        advance (or captured graph) back to back; -1 keep."""
         _check(lib().xg_batch_config(self.h, C.c_int(block), C.c_int(iters_per_launch), C.c_int(split_eval),
                                      C.c_int(concurrent_groups)), "xg_batch_config")
 

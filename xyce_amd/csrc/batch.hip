@@ -1663,31 +1663,38 @@ struct xg_batch {
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     graph_len = 0;
   }
+  // one dispatch of launch group j on stream s
+  void launch_bucket(size_t j, hipStream_t s) {
+    const Bucket& bk = buckets[j];
+    BatchArgs a = args;
+    a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
+    const bool lt = !split && bk.terms;
+    a.lds_terms = lt ? 1 : 0; a.lds_blob = bk.blob;
+    const size_t shm = lt ? bk.shmem_terms : bk.shmem;
+    const int bs = bk.block ? bk.block : block;
+    if (split) hipLaunchKernelGGL((k_batch_step<false, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
+    else if (a.n_kcls > kMaxKclsLds) hipLaunchKernelGGL((k_batch_step<true, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
+    else if (lt) hipLaunchKernelGGL((k_batch_step<true, true, true>), dim3(bk.count), dim3(bs), shm, s, a);
+    else hipLaunchKernelGGL((k_batch_step<true, true, false>), dim3(bk.count), dim3(bs), shm, s, a);
+  }
+  bool side_by_side() const {
+    return concurrent_groups && only_bucket < 0 && buckets.size() > 1 && buckets.size() <= 4;
+  }
   int launch() {
     if (split) {
       if (args.nt_mos > 0)
         hipLaunchKernelGGL(k_batch_eval, dim3((args.nt_mos + 255) / 256), dim3(256), 0, stream, args);
     }
-    const bool side = concurrent_groups && only_bucket < 0 && buckets.size() > 1 && buckets.size() <= 4;
+    const bool side = side_by_side();
     if (side && hipEventRecord(fork_ev, stream) != hipSuccess) return XG_E_HIP;
     for (size_t j = 0; j < buckets.size(); ++j) {
       if (only_bucket >= 0 && (int)j != only_bucket) continue;
-      const Bucket& bk = buckets[j];
-      BatchArgs a = args;
-      a.n_inst = bk.count; a.order = d_order.p + bk.start; a.lds_n = bk.n; a.lds_nlu = bk.nlu;
-      const bool lt = !split && bk.terms;
-      a.lds_terms = lt ? 1 : 0; a.lds_blob = bk.blob;
-      const size_t shm = lt ? bk.shmem_terms : bk.shmem;
-      const int bs = bk.block ? bk.block : block;
       hipStream_t s = stream;
       if (side) {
         s = gstream[j];
         if (hipStreamWaitEvent(s, fork_ev, 0) != hipSuccess) return XG_E_HIP;
       }
-      if (split) hipLaunchKernelGGL((k_batch_step<false, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
-      else if (a.n_kcls > kMaxKclsLds) hipLaunchKernelGGL((k_batch_step<true, false, false>), dim3(bk.count), dim3(bs), bk.shmem, s, a);
-      else if (lt) hipLaunchKernelGGL((k_batch_step<true, true, true>), dim3(bk.count), dim3(bs), shm, s, a);
-      else hipLaunchKernelGGL((k_batch_step<true, true, false>), dim3(bk.count), dim3(bs), shm, s, a);
+      launch_bucket(j, s);
       if (side) {
         if (hipEventRecord(join_ev[j], s) != hipSuccess) return XG_E_HIP;
         if (hipStreamWaitEvent(stream, join_ev[j], 0) != hipSuccess) return XG_E_HIP;
@@ -1697,6 +1704,29 @@ struct xg_batch {
       const int st = mlp_forward_on_stream(net, pred_rows, d_feat.p, d_pred_out.p, stream);
       if (st != XG_OK) return st;
       hipLaunchKernelGGL(k_pred_apply, dim3(n_inst), dim3(256), 0, stream, args);
+    }
+    return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
+  }
+  // k launches.  With the groups side by side and nothing between launches
+  // that spans the groups (no split evaluation, no learned predictor), each
+  // group runs its k launches back to back on its own stream between one fork
+  // and one join: the groups hold disjoint instances, so a group that finishes
+  // a launch early starts its next one instead of waiting for the slowest
+  // group at every launch.
+  int launch_chunk(int k) {
+    if (!side_by_side() || split || pred_mode == 2) {
+      for (int i = 0; i < k; ++i) {
+        const int st = launch();
+        if (st != XG_OK) return st;
+      }
+      return XG_OK;
+    }
+    if (hipEventRecord(fork_ev, stream) != hipSuccess) return XG_E_HIP;
+    for (size_t j = 0; j < buckets.size(); ++j) {
+      if (hipStreamWaitEvent(gstream[j], fork_ev, 0) != hipSuccess) return XG_E_HIP;
+      for (int i = 0; i < k; ++i) launch_bucket(j, gstream[j]);
+      if (hipEventRecord(join_ev[j], gstream[j]) != hipSuccess) return XG_E_HIP;
+      if (hipStreamWaitEvent(stream, join_ev[j], 0) != hipSuccess) return XG_E_HIP;
     }
     return hipGetLastError() == hipSuccess ? XG_OK : XG_E_HIP;
   }
@@ -2162,14 +2192,12 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
     if (!b->gexec || b->graph_len != graph_chunk) {
       b->drop_graph();
       HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < graph_chunk; ++i) {
-        const int st = b->launch();
-        if (st != XG_OK) {
-          hipGraph_t g = nullptr;
-          (void)hipStreamEndCapture(b->stream, &g);
-          if (g) (void)hipGraphDestroy(g);
-          return st;
-        }
+      const int st = b->launch_chunk(graph_chunk);
+      if (st != XG_OK) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return st;
       }
       HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
       HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
@@ -2180,8 +2208,8 @@ int xg_batch_advance(xg_batch* b, int launches, int graph_chunk) {
       left -= graph_chunk;
     }
   }
-  for (; left > 0; --left) {
-    const int st = b->launch();
+  if (left > 0) {
+    const int st = b->launch_chunk(left);
     if (st != XG_OK) return st;
   }
   HIPCHK(hipGetLastError());
